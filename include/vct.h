@@ -1,0 +1,335 @@
+/*
+ * vct.h - C ABI of the MI355X voxel cone tracing hot path.
+ *
+ * One shared library (libvct.so, HIP for gfx950) exports the per-pass entry
+ * points that the reference records as Vulkan command-buffer work. Each entry
+ * point replaces one reference pass helper; the citation beside it names the
+ * reference interface (path:line, relative to the reference repository).
+ *
+ * Conventions
+ *  - Every call returns vct_status; nothing aborts.
+ *  - Calls that take a vct_stream enqueue on that HIP stream (hipStream_t,
+ *    NULL = legacy default stream) and are ordered only by that stream.
+ *  - A vct_ctx owns the voxel volumes, the SVO pool and all scratch memory.
+ *    The caller owns meshes, G-buffers, shadow maps and output images
+ *    (device pointers). A vct_ctx is not thread-safe.
+ *  - Descriptor structs are byte-compatible with the reference's std140 /
+ *    push-constant structs (sizes are checked with static asserts in the
+ *    library).
+ */
+#ifndef VCT_H
+#define VCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VCT_ABI_VERSION 1u
+
+typedef struct vct_ctx vct_ctx;
+typedef void* vct_stream; /* hipStream_t */
+
+typedef enum vct_status {
+    VCT_OK = 0,
+    VCT_ERR_INVALID_ARGUMENT = 1,
+    VCT_ERR_OUT_OF_MEMORY = 2,
+    VCT_ERR_HIP = 3,
+    VCT_ERR_CAPACITY = 4,     /* a fixed-size pool would overflow */
+    VCT_ERR_UNSUPPORTED = 5,  /* e.g. textured materials */
+    VCT_ERR_NOT_READY = 6     /* a prerequisite pass has not run */
+} vct_status;
+
+typedef enum vct_volume {
+    VCT_VOLUME_OPACITY = 0,  /* "VoxelOpacity"  (Voxelizer.cpp:262-264) */
+    VCT_VOLUME_RADIANCE = 1  /* "VoxelRadiance" (Voxelizer.cpp:268-270) */
+} vct_volume;
+
+typedef enum vct_format {
+    VCT_FORMAT_RGBA8 = 0,   /* VK_FORMAT_R8G8B8A8_UNORM, the reference's only format */
+    VCT_FORMAT_RGBA16F = 1  /* half-float radiance (deterministic FP32 accumulate) */
+} vct_format;
+
+/* Reference behaviours that are defects or driver-defined. Each bit selects
+ * the reference's behaviour; a clear bit selects the evident intent. */
+enum {
+    VCT_QUIRK_CLAMP_X_RANGE = 1u << 0,      /* msaaVoxelizer.frag:46 float(vec3) clamp */
+    VCT_QUIRK_BORDER_SKIP_FAR = 1u << 1,    /* BorderWrapper.cpp:139-140 writes 0..127 only */
+    VCT_QUIRK_SHADOW_RAW_DEPTH = 1u << 2,   /* shadow.glsl:8-36 averages raw depth */
+    VCT_QUIRK_SVO_HALF_COORDS = 1u << 3,    /* octreeNodeFlag.comp:28-32 fragPos >> 1 */
+    VCT_QUIRK_SVO_MIP_LEVEL_TEST = 1u << 4, /* octreeNodeMipmapWrite.comp:50-52 */
+    VCT_QUIRK_SVO_UNTEXTURED_BLACK = 1u << 5, /* voxelizer.frag:72-77 */
+    VCT_QUIRK_ALL = 0x3fu
+};
+
+/* Rendering modes, VoxelConeTracingPass.h:17-28. */
+enum {
+    VCT_MODE_DIFFUSE = 0,
+    VCT_MODE_SPECULAR = 1,
+    VCT_MODE_NORMAL = 2,
+    VCT_MODE_MIN_LEVEL = 3,
+    VCT_MODE_DIRECT = 4,
+    VCT_MODE_INDIRECT_DIFFUSE = 5,
+    VCT_MODE_INDIRECT_SPECULAR = 6,
+    VCT_MODE_AMBIENT_OCCLUSION = 7,
+    VCT_MODE_COMBINED_GI = 8
+};
+
+typedef struct vct_config {
+    uint32_t voxel_resolution;       /* DEFAULT_VOXEL_RESOLUTION, EngineConfig.h:28 (power of two, 8..256) */
+    uint32_t clip_levels;            /* DEFAULT_CLIP_REGION_COUNT, EngineConfig.h:26 (1..8) */
+    uint32_t radiance_format;        /* vct_format */
+    uint32_t quirks;                 /* VCT_QUIRK_* bits */
+    uint32_t raster_samples;         /* samples per voxel cell for triangle coverage: 1 or 4 */
+    uint32_t downsample_region_size; /* DEFAULT_DOWNSAMPLE_REGION_SIZE, EngineConfig.h:29 */
+    float extent_l0;                 /* DEFAULT_VOXEL_EXTENT_L0, EngineConfig.h:30 */
+    uint32_t reserved;
+} vct_config;
+
+/* ClipmapRegion.h:10-19 */
+typedef struct vct_clipmap_region {
+    int32_t min_corner[3];
+    uint32_t extent[3];
+    float voxel_size;
+} vct_clipmap_region;
+
+/* Voxelizer.h:69-78 (std140 UBO) */
+typedef struct vct_voxelization_desc {
+    float region_min_corner[3];
+    uint32_t clip_level;
+    float region_max_corner[3];
+    float clip_max_extent;
+    float prev_region_min_corner[3];
+    float voxel_size;
+    float prev_region_max_corner[3];
+    int32_t clipmap_resolution;
+} vct_voxelization_desc;
+
+/* DownSampler.h:49-55 */
+typedef struct vct_downsample_desc {
+    int32_t prev_region_min_corner[3];
+    int32_t clip_level;
+    int32_t clipmap_resolution;
+    int32_t downsample_region_size;
+} vct_downsample_desc;
+
+/* ClipmapCleaner.h:40-47 (push constant) */
+typedef struct vct_image_cleaning_desc {
+    int32_t region_min_corner[3];
+    int32_t clip_level;
+    uint32_t clip_max_extent[3];
+    int32_t clipmap_resolution;
+    uint32_t face_count;
+} vct_image_cleaning_desc;
+
+/* BorderWrapper.h BorderWrappingDesc, values at BorderWrapper.cpp:69-73 */
+typedef struct vct_border_wrapping_desc {
+    uint32_t clipmap_resolution;
+    uint32_t clip_border_width;
+    uint32_t face_count;
+    uint32_t clip_region_count;
+} vct_border_wrapping_desc;
+
+/* CopyAlpha.h CopyAlphaDesc, values at CopyAlpha.cpp:98-103 */
+typedef struct vct_copy_alpha_desc {
+    int32_t clip_level;
+    int32_t clipmap_resolution;
+    int32_t face_count;
+} vct_copy_alpha_desc;
+
+/* VoxelConeTracingPass.h:46-59 (push constant) */
+typedef struct vct_cone_tracing_desc {
+    float volume_center[3];
+    uint32_t rendering_mode;
+    float voxel_size;
+    float volume_dimension;
+    float trace_start_offset;
+    float indirect_diffuse_intensity;
+    float ambient_occlusion_factor;
+    float min_trace_step_factor;
+    float indirect_specular_intensity;
+    float occlusion_decay;
+    int32_t enable_32_cones;
+} vct_cone_tracing_desc;
+
+/* light.glsl:8-13 */
+typedef struct vct_directional_light_desc {
+    float direction[3];
+    float intensity;
+    float color[3];
+    int32_t padding;
+} vct_directional_light_desc;
+
+/* light.glsl:15-20 (std140: 136 bytes of members, 144 with tail padding) */
+typedef struct vct_directional_light_shadow_desc {
+    float view[16]; /* column-major, glm layout */
+    float proj[16];
+    float z_near;
+    float z_far;
+    float padding[2];
+} vct_directional_light_shadow_desc;
+
+/* Camera.h:37-42 CameraUBO */
+typedef struct vct_camera_ubo {
+    float view_proj[16];
+    float view_proj_inv[16];
+    float eye_pos[3];
+    int32_t padding;
+} vct_camera_ubo;
+
+/* gltf.glsl:8-26 GltfShadeMaterial (std430, 80 bytes) */
+typedef struct vct_gltf_material {
+    float base_color_factor[4];
+    int32_t base_color_texture;
+    float metallic_factor;
+    float roughness_factor;
+    int32_t metallic_roughness_texture;
+    int32_t emissive_texture;
+    int32_t alpha_mode;
+    float alpha_cutoff;
+    int32_t double_sided;
+    float emissive_factor[3];
+    int32_t normal_texture;
+    float normal_texture_scale;
+    int32_t occlusion_texture;
+    float occlusion_texture_strength;
+    int32_t padding;
+} vct_gltf_material;
+
+/* One drawIndexed call of GLTFScene::cmdDraw (GLTFScene.cpp:457-489) with its
+ * push constants {instanceIndex, materialIndex} (msaaVoxelizer.vert:25-29). */
+typedef struct vct_primitive {
+    uint32_t first_index;
+    uint32_t index_count;
+    int32_t vertex_offset;
+    uint32_t instance_index;
+    uint32_t material_index;
+} vct_primitive;
+
+/* The triangle stream of SceneManager (SceneManager.cpp:124-204): device
+ * vertex/index/matrix/material buffers plus the host list of draws. */
+typedef struct vct_mesh {
+    const float* positions;     /* device, 3 floats per vertex */
+    const float* normals;       /* device, 3 floats per vertex */
+    const float* texcoords;     /* device, 2 floats per vertex, may be NULL */
+    const uint32_t* indices;    /* device */
+    const float* node_matrices; /* device, 32 floats per instance: model, itModel (NodeMatrix) */
+    const vct_gltf_material* materials; /* device */
+    uint32_t vertex_count;
+    uint32_t index_count;
+    uint32_t instance_count;
+    uint32_t material_count;
+    const vct_primitive* primitives; /* host */
+    uint32_t primitive_count;
+    float scene_bb_min[3]; /* SceneManager::getSceneBoundingBox, used by the SVO voxelizer */
+    float scene_bb_max[3];
+} vct_mesh;
+
+/* A D32 shadow map (DirectionalLight::createShadowMap, DirectionalLight.cpp:59-73). */
+typedef struct vct_shadow_map {
+    const float* depth; /* device, width*height floats, row-major */
+    uint32_t width;
+    uint32_t height;
+} vct_shadow_map;
+
+/* G-buffer targets read by the trace (GBufferPass.cpp:177-194). */
+typedef struct vct_gbuffer {
+    const uint32_t* diffuse;  /* RGBA8: rgb base colour, a perceptual roughness */
+    const uint16_t* normal;   /* RGBA16F: rgb = n * 0.5 + 0.5 */
+    const uint32_t* specular; /* RGBA8: rgb specular colour, a metallic */
+    const uint16_t* emission; /* RGBA16F */
+    const float* depth;       /* D32, 1.0 = background */
+    uint32_t width;
+    uint32_t height;
+} vct_gbuffer;
+
+/* ------------------------------------------------------------------------ */
+
+/* Library version string. */
+const char* vct_version(void);
+/* Human-readable name of a status code. */
+const char* vct_status_string(vct_status status);
+/* Last error detail recorded by the calling thread (empty if none). */
+const char* vct_last_error(void);
+
+/* Voxelizer::initializeVoxelizer + createVoxelClipmap (Voxelizer.cpp:143-153,
+ * 249-299): allocates both atlases and scratch. */
+vct_status vct_create(vct_ctx** out_ctx, const vct_config* config);
+vct_status vct_destroy(vct_ctx* ctx);
+
+/* The "ClipmapRegions" resource (RenderPassManager put/get,
+ * VoxelizationPass.cpp:335-357). */
+vct_status vct_set_clip_regions(vct_ctx* ctx, const vct_clipmap_region* regions, uint32_t count);
+vct_status vct_get_clip_regions(const vct_ctx* ctx, vct_clipmap_region* regions, uint32_t count);
+/* Application::updateClipRegionBoundingBox (Application.cpp:116-128) followed by
+ * VoxelizationPass::calculateChangeDelta (VoxelizationPass.cpp:438-449) from the
+ * initial regions of createVoxelClipmap. Writes the regions into ctx. */
+vct_status vct_update_clip_regions(vct_ctx* ctx, const float camera_pos[3]);
+/* Host half of Voxelizer::cmdVoxelize (Voxelizer.cpp:225-262). */
+vct_status vct_make_voxelization_desc(const vct_ctx* ctx, uint32_t level,
+                                      const vct_clipmap_region* region,
+                                      vct_voxelization_desc* out_desc);
+
+/* vkCmdClearColorImage of a whole atlas (VoxelizationPass.cpp:104-125). */
+vct_status vct_clear_volume(vct_ctx* ctx, vct_stream stream, vct_volume volume);
+
+/* K1: opacity voxelization of one clip level (VoxelizationPass::onUpdate,
+ * VoxelizationPass.cpp:190-212; Voxelizer::cmdVoxelize, Voxelizer.h:36). */
+vct_status vct_voxelize_opacity(vct_ctx* ctx, vct_stream stream, const vct_mesh* mesh,
+                                const vct_voxelization_desc* desc, uint32_t level);
+
+/* K2: radiance injection of one clip level (RadianceInjectionPass::onUpdate,
+ * RadianceInjectionPass.cpp:134-159). */
+vct_status vct_inject_radiance(vct_ctx* ctx, vct_stream stream, const vct_mesh* mesh,
+                               const vct_shadow_map* shadow,
+                               const vct_directional_light_desc* light,
+                               const vct_directional_light_shadow_desc* light_shadow,
+                               const vct_voxelization_desc* desc, uint32_t level);
+
+/* K3: ClipmapCleaner::cmdClearOpacityClipRegion / cmdClearRadianceClipRegion
+ * (ClipmapCleaner.h:27-37). */
+vct_status vct_clear_region(vct_ctx* ctx, vct_stream stream, vct_volume volume,
+                            const vct_image_cleaning_desc* desc);
+
+/* K4/K5: DownSampler::cmdDownSampleOpacity / cmdDownSampleRadiance
+ * (DownSampler.h:29-40). */
+vct_status vct_downsample(vct_ctx* ctx, vct_stream stream, vct_volume volume,
+                          const vct_downsample_desc* desc);
+
+/* K6: CopyAlpha::cmdImageCopyAlpha (CopyAlpha.h:26): radiance.a <- opacity.a. */
+vct_status vct_copy_alpha(vct_ctx* ctx, vct_stream stream, const vct_copy_alpha_desc* desc);
+
+/* K7: BorderWrapper::cmdWrappingOpacityBorder / cmdWrappingRadianceBorder
+ * (BorderWrapper.h:28-36). */
+vct_status vct_wrap_border(vct_ctx* ctx, vct_stream stream, vct_volume volume,
+                           const vct_border_wrapping_desc* desc);
+
+/* K8: VoxelConeTracingPass::onUpdate (VoxelConeTracingPass.cpp:75-106).
+ * diffuse_out / specular_out: device RGBA32F images (width*height*4 floats).
+ * samples_out: optional device counter, incremented by the executed cone
+ * samples. */
+vct_status vct_trace(vct_ctx* ctx, vct_stream stream, const vct_gbuffer* gbuffer,
+                     const vct_camera_ubo* camera, const vct_cone_tracing_desc* desc,
+                     const vct_shadow_map* shadow,
+                     const vct_directional_light_desc* light,
+                     const vct_directional_light_shadow_desc* light_shadow,
+                     float* diffuse_out, float* specular_out,
+                     unsigned long long* samples_out);
+
+/* Volume access in the reference atlas layout: a 3-D image of
+ * ((res+2)*6) x ((res+2)*levels) x (res+2) texels, x fastest
+ * (Voxelizer.h:40-52). RGBA8 texels are 4 bytes, RGBA16F 8 bytes.
+ * dst/src are HOST pointers; the call synchronises the stream. */
+vct_status vct_volume_bytes(const vct_ctx* ctx, vct_volume volume, size_t* out_bytes);
+vct_status vct_read_volume(vct_ctx* ctx, vct_stream stream, vct_volume volume,
+                           void* dst, size_t dst_bytes);
+vct_status vct_write_volume(vct_ctx* ctx, vct_stream stream, vct_volume volume,
+                            const void* src, size_t src_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VCT_H */

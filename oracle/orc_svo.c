@@ -1,0 +1,209 @@
+/*
+ * orc_svo.c - sparse voxel octree path: fragment-list voxelization
+ * (voxelizer.vert/geom/frag), the per-level node init/flag/alloc/modifyArg
+ * loop (octreeNode*.comp driven by OctreeBuilder.cpp:213-305) and the leaf /
+ * mipmap colour writes (octreeNodeLeafWrite.comp, octreeNodeMipmapWrite.comp).
+ * Test infrastructure only (see vct_oracle.h).
+ *
+ * Serialisation: fragments take list slots in rasterization order
+ * (triangle, cell row, cell column, sample); the node allocator hands out
+ * child blocks to flagged nodes in increasing pool index, which is the
+ * outcome of the reference's atomicAdd when invocations run in index order.
+ */
+#include "vct_oracle.h"
+#include "orc_internal.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    const orc_tris* tris;
+    const orc_material* mats;
+    const orc_shadow* sm;
+    const orc_light* L;
+    const float* npos; /* normalised positions, 9 floats per triangle */
+    int res;
+    uint32_t quirks;
+    uint32_t* frags;
+    uint32_t cap;
+    uint32_t counter;
+    float light_dir[3];
+} svo_ctx;
+
+static void svo_frag(void* user, uint32_t tri, const float pos[3], const double lam[3])
+{
+    svo_ctx* c = (svo_ctx*)user;
+    /* fs_in.position = biasAndScale(p) (voxelizer.geom:27-30, 51) */
+    float q[3];
+    for (int k = 0; k < 3; ++k) q[k] = (pos[k] + 1.0f) * 0.5f;
+    uint32_t cur = c->counter++; /* atomicAdd before any discard (voxelizer.frag:55) */
+    if (cur >= c->cap) return;
+    const orc_material* m = &c->mats[c->tris->mat[tri]];
+    float rad[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    if (m->emissive[0] > 0.0f || m->emissive[1] > 0.0f || m->emissive[2] > 0.0f) {
+        for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(m->emissive[k], 0.0f), 1.0f);
+    } else {
+        float color[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+        if (!(c->quirks & ORC_QUIRK_SVO_UNTEXTURED_BLACK))
+            for (int k = 0; k < 4; ++k) color[k] = m->base_color[k];
+        const float* n0 = c->tris->nrm + 9u * tri;
+        float ni[3];
+        for (int k = 0; k < 3; ++k)
+            ni[k] = (float)(lam[0] * (double)n0[k] + lam[1] * (double)n0[3 + k] + lam[2] * (double)n0[6 + k]);
+        float len = sqrtf(ni[0] * ni[0] + ni[1] * ni[1] + ni[2] * ni[2]);
+        float n[3] = {ni[0] / len, ni[1] / len, ni[2] / len};
+        float ndl = n[0] * c->light_dir[0] + n[1] * c->light_dir[1] + n[2] * c->light_dir[2];
+        ndl = fminf(fmaxf(ndl, 0.001f), 1.0f);
+        /* calcVisibility at fs_in.position, the normalised [0,1] position (voxelizer.frag:86) */
+        float vis = orc_visibility(c->sm, c->L, q, c->quirks);
+        float lc[3];
+        for (int k = 0; k < 3; ++k) lc[k] = ndl * vis * c->L->color[k] * c->L->intensity;
+        if (lc[0] == 0.0f && lc[1] == 0.0f && lc[2] == 0.0f) return; /* discard: the slot stays zero */
+        for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(lc[k] * color[k] * color[3], 0.0f), 1.0f);
+        rad[3] = 1.0f;
+    }
+    uint32_t vp[3];
+    for (int k = 0; k < 3; ++k) {
+        float f = q[k] * (float)c->res;
+        uint32_t u = f > 0.0f ? (uint32_t)f : 0u;
+        if (u > (uint32_t)c->res) u = (uint32_t)c->res;
+        vp[k] = u;
+    }
+    float v255[4] = {rad[0] * 255.0f, rad[1] * 255.0f, rad[2] * 255.0f, rad[3] * 255.0f};
+    c->frags[2u * cur + 0] = vp[0] | (vp[1] << 12u) | ((vp[2] & 0xffu) << 24u);
+    c->frags[2u * cur + 1] = ((vp[2] >> 8u) << 28u) | (orc_pack_rgba8(v255) & 0x0fffffffu);
+}
+
+uint32_t orc_svo_fragments(const orc_tris* tris, const orc_material* mats, const orc_shadow* sm,
+                           const orc_light* light, const float bb_min[3], const float bb_max[3], int resolution,
+                           int samples, uint32_t quirks, uint32_t* frags, uint32_t cap)
+{
+    /* voxelizer.vert:42-47: normalise by the scene box's largest half extent */
+    float ext[3], center[3];
+    for (int k = 0; k < 3; ++k) {
+        ext[k] = bb_max[k] - bb_min[k];
+        center[k] = (bb_min[k] + bb_max[k]) * 0.5f;
+    }
+    float ev = fmaxf(ext[0], fmaxf(ext[1], ext[2])) * 0.5f;
+    float* np = (float*)malloc(sizeof(float) * 9u * (tris->count ? tris->count : 1u));
+    for (uint32_t i = 0; i < 9u * tris->count; ++i) np[i] = (tris->pos[i] - center[i % 3u]) / ev;
+    orc_tris nt = {np, tris->nrm, tris->mat, tris->count};
+    svo_ctx c;
+    c.tris = &nt;
+    c.mats = mats;
+    c.sm = sm;
+    c.L = light;
+    c.npos = np;
+    c.res = resolution;
+    c.quirks = quirks;
+    c.frags = frags;
+    c.cap = cap;
+    c.counter = 0;
+    float nd[3] = {-light->direction[0], -light->direction[1], -light->direction[2]};
+    float len = sqrtf(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+    for (int k = 0; k < 3; ++k) c.light_dir[k] = nd[k] / len;
+    if (frags && cap) memset(frags, 0, sizeof(uint32_t) * 2u * cap);
+    /* viewport R x R over NDC [-1,1]: raster cells of size 2/R in normalised space */
+    float vs = 2.0f / (float)resolution;
+    int lo[3] = {-resolution / 2, -resolution / 2, -resolution / 2};
+    int hi[3] = {resolution / 2 - 1, resolution / 2 - 1, resolution / 2 - 1};
+    for (uint32_t t = 0; t < tris->count; ++t) orc_raster_triangle(np + 9u * t, vs, lo, hi, samples, svo_frag, &c, t);
+    free(np);
+    return c.counter;
+}
+
+static void frag_pos(const uint32_t* f, uint32_t quirks, uint32_t p[3])
+{
+    p[0] = f[0] & 0xfffu;
+    p[1] = (f[0] >> 12u) & 0xfffu;
+    p[2] = ((f[0] >> 24u) & 0xffu) | ((f[1] >> 20u) & 0xf00u);
+    if (quirks & ORC_QUIRK_SVO_HALF_COORDS)
+        for (int k = 0; k < 3; ++k) p[k] >>= 1;
+}
+
+/* The shared descent of octreeNodeFlag/LeafWrite/MipmapWrite: returns idx,
+ * and the resolution at which it stopped. */
+static uint32_t walk(const uint32_t* pool, uint32_t p[3], uint32_t res, uint32_t stop, uint32_t* res_out)
+{
+    uint32_t idx = 0, cur = 0;
+    do {
+        res >>= 1;
+        uint32_t cx = p[0] >= res, cy = p[1] >= res, cz = p[2] >= res;
+        idx = cur + (cz | (cx << 1) | (cy << 2));
+        cur = pool[2u * idx] & 0x7fffffffu;
+        p[0] -= cx * res;
+        p[1] -= cy * res;
+        p[2] -= cz * res;
+    } while (cur != 0u && res > stop);
+    *res_out = res;
+    return idx;
+}
+
+int orc_svo_build(const uint32_t* frags, uint32_t nfrag, int level, uint32_t quirks, uint32_t* pool,
+                  uint32_t pool_cap, uint32_t* node_count)
+{
+    const uint32_t R = 1u << level;
+    uint32_t alloc_begin = 0, alloc_num = 8, counter = 0;
+    if (pool_cap < 8) return -1;
+    for (int i = 1; i <= level; ++i) {
+        /* octreeNodeInit.comp */
+        if ((uint64_t)alloc_begin + alloc_num > pool_cap) return -1;
+        memset(pool + 2u * alloc_begin, 0, sizeof(uint32_t) * 2u * alloc_num);
+        /* octreeNodeFlag.comp */
+        for (uint32_t f = 0; f < nfrag; ++f) {
+            uint32_t p[3], r;
+            frag_pos(frags + 2u * f, quirks, p);
+            uint32_t idx = walk(pool, p, R, 1u, &r);
+            pool[2u * idx] |= 0x80000000u;
+        }
+        if (i != level) {
+            /* octreeNodeAlloc.comp, serialised in pool-index order */
+            for (uint32_t n = alloc_begin; n < alloc_begin + alloc_num; ++n) {
+                if (pool[2u * n] & 0x80000000u) {
+                    uint32_t cv = counter++;
+                    pool[2u * n] = ((cv + 1u) << 3u) | 0x80000000u;
+                    pool[2u * n + 1] = 0u;
+                }
+            }
+            /* octreeNodeModifyArg.comp */
+            alloc_begin += alloc_num;
+            alloc_num = (counter << 3) + 8u - alloc_begin;
+        }
+    }
+    uint32_t total = (counter << 3) + 8u;
+    /* octreeNodeLeafWrite.comp: CAS running average into the leaf, in fragment order */
+    for (uint32_t f = 0; f < nfrag; ++f) {
+        uint32_t p[3], r;
+        frag_pos(frags + 2u * f, quirks, p);
+        uint32_t idx = walk(pool, p, R, 1u, &r);
+        if (r == 1u) {
+            float v[4];
+            orc_unpack_rgba8(frags[2u * f + 1] & 0x0fffffffu, v);
+            pool[2u * idx + 1] = orc_rgba8_avg(pool[2u * idx + 1], v);
+        }
+    }
+    /* octreeNodeMipmapWrite.comp for i = 2..level (OctreeBuilder.cpp:328-344).
+     * The shader compares the descent resolution with the level index itself
+     * (:50-52); without that quirk the target is the resolution 2^(i-1). */
+    for (int i = 2; i <= level; ++i) {
+        uint32_t target = (quirks & ORC_QUIRK_SVO_MIP_LEVEL_TEST) ? (uint32_t)i : (1u << (i - 1));
+        for (uint32_t f = 0; f < nfrag; ++f) {
+            uint32_t p[3], r;
+            frag_pos(frags + 2u * f, quirks, p);
+            uint32_t idx = walk(pool, p, R, target, &r);
+            if (r == target) {
+                uint32_t child = pool[2u * idx] & 0x7fffffffu;
+                float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (uint32_t j = 0; j < 8; ++j) {
+                    float v[4];
+                    orc_unpack_rgba8(pool[2u * (child + j) + 1], v);
+                    for (int k = 0; k < 4; ++k) sum[k] = sum[k] + v[k] / 8.0f;
+                }
+                pool[2u * idx + 1] = orc_pack_rgba8(sum);
+            }
+        }
+    }
+    if (node_count) *node_count = total;
+    return 0;
+}

@@ -1,0 +1,192 @@
+/*
+ * vct_oracle.h - CPU restatement of the reference voxel-GI shaders.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product library links, loads or
+ * calls this code; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, and only as the checker / CPU baseline.
+ *
+ * Every function restates one GLSL program or helper of the reference from
+ * its text (citations are path:line relative to the reference repository).
+ * Volumes use the reference atlas layout: a 3-D image of
+ * W = (res+2)*6, H = (res+2)*levels, D = res+2 texels with x fastest
+ * (Voxelizer.h:40-52, Voxelizer.cpp:249-270). RGBA8 texels are uint32 with R
+ * in the low byte; RGBA16F texels are 4 x uint16 half floats.
+ *
+ * Arithmetic: FP32 with IEEE basic operations and no contraction
+ * (gcc -ffp-contract=off), so integer/byte results (occupancy, UNORM8 mips,
+ * RGBA8 CAS averages, node pools) are reproducible bit for bit. The cone
+ * trace also has an FP64 twin (orc_trace_f64) used to state error.
+ *
+ * Order-dependent reference operations (imageAtomicCompSwap running averages,
+ * atomicAdd node allocation) are serialised in a fixed order: fragments in
+ * (triangle, cell row, cell column, sample) order, flagged nodes in pool-index
+ * order.
+ */
+#ifndef VCT_ORACLE_H
+#define VCT_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_FMT_RGBA8 = 0, ORC_FMT_RGBA16F = 1 };
+
+enum {
+    ORC_QUIRK_CLAMP_X_RANGE = 1u << 0,
+    ORC_QUIRK_BORDER_SKIP_FAR = 1u << 1,
+    ORC_QUIRK_SHADOW_RAW_DEPTH = 1u << 2,
+    ORC_QUIRK_SVO_HALF_COORDS = 1u << 3,
+    ORC_QUIRK_SVO_MIP_LEVEL_TEST = 1u << 4,
+    ORC_QUIRK_SVO_UNTEXTURED_BLACK = 1u << 5
+};
+
+typedef struct orc_volume {
+    void* texels;   /* uint32_t* (RGBA8) or uint16_t* (RGBA16F, 4 per texel) */
+    int fmt;
+    int res;        /* interior resolution (128) */
+    int levels;     /* clip levels (6) */
+} orc_volume;
+
+/* World-space triangle soup after the vertex stage (msaaVoxelizer.vert:31-37). */
+typedef struct orc_tris {
+    const float* pos;      /* 9 floats per triangle */
+    const float* nrm;      /* 9 floats per triangle (itModel * normal, not normalised) */
+    const uint32_t* mat;   /* material index per triangle */
+    uint32_t count;
+} orc_tris;
+
+typedef struct orc_material {
+    float base_color[4];
+    float emissive[3];
+} orc_material;
+
+typedef struct orc_light {
+    float direction[3];
+    float intensity;
+    float color[3];
+    float view[16];
+    float proj[16];
+    float z_near;
+    float z_far;
+} orc_light;
+
+typedef struct orc_shadow {
+    const float* depth;
+    int width;
+    int height;
+} orc_shadow;
+
+typedef struct orc_vox_desc {
+    int region_cell_min[3];    /* ClipmapRegion.minCorner of the level (voxels) */
+    int region_cell_extent[3]; /* ClipmapRegion.extent */
+    float region_min[3];
+    float region_max[3];
+    float clip_max_extent;
+    float voxel_size;
+    int clip_level;
+    int clipmap_resolution;
+} orc_vox_desc;
+
+typedef struct orc_trace_params {
+    float volume_center[3];
+    uint32_t rendering_mode;
+    float voxel_size;
+    float volume_dimension;
+    float trace_start_offset;
+    float indirect_diffuse_intensity;
+    float ambient_occlusion_factor;
+    float min_trace_step_factor;
+    float indirect_specular_intensity;
+    float occlusion_decay;
+    int enable_32_cones;
+    int clip_levels;
+    float view_proj_inv[16];
+    float eye_pos[3];
+} orc_trace_params;
+
+typedef struct orc_gbuffer {
+    const uint32_t* diffuse;
+    const uint16_t* normal;
+    const uint32_t* specular;
+    const uint16_t* emission;
+    const float* depth;
+    int width;
+    int height;
+} orc_gbuffer;
+
+/* ---- numeric helpers (atomic.glsl:4-22, Vulkan UNORM / half conversions) ---- */
+void orc_unpack_rgba8(uint32_t v, float out[4]);          /* convRGBA8ToVec4 */
+uint32_t orc_pack_rgba8(const float v[4]);                /* convVec4ToRGBA8 */
+uint32_t orc_float_to_unorm8_rgba(const float v[4]);      /* imageStore to RGBA8_UNORM */
+uint16_t orc_float_to_half(float f);
+float orc_half_to_float(uint16_t h);
+uint32_t orc_rgba8_avg(uint32_t stored, const float value[4]); /* one uncontended CAS-loop step */
+
+/* ---- atlas kernels ---- */
+size_t orc_volume_texels(int res, int levels);
+void orc_clear_region(orc_volume* vol, const int min_corner[3], const int extent[3],
+                      int clip_level, int face_count);                 /* clipmapCleaning.comp */
+void orc_copy_alpha(orc_volume* radiance, const orc_volume* opacity, int clip_level,
+                    int face_count);                                   /* copyAlphaImage.comp */
+void orc_wrap_border(orc_volume* vol, int border_width_total, int face_count,
+                     int region_count, int skip_far);                  /* borderWrapping.comp */
+void orc_downsample_opacity(orc_volume* vol, const int prev_min_corner[3], int clip_level,
+                            int region_size);                          /* opacityDownSample.comp */
+void orc_downsample_radiance(orc_volume* vol, const int prev_min_corner[3], int clip_level,
+                             int region_size);                         /* radianceDownSample.comp */
+
+/* ---- triangle voxelization (K1) and radiance injection (K2) ---- */
+int orc_dominant_axis(const float* p);                     /* msaaVoxelizer.geom:27-32 */
+void orc_voxelize_opacity(orc_volume* vol, const orc_tris* tris, const orc_vox_desc* d,
+                          int samples, uint32_t quirks);
+float orc_visibility(const orc_shadow* sm, const orc_light* light, const float pos[3],
+                     uint32_t quirks);                     /* shadow.glsl:28-36 */
+void orc_inject_radiance(orc_volume* vol, const orc_tris* tris, const orc_material* mats,
+                         const orc_shadow* sm, const orc_light* light, const orc_vox_desc* d,
+                         int samples, uint32_t quirks);
+
+/* ---- cone tracing (K8) ---- */
+float orc_calc_min_level(const orc_trace_params* p, const float pos[3]);
+void orc_trace_cone(const orc_volume* rad, const orc_trace_params* p, const float start[3],
+                    const float dir[3], float aperture, float max_dist, float start_level,
+                    float step_factor, float out[4], uint32_t* steps);
+void orc_trace(const orc_volume* rad, const orc_gbuffer* g, const orc_trace_params* p,
+               const orc_shadow* sm, const orc_light* light, uint32_t quirks,
+               int row_begin, int row_end, float* diffuse_out, float* specular_out,
+               uint64_t* samples);
+void orc_trace_f64(const orc_volume* rad, const orc_gbuffer* g, const orc_trace_params* p,
+                   const orc_shadow* sm, const orc_light* light, uint32_t quirks,
+                   int row_begin, int row_end, double* diffuse_out, double* specular_out,
+                   uint64_t* samples);
+void orc_brdf(const float n[3], const float v[3], const float l[3], const float diffuse_color[3],
+              const float specular_color[3], float perceptual_roughness, float metallic,
+              float out[3]);                                /* brdf.glsl:65-78 */
+
+/* ---- G-buffer / shadow producers (ray cast) ---- */
+void orc_raycast_gbuffer(const orc_tris* tris, const orc_material* mats,
+                         const float* roughness_metal, /* per material: perceptual roughness, metallic, spec rgb */
+                         const float view_proj_inv[16], const float view_proj[16],
+                         int width, int height, uint32_t* diffuse, uint16_t* normal,
+                         uint32_t* specular, uint16_t* emission, float* depth);
+void orc_raycast_shadow(const orc_tris* tris, const float view[16], const float proj[16],
+                        int width, int height, float* depth);
+
+/* ---- sparse voxel octree (K9-K16) ---- */
+uint32_t orc_svo_fragments(const orc_tris* tris, const orc_material* mats, const orc_shadow* sm,
+                           const orc_light* light, const float bb_min[3], const float bb_max[3],
+                           int resolution, int samples, uint32_t quirks, uint32_t* frags, uint32_t cap);
+int orc_svo_build(const uint32_t* frags, uint32_t nfrag, int level, uint32_t quirks,
+                  uint32_t* pool, uint32_t pool_cap, uint32_t* node_count);
+void orc_svo_trace(const uint32_t* pool, const orc_gbuffer* g, const orc_trace_params* p,
+                   const float bb_min[3], const float bb_max[3], const orc_shadow* sm,
+                   const orc_light* light, uint32_t quirks, int row_begin, int row_end,
+                   float* diffuse_out, float* specular_out, uint64_t* samples);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

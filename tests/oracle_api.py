@@ -1,0 +1,248 @@
+"""ctypes access to the oracle (oracle/build/liboracle_vct.so) for the tests.
+
+The oracle is the checker: a CPU restatement of the reference shaders. This
+module is only imported by tests, smoke() and bench.py's CPU-baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "build" / "liboracle_vct.so"
+
+FMT_RGBA8 = 0
+FMT_RGBA16F = 1
+
+
+def build_oracle() -> Path:
+    if not ORACLE_LIB.exists():
+        subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    return ORACLE_LIB
+
+
+class Volume(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("fmt", C.c_int), ("res", C.c_int), ("levels", C.c_int)]
+
+
+class Tris(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("nrm", C.c_void_p), ("mat", C.c_void_p), ("count", C.c_uint32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("base_color", C.c_float * 4), ("emissive", C.c_float * 3)]
+
+
+class Light(C.Structure):
+    _fields_ = [
+        ("direction", C.c_float * 3),
+        ("intensity", C.c_float),
+        ("color", C.c_float * 3),
+        ("view", C.c_float * 16),
+        ("proj", C.c_float * 16),
+        ("z_near", C.c_float),
+        ("z_far", C.c_float),
+    ]
+
+
+class Shadow(C.Structure):
+    _fields_ = [("depth", C.c_void_p), ("width", C.c_int), ("height", C.c_int)]
+
+
+class VoxDesc(C.Structure):
+    _fields_ = [
+        ("region_cell_min", C.c_int * 3),
+        ("region_cell_extent", C.c_int * 3),
+        ("region_min", C.c_float * 3),
+        ("region_max", C.c_float * 3),
+        ("clip_max_extent", C.c_float),
+        ("voxel_size", C.c_float),
+        ("clip_level", C.c_int),
+        ("clipmap_resolution", C.c_int),
+    ]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [
+        ("volume_center", C.c_float * 3),
+        ("rendering_mode", C.c_uint32),
+        ("voxel_size", C.c_float),
+        ("volume_dimension", C.c_float),
+        ("trace_start_offset", C.c_float),
+        ("indirect_diffuse_intensity", C.c_float),
+        ("ambient_occlusion_factor", C.c_float),
+        ("min_trace_step_factor", C.c_float),
+        ("indirect_specular_intensity", C.c_float),
+        ("occlusion_decay", C.c_float),
+        ("enable_32_cones", C.c_int),
+        ("clip_levels", C.c_int),
+        ("view_proj_inv", C.c_float * 16),
+        ("eye_pos", C.c_float * 3),
+    ]
+
+
+class GBuf(C.Structure):
+    _fields_ = [
+        ("diffuse", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("specular", C.c_void_p),
+        ("emission", C.c_void_p),
+        ("depth", C.c_void_p),
+        ("width", C.c_int),
+        ("height", C.c_int),
+    ]
+
+
+_L = None
+
+
+def lib() -> C.CDLL:
+    global _L
+    if _L is None:
+        _L = C.CDLL(str(build_oracle()))
+        _L.orc_pack_rgba8.restype = C.c_uint32
+        _L.orc_pack_rgba8.argtypes = [C.POINTER(C.c_float)]
+        _L.orc_unpack_rgba8.argtypes = [C.c_uint32, C.POINTER(C.c_float)]
+        _L.orc_float_to_half.restype = C.c_uint16
+        _L.orc_float_to_half.argtypes = [C.c_float]
+        _L.orc_half_to_float.restype = C.c_float
+        _L.orc_half_to_float.argtypes = [C.c_uint16]
+        _L.orc_rgba8_avg.restype = C.c_uint32
+        _L.orc_rgba8_avg.argtypes = [C.c_uint32, C.POINTER(C.c_float)]
+        _L.orc_float_to_unorm8_rgba.restype = C.c_uint32
+        _L.orc_float_to_unorm8_rgba.argtypes = [C.POINTER(C.c_float)]
+        _L.orc_volume_texels.restype = C.c_size_t
+        _L.orc_dominant_axis.restype = C.c_int
+        _L.orc_visibility.restype = C.c_float
+        _L.orc_calc_min_level.restype = C.c_float
+        _L.orc_svo_fragments.restype = C.c_uint32
+        _L.orc_svo_build.restype = C.c_int
+    return _L
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def vptr(a):
+    return C.c_void_p(0) if a is None else C.c_void_p(a.ctypes.data)
+
+
+# --------------------------------------------------------------------------
+# volumes in the reference atlas layout
+
+def new_volume(res: int, levels: int, fmt: int = FMT_RGBA8) -> np.ndarray:
+    D = res + 2
+    if fmt == FMT_RGBA8:
+        return np.zeros((D, D * levels, D * 6), np.uint32)
+    return np.zeros((D, D * levels, D * 6, 4), np.uint16)
+
+
+def vol_struct(arr: np.ndarray, res: int, levels: int) -> Volume:
+    fmt = FMT_RGBA8 if arr.dtype == np.uint32 else FMT_RGBA16F
+    assert arr.flags.c_contiguous
+    return Volume(arr.ctypes.data, fmt, res, levels)
+
+
+def tris_struct(pos, nrm, mat) -> Tris:
+    return Tris(pos.ctypes.data, nrm.ctypes.data, mat.ctypes.data, len(mat))
+
+
+def materials_struct(scene):
+    arr = (Material * len(scene.materials))()
+    for i, m in enumerate(scene.materials):
+        arr[i].base_color = (C.c_float * 4)(*m["base_color_factor"].tolist())
+        arr[i].emissive = (C.c_float * 3)(*m["emissive_factor"].tolist())
+    return arr
+
+
+def light_struct(light: dict) -> Light:
+    from vk_voxel_cone_tracing_amd import glm
+
+    L = Light()
+    L.direction = (C.c_float * 3)(*np.asarray(light["direction"], np.float32).tolist())
+    L.intensity = float(light["intensity"])
+    L.color = (C.c_float * 3)(*np.asarray(light["color"], np.float32).tolist())
+    L.view = (C.c_float * 16)(*glm.flat(light["view"]).tolist())
+    L.proj = (C.c_float * 16)(*glm.flat(light["proj"]).tolist())
+    L.z_near = float(light["z_near"])
+    L.z_far = float(light["z_far"])
+    return L
+
+
+def vox_desc(region, level: int, res: int, extent_l0: float = 16.0, voxel_size=None) -> VoxDesc:
+    """Restates Voxelizer::cmdVoxelize's host math (Voxelizer.cpp:225-262) for
+    a full-level region (min_corner, extent, voxel_size)."""
+    mn, ext, vs = region
+    vs = np.float32(vs if voxel_size is None else voxel_size)
+    d = VoxDesc()
+    d.region_cell_min = (C.c_int * 3)(*mn)
+    d.region_cell_extent = (C.c_int * 3)(*ext)
+    d.region_min = (C.c_float * 3)(*[float(np.float32(np.float32(m) * vs) - np.float32(1e-6)) for m in mn])
+    d.region_max = (C.c_float * 3)(*[float(np.float32(np.float32(m + e) * vs) + np.float32(1e-6)) for m, e in zip(mn, ext)])
+    d.clip_max_extent = float(np.float32(np.float32(ext[0]) * vs))
+    d.voxel_size = float(vs)
+    d.clip_level = level
+    d.clipmap_resolution = res
+    return d
+
+
+def gbuf_struct(g: dict) -> GBuf:
+    h, w = g["depth"].shape
+    return GBuf(g["diffuse"].ctypes.data, g["normal"].ctypes.data, g["specular"].ctypes.data,
+                g["emission"].ctypes.data, g["depth"].ctypes.data, w, h)
+
+
+def raycast_gbuffer(scene, pos, nrm, mat, vp, vp_inv, width, height):
+    from vk_voxel_cone_tracing_amd import glm
+
+    L = lib()
+    g = dict(
+        diffuse=np.zeros((height, width), np.uint32),
+        normal=np.zeros((height, width, 4), np.uint16),
+        specular=np.zeros((height, width), np.uint32),
+        emission=np.zeros((height, width, 4), np.uint16),
+        depth=np.zeros((height, width), np.float32),
+    )
+    mats = materials_struct(scene)
+    surf = np.ascontiguousarray(scene.surface, np.float32)
+    t = tris_struct(pos, nrm, mat)
+    L.orc_raycast_gbuffer(C.byref(t), mats, fptr(surf), fptr(glm.flat(vp_inv)), fptr(glm.flat(vp)), width, height,
+                          vptr(g["diffuse"]), vptr(g["normal"]), vptr(g["specular"]), vptr(g["emission"]),
+                          vptr(g["depth"]))
+    return g
+
+
+def raycast_shadow(pos, nrm, mat, light, width, height):
+    from vk_voxel_cone_tracing_amd import glm
+
+    depth = np.zeros((height, width), np.float32)
+    t = tris_struct(pos, nrm, mat)
+    lib().orc_raycast_shadow(C.byref(t), fptr(glm.flat(light["view"])), fptr(glm.flat(light["proj"])), width,
+                             height, vptr(depth))
+    return depth
+
+
+def trace_params(desc_fields: dict, vp_inv, eye, levels) -> TraceParams:
+    from vk_voxel_cone_tracing_amd import glm
+
+    p = TraceParams()
+    p.volume_center = (C.c_float * 3)(*desc_fields["volume_center"])
+    p.rendering_mode = desc_fields["rendering_mode"]
+    p.voxel_size = desc_fields["voxel_size"]
+    p.volume_dimension = desc_fields["volume_dimension"]
+    p.trace_start_offset = desc_fields["trace_start_offset"]
+    p.indirect_diffuse_intensity = desc_fields["indirect_diffuse_intensity"]
+    p.ambient_occlusion_factor = desc_fields["ambient_occlusion_factor"]
+    p.min_trace_step_factor = desc_fields["min_trace_step_factor"]
+    p.indirect_specular_intensity = desc_fields["indirect_specular_intensity"]
+    p.occlusion_decay = desc_fields["occlusion_decay"]
+    p.enable_32_cones = desc_fields["enable_32_cones"]
+    p.clip_levels = levels
+    p.view_proj_inv = (C.c_float * 16)(*glm.flat(vp_inv).tolist())
+    p.eye_pos = (C.c_float * 3)(*np.asarray(eye, np.float32).tolist())
+    return p

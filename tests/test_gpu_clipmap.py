@@ -1,0 +1,255 @@
+"""GPU parity of the clipmap passes (K1-K8) against the oracle.
+
+Integer/byte outputs (opacity occupancy, UNORM8 mips, RGBA8 running averages,
+border/clear/copy-alpha results) must be bit-identical; the cone trace must
+match the FP32 oracle to a relative L2 error of 1e-3 per output image, and its
+error against the FP64 oracle is reported.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_api as O
+
+pytestmark = pytest.mark.gpu
+
+RES = 32
+LEVELS = 3
+EXTENT_L0 = 4.0
+SCALE = 4.0
+
+
+def _setup_case(quirks, samples=1, fmt=0, seed=0):
+    import torch
+    import vk_voxel_cone_tracing_amd as V
+    from vk_voxel_cone_tracing_amd import scenes
+
+    scene = scenes.room(seed, scale=SCALE, n_boxes=24, n_spheres=4, emissive_boxes=2)
+    ctx = V.Context(resolution=RES, levels=LEVELS, radiance_format=fmt, quirks=quirks, raster_samples=samples,
+                    downsample_region_size=2, extent_l0=EXTENT_L0)
+    cam = np.full(3, 0.5 * SCALE) + np.array([0.13, -0.07, 0.21])
+    ctx.update_clip_regions(cam)
+    ds = V.DeviceScene(scene)
+    torch.cuda.synchronize()
+    return scene, ctx, ds
+
+
+def _desc_to_oracle(desc, region):
+    d = O.VoxDesc()
+    mn, ext, vs = region
+    d.region_cell_min = (C.c_int * 3)(*mn)
+    d.region_cell_extent = (C.c_int * 3)(*ext)
+    d.region_min = (C.c_float * 3)(*desc.region_min_corner)
+    d.region_max = (C.c_float * 3)(*desc.region_max_corner)
+    d.clip_max_extent = desc.clip_max_extent
+    d.voxel_size = desc.voxel_size
+    d.clip_level = desc.clip_level
+    d.clipmap_resolution = desc.clipmap_resolution
+    return d
+
+
+@pytest.mark.parametrize("quirks,samples", [(0x3F, 1), (0x3F, 4), (0x00, 1)])
+def test_opacity_voxelization_bit_exact(quirks, samples):
+    import vk_voxel_cone_tracing_amd as V
+
+    scene, ctx, ds = _setup_case(quirks, samples)
+    pos, nrm, mat = scene.world_triangles()
+    ctx.clear_volume(V.VOLUME_OPACITY)
+    ref = O.new_volume(RES, LEVELS)
+    L = O.lib()
+    tris = O.tris_struct(pos, nrm, mat)
+    for level in range(LEVELS):
+        desc = ctx.voxelization_desc(level)
+        ctx.voxelize_opacity(ds, desc, level)
+        L.orc_voxelize_opacity(C.byref(O.vol_struct(ref, RES, LEVELS)), C.byref(tris),
+                               C.byref(_desc_to_oracle(desc, ctx.regions[level])), samples, quirks)
+    got = ctx.read_volume(V.VOLUME_OPACITY)
+    occupied = int((ref != 0).sum())
+    print(f"quirks={quirks:#x} samples={samples}: {occupied} occupied face texels")
+    assert occupied > 1000
+    np.testing.assert_array_equal(got, ref)
+
+
+def _random_volume(rng, fmt):
+    D = RES + 2
+    if fmt == 0:
+        bytes_ = rng.integers(0, 256, size=(D, D * LEVELS, D * 6, 4), dtype=np.uint8)
+        # make some texels empty / opaque
+        mask = rng.random((D, D * LEVELS, D * 6)) < 0.3
+        bytes_[mask] = 0
+        return bytes_.view(np.uint32)[..., 0].copy()
+    vals = rng.random((D, D * LEVELS, D * 6, 4)).astype(np.float32)
+    return np.vectorize(lambda x: O.lib().orc_float_to_half(float(x)), otypes=[np.uint16])(vals)
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_downsample_bit_exact(fmt):
+    import vk_voxel_cone_tracing_amd as V
+
+    _, ctx, _ = _setup_case(0x3F, fmt=fmt)
+    rng = np.random.default_rng(7)
+    L = O.lib()
+    for vol_id in (V.VOLUME_OPACITY, V.VOLUME_RADIANCE):
+        this_fmt = fmt if vol_id == V.VOLUME_RADIANCE else 0
+        ref = _random_volume(rng, this_fmt)
+        ctx.write_volume(vol_id, ref)
+        regions = ctx.regions
+        for level in range(1, LEVELS):
+            prev_min = regions[level - 1][0]
+            ctx.downsample(vol_id, level, prev_min, region_size=2)
+            pm = (C.c_int * 3)(*prev_min)
+            vs = O.vol_struct(ref, RES, LEVELS)
+            if vol_id == V.VOLUME_OPACITY:
+                L.orc_downsample_opacity(C.byref(vs), pm, level, 2)
+            else:
+                L.orc_downsample_radiance(C.byref(vs), pm, level, 2)
+        got = ctx.read_volume(vol_id)
+        np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("skip_far", [True, False])
+def test_border_wrap_bit_exact(skip_far):
+    import vk_voxel_cone_tracing_amd as V
+
+    _, ctx, _ = _setup_case(0x3F if skip_far else 0x00)
+    rng = np.random.default_rng(3)
+    ref = _random_volume(rng, 0)
+    ctx.write_volume(V.VOLUME_OPACITY, ref)
+    ctx.wrap_border(V.VOLUME_OPACITY)
+    O.lib().orc_wrap_border(C.byref(O.vol_struct(ref, RES, LEVELS)), 2, 6, LEVELS, 1 if skip_far else 0)
+    np.testing.assert_array_equal(ctx.read_volume(V.VOLUME_OPACITY), ref)
+
+
+def test_clear_region_and_copy_alpha_bit_exact():
+    import vk_voxel_cone_tracing_amd as V
+
+    _, ctx, _ = _setup_case(0x3F)
+    rng = np.random.default_rng(11)
+    opa = _random_volume(rng, 0)
+    rad = _random_volume(rng, 0)
+    ctx.write_volume(V.VOLUME_OPACITY, opa)
+    ctx.write_volume(V.VOLUME_RADIANCE, rad)
+    L = O.lib()
+    # toroidal sub-region clear of level 1 (wraps around the far edge)
+    ctx.clear_region(V.VOLUME_RADIANCE, (25, 3, 30), (12, 5, 7), 1)
+    L.orc_clear_region(C.byref(O.vol_struct(rad, RES, LEVELS)), (C.c_int * 3)(25, 3, 30), (C.c_int * 3)(12, 5, 7), 1, 6)
+    for level in range(LEVELS):
+        ctx.copy_alpha(level)
+        L.orc_copy_alpha(C.byref(O.vol_struct(rad, RES, LEVELS)), C.byref(O.vol_struct(opa, RES, LEVELS)), level, 6)
+    np.testing.assert_array_equal(ctx.read_volume(V.VOLUME_RADIANCE), rad)
+
+
+def _light_and_shadow(scene, pos, nrm, mat, size=256):
+    import torch
+    from vk_voxel_cone_tracing_amd import scenes
+
+    light = scenes.light(0, scale=SCALE)
+    depth = O.raycast_shadow(pos, nrm, mat, light, size, size)
+    return light, depth, torch.from_numpy(depth).cuda()
+
+
+@pytest.mark.parametrize("fmt,quirks", [(0, 0x3F), (1, 0x3F), (0, 0x00)])
+def test_radiance_injection_bit_exact(fmt, quirks):
+    import vk_voxel_cone_tracing_amd as V
+
+    scene, ctx, ds = _setup_case(quirks, fmt=fmt)
+    pos, nrm, mat = scene.world_triangles()
+    light, depth, depth_d = _light_and_shadow(scene, pos, nrm, mat)
+    ld, ls = V.light_structs(light)
+    sm = V.shadow_struct(depth_d)
+    ctx.clear_volume(V.VOLUME_RADIANCE)
+    ref = O.new_volume(RES, LEVELS, fmt)
+    L = O.lib()
+    tris = O.tris_struct(pos, nrm, mat)
+    mats = O.materials_struct(scene)
+    osm = O.Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
+    olight = O.light_struct(light)
+    for level in range(LEVELS):
+        desc = ctx.voxelization_desc(level)
+        ctx.inject_radiance(ds, sm, ld, ls, desc, level)
+        L.orc_inject_radiance(C.byref(O.vol_struct(ref, RES, LEVELS)), C.byref(tris), mats, C.byref(osm),
+                              C.byref(olight), C.byref(_desc_to_oracle(desc, ctx.regions[level])), 1, quirks)
+    got = ctx.read_volume(V.VOLUME_RADIANCE)
+    nz = int((ref.reshape(ref.shape[0], ref.shape[1], ref.shape[2], -1) != 0).any(-1).sum())
+    print(f"fmt={fmt} quirks={quirks:#x}: {nz} injected face texels")
+    assert nz > 500
+    np.testing.assert_array_equal(got, ref)
+
+
+def _pipeline(ctx, ds, scene, sm, ld, ls):
+    """A full clipmap frame on the device: voxelize, mip, wrap, inject, copy alpha, mip."""
+    import vk_voxel_cone_tracing_amd as V
+
+    ctx.clear_volume(V.VOLUME_OPACITY)
+    ctx.clear_volume(V.VOLUME_RADIANCE)
+    regions = ctx.regions
+    for level in range(LEVELS):
+        ctx.voxelize_opacity(ds, ctx.voxelization_desc(level), level)
+    for level in range(1, LEVELS):
+        ctx.downsample(V.VOLUME_OPACITY, level, regions[level - 1][0], region_size=2)
+    ctx.wrap_border(V.VOLUME_OPACITY)
+    for level in range(LEVELS):
+        ctx.inject_radiance(ds, sm, ld, ls, ctx.voxelization_desc(level), level)
+    for level in range(LEVELS):
+        ctx.copy_alpha(level)
+    for level in range(1, LEVELS):
+        ctx.downsample(V.VOLUME_RADIANCE, level, regions[level - 1][0], region_size=2)
+
+
+@pytest.mark.parametrize("fmt,mode,cones32", [(0, 8, 0), (1, 8, 0), (0, 7, 1), (0, 5, 0)])
+def test_trace_matches_oracle(fmt, mode, cones32):
+    import torch
+    import vk_voxel_cone_tracing_amd as V
+    from vk_voxel_cone_tracing_amd import scenes
+
+    scene, ctx, ds = _setup_case(0x3F, fmt=fmt)
+    pos, nrm, mat = scene.world_triangles()
+    light, depth, depth_d = _light_and_shadow(scene, pos, nrm, mat)
+    ld, ls = V.light_structs(light)
+    sm = V.shadow_struct(depth_d)
+    _pipeline(ctx, ds, scene, sm, ld, ls)
+    W, H = 48, 32
+    vp, vp_inv, eye = scenes.camera(0, W, H, scale=SCALE)
+    g = O.raycast_gbuffer(scene, pos, nrm, mat, vp, vp_inv, W, H)
+    gd = {k: torch.from_numpy(v.view(np.int32) if v.dtype == np.uint32 else (v.view(np.int16) if v.dtype == np.uint16 else v)).cuda() for k, v in g.items()}
+    reg0 = ctx.regions[0]
+    vs0 = reg0[2]
+    center = [(reg0[0][k] * vs0) + (reg0[1][k] * vs0) * 0.5 for k in range(3)]
+    fields = dict(volume_center=[float(np.float32(c)) for c in center], rendering_mode=mode, voxel_size=vs0,
+                  volume_dimension=float(RES), trace_start_offset=1.0, indirect_diffuse_intensity=8.0,
+                  ambient_occlusion_factor=0.5, min_trace_step_factor=1.0, indirect_specular_intensity=3.0,
+                  occlusion_decay=2.0, enable_32_cones=cones32)
+    desc = V._abi.ConeTracingDesc(V._abi.f3(*fields["volume_center"]), mode, vs0, float(RES), 1.0, 8.0, 0.5, 1.0, 3.0,
+                                  2.0, cones32)
+    cam = V.camera_ubo(vp, vp_inv, eye)
+    dout = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    sout = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    samples = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ctx.trace(V.gbuffer_struct(gd), cam, desc, sm, ld, ls, dout, sout, samples)
+    torch.cuda.synchronize()
+    # oracle on the same volumes
+    rad = ctx.read_volume(V.VOLUME_RADIANCE)
+    L = O.lib()
+    p = O.trace_params(fields, vp_inv, eye, LEVELS)
+    og = O.gbuf_struct(g)
+    osm = O.Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
+    olight = O.light_struct(light)
+    rd = np.zeros((H, W, 4), np.float32)
+    rs = np.zeros((H, W, 4), np.float32)
+    n_ref = C.c_uint64(0)
+    L.orc_trace(C.byref(O.vol_struct(rad, RES, LEVELS)), C.byref(og), C.byref(p), C.byref(osm), C.byref(olight),
+                0x3F, 0, H, O.vptr(rd), O.vptr(rs), C.byref(n_ref))
+    rd64 = np.zeros((H, W, 4), np.float64)
+    rs64 = np.zeros((H, W, 4), np.float64)
+    L.orc_trace_f64(C.byref(O.vol_struct(rad, RES, LEVELS)), C.byref(og), C.byref(p), C.byref(osm), C.byref(olight),
+                    0x3F, 0, H, O.vptr(rd64), O.vptr(rs64), None)
+    gd_out, gs_out = dout.cpu().numpy(), sout.cpu().numpy()
+    for name, got, ref, ref64 in (("diffuse", gd_out, rd, rd64), ("specular", gs_out, rs, rs64)):
+        err = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-12)
+        err64 = np.linalg.norm(got - ref64) / max(np.linalg.norm(ref64), 1e-12)
+        print(f"fmt={fmt} mode={mode} {name}: rel L2 vs fp32 oracle {err:.2e}, vs fp64 oracle {err64:.2e}, "
+              f"max abs {np.abs(got - ref).max():.2e}")
+        assert err <= 1e-3, (name, err)
+    if mode == 8:
+        assert int(samples.item()) == int(n_ref.value)

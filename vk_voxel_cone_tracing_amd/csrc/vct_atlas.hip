@@ -1,0 +1,315 @@
+// vct_atlas.hip - clipmap maintenance kernels for gfx950: region clear (K3),
+// anisotropic opacity/radiance downsample (K4/K5), copy-alpha (K6) and
+// toroidal border wrap (K7).
+//
+// All of these are HBM-streaming kernels. Threads are laid out x-fastest so
+// that a wavefront touches contiguous rows of one (level, face) block; the
+// 2x2x2 downsample footprint of a thread pair is 16 contiguous bytes per row.
+// Arithmetic is FP32 without contraction, in the reference's operation order,
+// so the UNORM8 results are reproducible bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "vct_internal.hpp"
+#include "vct_texel.hpp"
+
+namespace vct {
+
+template <bool kHalf>
+struct Texel;
+
+template <>
+struct Texel<false> {
+    using T = uint32_t;
+    __device__ static f4 load(const T* v, size_t i) { return unpack_unorm(v[i]); }
+    __device__ static void store(T* v, size_t i, f4 x) { v[i] = pack_unorm(x); }
+};
+
+template <>
+struct Texel<true> {
+    using T = uint2;
+    __device__ static f4 load(const T* v, size_t i) { return unpack_half(v[i]); }
+    __device__ static void store(T* v, size_t i, f4 x) { v[i] = pack_half(x); }
+};
+
+// ---------------------------------------------------------------- K3
+template <bool kHalf>
+__global__ void k_clear_region(typename Texel<kHalf>::T* __restrict__ vol, int D, int res, int level, int3 minc,
+                               uint3 ext, int faces)
+{
+    uint32_t gx = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t gy = blockIdx.y * blockDim.y + threadIdx.y;
+    uint32_t gz = blockIdx.z * blockDim.z + threadIdx.z;
+    if (gx >= ext.x || gy >= ext.y || gz >= ext.z) return;
+    // clipmapCleaning.comp:22-30
+    int x = ((int)gx + minc.x) % res + 1;
+    int y = ((int)gy + minc.y) % res + 1;
+    int z = ((int)gz + minc.z) % res + 1;
+    for (int f = 0; f < faces; ++f) {
+        size_t i = vol_index(D, level, f, x, y, z);
+        if (kHalf) ((uint2*)vol)[i] = make_uint2(0u, 0u);
+        else ((uint32_t*)vol)[i] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------- K6
+template <bool kHalf>
+__global__ void k_copy_alpha(typename Texel<kHalf>::T* __restrict__ rad, const uint32_t* __restrict__ opa, int D,
+                             int res, int level, int faces)
+{
+    uint32_t gx = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t gy = blockIdx.y * blockDim.y + threadIdx.y;
+    uint32_t gz = blockIdx.z * blockDim.z + threadIdx.z;
+    if (gx >= (uint32_t)res || gy >= (uint32_t)res || gz >= (uint32_t)res) return;
+    for (int f = 0; f < faces; ++f) {
+        size_t i = vol_index(D, level, f, (int)gx + 1, (int)gy + 1, (int)gz + 1);
+        uint32_t src = opa[i];
+        if (kHalf) {
+            uint2 t = ((uint2*)rad)[i];
+            f4 v = unpack_half(t);
+            v.w = u8_to_f(src >> 24);
+            ((uint2*)rad)[i] = pack_half(v);
+        } else {
+            // UNORM8 -> float -> UNORM8 reproduces every byte, so the alpha
+            // byte moves unchanged (copyAlphaImage.comp:25-28).
+            uint32_t* r = (uint32_t*)rad;
+            r[i] = (r[i] & 0x00ffffffu) | (src & 0xff000000u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- K7
+template <typename T>
+__global__ void k_wrap_border(T* __restrict__ vol, int D, int res, int n, int levels, int faces, int rwb)
+{
+    uint32_t gx = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t gy = blockIdx.y * blockDim.y + threadIdx.y;
+    uint32_t gz = blockIdx.z * blockDim.z + threadIdx.z;
+    if (gx >= (uint32_t)n || gy >= (uint32_t)n || gz >= (uint32_t)n) return;
+    if (gx >= (uint32_t)rwb || gy >= (uint32_t)rwb || gz >= (uint32_t)rwb) return;
+    // borderWrapping.comp:20-22: interior texels return
+    if (gx < (uint32_t)res + 1 && gy < (uint32_t)res + 1 && gz < (uint32_t)res + 1 && gx > 0 && gy > 0 && gz > 0)
+        return;
+    int rx = (((int)gx + res - 1) & (res - 1)) + 1;
+    int ry = (((int)gy + res - 1) & (res - 1)) + 1;
+    int rz = (((int)gz + res - 1) & (res - 1)) + 1;
+    for (int l = 0; l < levels; ++l)
+        for (int f = 0; f < faces; ++f)
+            vol[vol_index(D, l, f, (int)gx, (int)gy, (int)gz)] = vol[vol_index(D, l, f, rx, ry, rz)];
+}
+
+// ---------------------------------------------------------------- K4 / K5
+
+__device__ __forceinline__ int image_coord(int p, int res)
+{
+    int q = p / res;
+    if (q < 0) q = -q;
+    return (p + res * (q + 1)) & (res - 1);
+}
+
+__device__ __forceinline__ float mixf(float x, float y, float a)
+{
+#pragma clang fp contract(off)
+    return x * (1.0f - a) + y * a;
+}
+
+// Front/back texel pairs of the six faces (opacityDownSample.comp:94-138),
+// texel k at offset (k & 1, (k >> 1) & 1, k >> 2).
+__constant__ int8_t c_pairs[6][4][2] = {
+    {{0, 1}, {2, 3}, {4, 5}, {6, 7}}, {{1, 0}, {3, 2}, {5, 4}, {7, 6}},
+    {{0, 2}, {1, 3}, {4, 6}, {5, 7}}, {{2, 0}, {3, 1}, {6, 4}, {7, 5}},
+    {{0, 4}, {1, 5}, {2, 6}, {3, 7}}, {{4, 0}, {5, 1}, {6, 2}, {7, 3}},
+};
+
+template <bool kHalf, bool kRadiance>
+__global__ __launch_bounds__(256) void k_downsample(typename Texel<kHalf>::T* __restrict__ vol, int D, int res,
+                                                    int level, int3 prev_min, int region_size)
+{
+#pragma clang fp contract(off)
+    using TX = Texel<kHalf>;
+    const int half = res >> 1;
+    int gx = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int gy = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    int gz = (int)(blockIdx.z * blockDim.z + threadIdx.z);
+    if (gx >= half || gy >= half || gz >= half) return;
+    const int cur[3] = {(prev_min.x >> 1) + gx, (prev_min.y >> 1) + gy, (prev_min.z >> 1) + gz};
+    const int pm[3] = {prev_min.x, prev_min.y, prev_min.z};
+    int wp[3], ps[3];
+    for (int k = 0; k < 3; ++k) {
+        wp[k] = image_coord(cur[k], res) + 1;
+        ps[k] = image_coord(cur[k] << 1, res) + 1;
+    }
+    // transition band (opacityDownSample.comp:55-64)
+    float dist[3];
+    for (int k = 0; k < 3; ++k) {
+        float center = (float)(pm[k] >> 1) + (float)(half >> 1);
+        dist[k] = fabsf((float)cur[k] + 0.5f - center) - 0.5f;
+    }
+    float lerp = 0.0f;
+    const float inv = 1.0f / ((float)region_size + 1.0f);
+    const float thr = (float)((half >> 1) - region_size);
+    if (dist[0] >= thr || dist[1] >= thr || dist[2] >= thr) {
+        float m = fmaxf(dist[0], fmaxf(dist[1], dist[2]));
+        lerp = m - (float)((half >> 1) - region_size) + 1.0f;
+        lerp = lerp * inv;
+    }
+    for (int f = 0; f < kFaces; ++f) {
+        f4 v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            v[i] = TX::load(vol, vol_index(D, level - 1, f, ps[0] + (i & 1), ps[1] + ((i >> 1) & 1), ps[2] + (i >> 2)));
+        const size_t wi = vol_index(D, level, f, wp[0], wp[1], wp[2]);
+        f4 self = TX::load(vol, wi);
+        if (!kRadiance) {
+            float s = 0.0f;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const f4 a = v[c_pairs[f][p][0]];
+                const f4 b = v[c_pairs[f][p][1]];
+                s = s + a.w;
+                s = s + (1.0f - a.w) * b.w;
+            }
+            float down = s * 0.25f;
+            self.w = mixf(down, self.x, lerp);
+            self.y = self.w;
+            TX::store(vol, wi, self);
+        } else {
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const f4 a = v[c_pairs[f][p][0]];
+                const f4 b = v[c_pairs[f][p][1]];
+                const float t = 1.0f - a.w;
+                acc[0] = acc[0] + a.x;
+                acc[0] = acc[0] + t * b.x;
+                acc[1] = acc[1] + a.y;
+                acc[1] = acc[1] + t * b.y;
+                acc[2] = acc[2] + a.z;
+                acc[2] = acc[2] + t * b.z;
+                acc[3] = acc[3] + a.w;
+                acc[3] = acc[3] + t * b.w;
+            }
+            f4 out = {mixf(acc[0] * 0.25f, self.x, lerp), mixf(acc[1] * 0.25f, self.y, lerp),
+                      mixf(acc[2] * 0.25f, self.z, lerp), mixf(acc[3] * 0.25f, self.w, lerp)};
+            TX::store(vol, wi, out);
+        }
+    }
+}
+
+}  // namespace vct
+
+using namespace vct;
+
+static vct_status check_volume(vct_ctx* c, vct_volume v)
+{
+    if (!c) return fail(VCT_ERR_INVALID_ARGUMENT, "null context");
+    if (v != VCT_VOLUME_OPACITY && v != VCT_VOLUME_RADIANCE) return fail(VCT_ERR_INVALID_ARGUMENT, "bad volume");
+    return VCT_OK;
+}
+
+static bool is_half(const vct_ctx* c, vct_volume v)
+{
+    return v == VCT_VOLUME_RADIANCE && c->cfg.radiance_format == VCT_FORMAT_RGBA16F;
+}
+
+extern "C" {
+
+vct_status vct_clear_region(vct_ctx* c, vct_stream stream, vct_volume v, const vct_image_cleaning_desc* d)
+{
+    VCT_TRY(check_volume(c, v));
+    if (!d) return fail(VCT_ERR_INVALID_ARGUMENT, "null desc");
+    if (d->clip_level < 0 || d->clip_level >= c->levels) return fail(VCT_ERR_INVALID_ARGUMENT, "clip_level");
+    if (d->clipmap_resolution != c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "clipmap_resolution");
+    if (d->face_count > (uint32_t)kFaces) return fail(VCT_ERR_INVALID_ARGUMENT, "face_count");
+    for (int k = 0; k < 3; ++k) {
+        if (d->region_min_corner[k] < 0) return fail(VCT_ERR_INVALID_ARGUMENT, "region_min_corner must be >= 0");
+        if (d->clip_max_extent[k] > (uint32_t)c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "extent > resolution");
+    }
+    if (!d->clip_max_extent[0] || !d->clip_max_extent[1] || !d->clip_max_extent[2] || !d->face_count) return VCT_OK;
+    dim3 block(64, 2, 2);
+    dim3 grid((d->clip_max_extent[0] + 63) / 64, (d->clip_max_extent[1] + 1) / 2, (d->clip_max_extent[2] + 1) / 2);
+    int3 minc = make_int3(d->region_min_corner[0], d->region_min_corner[1], d->region_min_corner[2]);
+    uint3 ext = make_uint3(d->clip_max_extent[0], d->clip_max_extent[1], d->clip_max_extent[2]);
+    hipStream_t s = (hipStream_t)stream;
+    if (is_half(c, v))
+        hipLaunchKernelGGL(k_clear_region<true>, grid, block, 0, s, (uint2*)c->radiance.ptr, c->D, c->res,
+                           d->clip_level, minc, ext, (int)d->face_count);
+    else
+        hipLaunchKernelGGL(k_clear_region<false>, grid, block, 0, s,
+                           (uint32_t*)(v == VCT_VOLUME_OPACITY ? c->opacity.ptr : c->radiance.ptr), c->D, c->res,
+                           d->clip_level, minc, ext, (int)d->face_count);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status vct_copy_alpha(vct_ctx* c, vct_stream stream, const vct_copy_alpha_desc* d)
+{
+    if (!c || !d) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (d->clip_level < 0 || d->clip_level >= c->levels) return fail(VCT_ERR_INVALID_ARGUMENT, "clip_level");
+    if (d->clipmap_resolution != c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "clipmap_resolution");
+    if (d->face_count < 0 || d->face_count > kFaces) return fail(VCT_ERR_INVALID_ARGUMENT, "face_count");
+    dim3 block(64, 2, 2);
+    dim3 grid((c->res + 63) / 64, (c->res + 1) / 2, (c->res + 1) / 2);
+    hipStream_t s = (hipStream_t)stream;
+    if (is_half(c, VCT_VOLUME_RADIANCE))
+        hipLaunchKernelGGL(k_copy_alpha<true>, grid, block, 0, s, (uint2*)c->radiance.ptr,
+                           (const uint32_t*)c->opacity.ptr, c->D, c->res, d->clip_level, d->face_count);
+    else
+        hipLaunchKernelGGL(k_copy_alpha<false>, grid, block, 0, s, (uint32_t*)c->radiance.ptr,
+                           (const uint32_t*)c->opacity.ptr, c->D, c->res, d->clip_level, d->face_count);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status vct_wrap_border(vct_ctx* c, vct_stream stream, vct_volume v, const vct_border_wrapping_desc* d)
+{
+    VCT_TRY(check_volume(c, v));
+    if (!d) return fail(VCT_ERR_INVALID_ARGUMENT, "null desc");
+    if (d->clipmap_resolution != (uint32_t)c->res || d->clip_border_width != 2u)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "desc must be {resolution, 2, faces, levels}");
+    if (d->face_count > (uint32_t)kFaces || d->clip_region_count > (uint32_t)c->levels)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "face_count / clip_region_count out of range");
+    const int rwb = c->res + 2;
+    // BorderWrapper.cpp:139-140 dispatches ((res + border) >> 3) groups of 8.
+    const int n = (c->cfg.quirks & VCT_QUIRK_BORDER_SKIP_FAR) ? 8 * (rwb >> 3) : rwb;
+    dim3 block(64, 2, 2);
+    dim3 grid((n + 63) / 64, (n + 1) / 2, (n + 1) / 2);
+    hipStream_t s = (hipStream_t)stream;
+    if (is_half(c, v))
+        hipLaunchKernelGGL(k_wrap_border<uint2>, grid, block, 0, s, (uint2*)c->radiance.ptr, c->D, c->res, n,
+                           (int)d->clip_region_count, (int)d->face_count, rwb);
+    else
+        hipLaunchKernelGGL(k_wrap_border<uint32_t>, grid, block, 0, s,
+                           (uint32_t*)(v == VCT_VOLUME_OPACITY ? c->opacity.ptr : c->radiance.ptr), c->D, c->res, n,
+                           (int)d->clip_region_count, (int)d->face_count, rwb);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status vct_downsample(vct_ctx* c, vct_stream stream, vct_volume v, const vct_downsample_desc* d)
+{
+    VCT_TRY(check_volume(c, v));
+    if (!d) return fail(VCT_ERR_INVALID_ARGUMENT, "null desc");
+    if (d->clip_level < 1 || d->clip_level >= c->levels)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "clip_level must be in [1, levels)");
+    if (d->clipmap_resolution != c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "clipmap_resolution");
+    if (d->downsample_region_size < 0 || d->downsample_region_size >= c->res / 4)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "downsample_region_size");
+    const int half = c->res >> 1;
+    dim3 block(half < 64 ? half : 64, 2, 2);
+    dim3 grid((half + block.x - 1) / block.x, (half + 1) / 2, (half + 1) / 2);
+    int3 pm = make_int3(d->prev_region_min_corner[0], d->prev_region_min_corner[1], d->prev_region_min_corner[2]);
+    hipStream_t s = (hipStream_t)stream;
+    if (v == VCT_VOLUME_OPACITY)
+        hipLaunchKernelGGL((k_downsample<false, false>), grid, block, 0, s, (uint32_t*)c->opacity.ptr, c->D, c->res,
+                           d->clip_level, pm, d->downsample_region_size);
+    else if (is_half(c, v))
+        hipLaunchKernelGGL((k_downsample<true, true>), grid, block, 0, s, (uint2*)c->radiance.ptr, c->D, c->res,
+                           d->clip_level, pm, d->downsample_region_size);
+    else
+        hipLaunchKernelGGL((k_downsample<false, true>), grid, block, 0, s, (uint32_t*)c->radiance.ptr, c->D, c->res,
+                           d->clip_level, pm, d->downsample_region_size);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// vct_internal.hpp - context, memory layout and device helpers shared by the
+// gfx950 kernels of libvct.
+//
+// Volume layout in HBM. Each volume (opacity, radiance) is a dense array of
+// (levels x 6 faces) blocks of D^3 texels, D = res + 2 (one border texel per
+// side), x fastest inside a block:
+//     index(level, face, x, y, z) = ((level*6 + face)*D + z)*D*D + y*D + x
+// This is the reference atlas (Voxelizer.h:40-52: x = face*D + x, y = level*D
+// + y) with the face/level blocks made contiguous, so one trilinear footprint
+// stays inside one 2.2 MB block and the per-level kernels stream contiguous
+// memory. RGBA8 texels are uint32 (R in the low byte); RGBA16F texels are
+// 4 halves (8 bytes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/vct.h"
+
+namespace vct {
+
+constexpr int kFaces = 6;
+constexpr int kMaxLevels = 8;
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// Grow-only device scratch allocation.
+vct_status ensure(DeviceBuffer& b, size_t bytes);
+void release(DeviceBuffer& b);
+
+// Per-thread error detail for vct_last_error().
+vct_status fail(vct_status s, const char* fmt, ...);
+
+#define VCT_HIP(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            return ::vct::fail(VCT_ERR_HIP, "%s failed: %s (%s:%d)", #expr,              \
+                               hipGetErrorString(_e), __FILE__, __LINE__);               \
+    } while (0)
+
+#define VCT_TRY(expr)                           \
+    do {                                        \
+        vct_status _s = (expr);                 \
+        if (_s != VCT_OK) return _s;            \
+    } while (0)
+
+// World-space triangle soup produced by the vertex stage.
+struct TriSoup {
+    const float* pos;     // 9 floats per triangle
+    const float* nrm;     // 9 floats per triangle
+    const uint32_t* mat;  // material index per triangle
+    uint32_t count;
+};
+
+struct SvoState {
+    DeviceBuffer frags;      // uvec2 per fragment
+    DeviceBuffer pool;       // uvec2 per node
+    DeviceBuffer counters;   // small device counters
+    uint32_t nfrag = 0;
+    uint32_t level = 0;
+    uint32_t pool_nodes = 0; // capacity
+    uint32_t node_count = 0; // 8 + 8 * allocated blocks, known after build
+    bool built = false;
+};
+
+}  // namespace vct
+
+struct vct_ctx {
+    vct_config cfg{};
+    int res = 0;
+    int D = 0;
+    int levels = 0;
+    int device = 0;
+    vct_clipmap_region regions[vct::kMaxLevels]{};
+    bool regions_set = false;
+
+    vct::DeviceBuffer opacity;   // uint32 texels
+    vct::DeviceBuffer radiance;  // uint32 or uint2 texels
+
+    // scratch
+    vct::DeviceBuffer soup_pos, soup_nrm, soup_mat;
+    vct::DeviceBuffer prims, prim_first;
+    vct::DeviceBuffer counts, offsets, scan_tmp, totals;
+    vct::DeviceBuffer rec_key[2], rec_idx[2], rec_val, sort_tmp;
+    vct::DeviceBuffer staging;
+    vct::DeviceBuffer materials_flat;
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+
+    vct::SvoState svo;
+};
+
+namespace vct {
+
+inline size_t texels_per_volume(const vct_ctx* c)
+{
+    return (size_t)c->D * c->D * c->D * kFaces * (size_t)c->levels;
+}
+inline size_t texel_bytes(const vct_ctx* c, vct_volume v)
+{
+    return (v == VCT_VOLUME_RADIANCE && c->cfg.radiance_format == VCT_FORMAT_RGBA16F) ? 8u : 4u;
+}
+
+__host__ __device__ inline size_t vol_index(int D, int level, int face, int x, int y, int z)
+{
+    return ((((size_t)level * kFaces + (size_t)face) * (size_t)D + (size_t)z) * (size_t)D + (size_t)y) * (size_t)D +
+           (size_t)x;
+}
+
+// Mesh vertex stage (msaaVoxelizer.vert:31-37): world-space triangles.
+vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, TriSoup* out);
+
+// Pinned staging copy of a small host array to a device buffer (stream ordered).
+vct_status upload_small(vct_ctx* ctx, hipStream_t s, DeviceBuffer& dst, const void* src, size_t bytes);
+
+}  // namespace vct

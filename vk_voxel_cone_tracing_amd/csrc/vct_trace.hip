@@ -1,0 +1,509 @@
+// vct_trace.hip - per-pixel voxel cone tracing (K8) for gfx950.
+//
+// One lane per pixel; a 64-lane wavefront covers an 8x8 pixel tile so that
+// neighbouring lanes march nearby cones and share cache lines. Texture
+// filtering is done in registers: the sampler of the reference
+// (voxelConeTracing.frag:313-327, REPEAT + LINEAR over the atlas) reduces to
+// a trilinear fetch inside one (level, face) block at the block coordinate
+// u = fract(p / extent) * res + 0.5, whose 2x2x2 footprint never leaves the
+// block's [0, res+1] range. The three faces selected by the cone direction
+// share the footprint offsets, so one address computation serves 24 loads.
+#include <hip/hip_runtime.h>
+
+#include "vct_internal.hpp"
+#include "vct_texel.hpp"
+
+namespace vct {
+
+__constant__ float c_dirs16[16][3] = {
+    {0.57735f, 0.57735f, 0.57735f},       {0.57735f, -0.57735f, -0.57735f},
+    {-0.57735f, 0.57735f, -0.57735f},     {-0.57735f, -0.57735f, 0.57735f},
+    {-0.903007f, -0.182696f, -0.388844f}, {-0.903007f, 0.182696f, 0.388844f},
+    {0.903007f, -0.182696f, 0.388844f},   {0.903007f, 0.182696f, -0.388844f},
+    {-0.388844f, -0.903007f, -0.182696f}, {0.388844f, -0.903007f, 0.182696f},
+    {0.388844f, 0.903007f, -0.182696f},   {-0.388844f, 0.903007f, 0.182696f},
+    {-0.182696f, -0.388844f, -0.903007f}, {0.182696f, 0.388844f, -0.903007f},
+    {-0.182696f, 0.388844f, 0.903007f},   {0.182696f, -0.388844f, 0.903007f}};
+
+__constant__ float c_dirs32[32][3] = {
+    {0.898904f, 0.435512f, 0.0479745f},   {0.898904f, -0.435512f, -0.0479745f},
+    {0.898904f, 0.0479745f, -0.435512f},  {0.898904f, -0.0479745f, 0.435512f},
+    {-0.898904f, 0.435512f, -0.0479745f}, {-0.898904f, -0.435512f, 0.0479745f},
+    {-0.898904f, 0.0479745f, 0.435512f},  {-0.898904f, -0.0479745f, -0.435512f},
+    {0.0479745f, 0.898904f, 0.435512f},   {-0.0479745f, 0.898904f, -0.435512f},
+    {-0.435512f, 0.898904f, 0.0479745f},  {0.435512f, 0.898904f, -0.0479745f},
+    {-0.0479745f, -0.898904f, 0.435512f}, {0.0479745f, -0.898904f, -0.435512f},
+    {0.435512f, -0.898904f, 0.0479745f},  {-0.435512f, -0.898904f, -0.0479745f},
+    {0.435512f, 0.0479745f, 0.898904f},   {-0.435512f, -0.0479745f, 0.898904f},
+    {0.0479745f, -0.435512f, 0.898904f},  {-0.0479745f, 0.435512f, 0.898904f},
+    {0.435512f, -0.0479745f, -0.898904f}, {-0.435512f, 0.0479745f, -0.898904f},
+    {0.0479745f, 0.435512f, -0.898904f},  {-0.0479745f, -0.435512f, -0.898904f},
+    {0.57735f, 0.57735f, 0.57735f},       {0.57735f, 0.57735f, -0.57735f},
+    {0.57735f, -0.57735f, 0.57735f},      {0.57735f, -0.57735f, -0.57735f},
+    {-0.57735f, 0.57735f, 0.57735f},      {-0.57735f, 0.57735f, -0.57735f},
+    {-0.57735f, -0.57735f, 0.57735f},     {-0.57735f, -0.57735f, -0.57735f}};
+
+struct TraceParams {
+    float center[3];
+    uint32_t mode;
+    float voxel_size;
+    float dim;
+    float start_offset;
+    float diffuse_intensity;
+    float ao_factor;
+    float step_factor;  // max(0.2, minTraceStepFactor)
+    float specular_intensity;
+    float decay;
+    int cones32;
+    int levels;
+    int res;
+    int D;
+    float vp_inv[16];
+    float eye[3];
+    int width, height;
+    uint32_t quirks;
+    // light
+    float light_dir[3];  // normalize(-direction)
+    float light_view[16];
+    float light_proj[16];
+    float z_near, z_far;
+    const float* shadow;
+    int sw, sh;
+};
+
+template <bool kHalf>
+struct Vol;
+
+template <>
+struct Vol<false> {
+    using T = uint32_t;
+    __device__ static __forceinline__ f4 load(const T* __restrict__ v, size_t i)
+    {
+        const uint32_t t = v[i];
+        const float s = 1.0f / 255.0f;
+        return {(float)(t & 0xffu) * s, (float)((t >> 8) & 0xffu) * s, (float)((t >> 16) & 0xffu) * s,
+                (float)(t >> 24) * s};
+    }
+};
+
+template <>
+struct Vol<true> {
+    using T = uint2;
+    __device__ static __forceinline__ f4 load(const T* __restrict__ v, size_t i) { return unpack_half(v[i]); }
+};
+
+// Trilinear sample of the three selected faces of one level, weighted.
+template <bool kHalf>
+__device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P,
+                                           const float pos[3], int level, const int face[3], const float w[3])
+{
+    const float vs = P.voxel_size * exp2f((float)level);
+    const float inv_extent = 1.0f / (vs * P.dim);
+    int i0[3];
+    float fr[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float q = pos[k] * inv_extent;
+        const float u = (q - floorf(q)) * (float)P.res + 0.5f;
+        const float fu = floorf(u);
+        fr[k] = u - fu;
+        int i = (int)fu;
+        i = i < 0 ? 0 : (i > P.res ? P.res : i);
+        i0[k] = i;
+    }
+    const size_t D = (size_t)P.D;
+    const size_t o000 = ((size_t)i0[2] * D + (size_t)i0[1]) * D + (size_t)i0[0];
+    const size_t oy = D, oz = D * D;
+    const float ax = fr[0], ay = fr[1], az = fr[2];
+    const float w000 = (1.0f - ax) * (1.0f - ay) * (1.0f - az), w100 = ax * (1.0f - ay) * (1.0f - az);
+    const float w010 = (1.0f - ax) * ay * (1.0f - az), w110 = ax * ay * (1.0f - az);
+    const float w001 = (1.0f - ax) * (1.0f - ay) * az, w101 = ax * (1.0f - ay) * az;
+    const float w011 = (1.0f - ax) * ay * az, w111 = ax * ay * az;
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const size_t base = (((size_t)level * kFaces + (size_t)face[a]) * D) * D * D + o000;
+        const typename Vol<kHalf>::T* b = vol + base;
+        const f4 t000 = Vol<kHalf>::load(b, 0), t100 = Vol<kHalf>::load(b, 1);
+        const f4 t010 = Vol<kHalf>::load(b, oy), t110 = Vol<kHalf>::load(b, oy + 1);
+        const f4 t001 = Vol<kHalf>::load(b, oz), t101 = Vol<kHalf>::load(b, oz + 1);
+        const f4 t011 = Vol<kHalf>::load(b, oz + oy), t111 = Vol<kHalf>::load(b, oz + oy + 1);
+        const float wa = w[a];
+#define VCT_TRI(c)                                                                                        \
+    acc.c += wa * (w000 * t000.c + w100 * t100.c + w010 * t010.c + w110 * t110.c + w001 * t001.c +       \
+                   w101 * t101.c + w011 * t011.c + w111 * t111.c)
+        VCT_TRI(x);
+        VCT_TRI(y);
+        VCT_TRI(z);
+        VCT_TRI(w);
+#undef VCT_TRI
+    }
+    return acc;
+}
+
+// voxelConeTracing.frag:341-392 traceCone
+template <bool kHalf>
+__device__ f4 trace_cone(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P, const float start_in[3],
+                         const float dir[3], float aperture, float max_dist, float start_level, float step_factor,
+                         uint32_t& steps)
+{
+    f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float cone_coef = 2.0f * tanf(aperture * 0.5f);
+    float voxel_size = P.voxel_size * exp2f(start_level);
+    float start[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) start[k] = start_in[k] + dir[k] * voxel_size * P.start_offset * 0.5f;
+    float step = 0.0f;
+    float diameter = fmaxf(step * cone_coef, P.voxel_size);
+    float occlusion = 0.0f;
+    const int face[3] = {dir[0] > 0.0f ? 0 : 1, dir[1] > 0.0f ? 2 : 3, dir[2] > 0.0f ? 4 : 5};
+    const float w[3] = {dir[0] * dir[0], dir[1] * dir[1], dir[2] * dir[2]};
+    float seg = voxel_size;
+    const float min_radius = P.voxel_size * P.dim * 0.5f;
+    const float max_level = (float)(P.levels - 1);
+    const float inv_vs = 1.0f / P.voxel_size;
+    while (step < max_dist && occlusion < 1.0f) {
+        float pos[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) pos[k] = start[k] + dir[k] * step;
+        const float dx = P.center[0] - pos[0], dy = P.center[1] - pos[1], dz = P.center[2] - pos[2];
+        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float min_level = ceilf(log2f(dist / min_radius));
+        float cur = log2f(diameter * inv_vs);
+        cur = fminf(fmaxf(fmaxf(start_level, cur), min_level), max_level);
+        const int lower = (int)floorf(cur);
+        const int upper = (int)ceilf(cur);
+        const float fl = cur - floorf(cur);
+        f4 s = sample_level<kHalf>(vol, P, pos, lower, face, w);
+        if (upper != lower) {
+            const f4 hi = sample_level<kHalf>(vol, P, pos, upper, face, w);
+            s.x = s.x * (1.0f - fl) + hi.x * fl;
+            s.y = s.y * (1.0f - fl) + hi.y * fl;
+            s.z = s.z * (1.0f - fl) + hi.z * fl;
+            s.w = s.w * (1.0f - fl) + hi.w * fl;
+        }
+        voxel_size = P.voxel_size * exp2f(cur);
+        const float correction = seg / voxel_size;
+        float opacity = 1.0f - powf(1.0f - s.w, correction);
+        opacity = fminf(fmaxf(opacity, 0.0f), 1.0f);
+        const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
+        result.x += blend * (s.x * correction);
+        result.y += blend * (s.y * correction);
+        result.z += blend * (s.z * correction);
+        result.w += blend * opacity;
+        occlusion += (1.0f - occlusion) * opacity / (1.0f + (step + voxel_size) * P.decay);
+        const float prev = step;
+        step += fmaxf(diameter, P.voxel_size) * step_factor;
+        seg = step - prev;
+        diameter = step * cone_coef;
+        ++steps;
+    }
+    return {result.x, result.y, result.z, 1.0f - occlusion};
+}
+
+__device__ __forceinline__ float shadow_fetch(const TraceParams& P, int i, int j)
+{
+    if (i < 0 || j < 0 || i >= P.sw || j >= P.sh) return 0.0f;
+    return P.shadow[(size_t)j * (size_t)P.sw + (size_t)i];
+}
+
+// shadow.glsl:28-36 (PCF16 of bilinear taps)
+__device__ float trace_visibility(const TraceParams& P, const float pos[3])
+{
+    const float* V = P.light_view;
+    const float* Pm = P.light_proj;
+    float ls[3];
+    for (int k = 0; k < 3; ++k) ls[k] = V[0 + k] * pos[0] + V[4 + k] * pos[1] + V[8 + k] * pos[2] + V[12 + k];
+    const bool compare = !(P.quirks & VCT_QUIRK_SHADOW_RAW_DEPTH);
+    float ref = 0.0f;
+    if (compare) ref = Pm[2] * ls[0] + Pm[6] * ls[1] + Pm[10] * ls[2] + Pm[14] - 0.001f;
+    ls[2] = ls[2] / (P.z_far - P.z_near);
+    float xy[2];
+    for (int k = 0; k < 2; ++k) xy[k] = Pm[0 + k] * ls[0] + Pm[4 + k] * ls[1] + Pm[12 + k];
+    xy[0] = xy[0] * 0.5f + 0.5f;
+    xy[1] = xy[1] * 0.5f + 0.5f;
+    const float sx = 1.0f / (float)P.sw, sy = 1.0f / (float)P.sh;
+    float sum = 0.0f;
+    for (int yi = 0; yi < 4; ++yi)
+        for (int xi = 0; xi < 4; ++xi) {
+            const float s = xy[0] + ((float)xi - 1.5f) * sx, t = xy[1] + ((float)yi - 1.5f) * sy;
+            float uu = s * (float)P.sw - 0.5f, vv = t * (float)P.sh - 0.5f;
+            float fu = floorf(uu), fv = floorf(vv);
+            const float a = uu - fu, b = vv - fv;
+            fu = fminf(fmaxf(fu, -4.0f), (float)P.sw + 4.0f);
+            fv = fminf(fmaxf(fv, -4.0f), (float)P.sh + 4.0f);
+            const int i0 = (int)fu, j0 = (int)fv;
+            float tq[4];
+            const int ii[4] = {i0, i0 + 1, i0, i0 + 1}, jj[4] = {j0, j0, j0 + 1, j0 + 1};
+            for (int q = 0; q < 4; ++q) {
+                float d = shadow_fetch(P, ii[q], jj[q]);
+                if (compare) {
+                    const bool in = ii[q] >= 0 && jj[q] >= 0 && ii[q] < P.sw && jj[q] < P.sh;
+                    d = (!in || ref <= d) ? 1.0f : 0.0f;
+                }
+                tq[q] = d;
+            }
+            sum += (1.0f - a) * (1.0f - b) * tq[0] + a * (1.0f - b) * tq[1] + (1.0f - a) * b * tq[2] + a * b * tq[3];
+        }
+    return sum * 0.0625f;
+}
+
+// brdf.glsl:28-78 microfacetBRDF
+__device__ void brdf(const float n[3], const float v[3], const float l[3], const float dc[3], const float sc[3],
+                     float rough, float out[3])
+{
+    const float PI = 3.141592f;
+    const float alpha = rough * rough;
+    const float reflectance = fmaxf(fmaxf(sc[0], sc[1]), sc[2]);
+    const float r90 = fminf(fmaxf(reflectance * 50.0f, 0.0f), 1.0f);
+    float h[3] = {l[0] + v[0], l[1] + v[1], l[2] + v[2]};
+    const float hl = sqrtf(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    h[0] /= hl;
+    h[1] /= hl;
+    h[2] /= hl;
+    const float ndl = fminf(fmaxf(n[0] * l[0] + n[1] * l[1] + n[2] * l[2], 0.001f), 1.0f);
+    const float ndv = fminf(fmaxf(fabsf(n[0] * v[0] + n[1] * v[1] + n[2] * v[2]), 0.001f), 1.0f);
+    const float ndh = fminf(fmaxf(n[0] * h[0] + n[1] * h[1] + n[2] * h[2], 0.0f), 1.0f);
+    const float vdh = fminf(fmaxf(v[0] * h[0] + v[1] * h[1] + v[2] * h[2], 0.0f), 1.0f);
+    const float r = alpha;
+    const float att_l = 2.0f * ndl / (ndl + sqrtf(r * r + (1.0f - r * r) * (ndl * ndl)));
+    const float att_v = 2.0f * ndv / (ndv + sqrtf(r * r + (1.0f - r * r) * (ndv * ndv)));
+    const float G = att_l * att_v;
+    const float rsq = alpha * alpha;
+    const float f = (ndh * rsq - ndh) * ndh + 1.0f;
+    const float Dd = rsq / (PI * f * f);
+    const float omv = fminf(fmaxf(1.0f - vdh, 0.0f), 1.0f);
+    const float fres = omv * omv * omv * omv * omv;
+    for (int k = 0; k < 3; ++k) {
+        const float F = sc[k] + (r90 - sc[k]) * fres;
+        const float diffuse_contrib = (1.0f - F) * (dc[k] / PI);
+        const float spec_contrib = F * G * Dd / (4.0f * ndl * ndv);
+        out[k] = ndl * (diffuse_contrib + spec_contrib);
+    }
+}
+
+__device__ __forceinline__ float half_to_float(uint16_t h)
+{
+    _Float16 f;
+    __builtin_memcpy(&f, &h, 2);
+    return (float)f;
+}
+
+template <bool kHalf>
+__global__ __launch_bounds__(256) void k_trace(const typename Vol<kHalf>::T* __restrict__ vol, TraceParams P,
+                                               const uint32_t* __restrict__ g_diffuse,
+                                               const uint16_t* __restrict__ g_normal,
+                                               const uint32_t* __restrict__ g_specular,
+                                               const uint16_t* __restrict__ g_emission,
+                                               const float* __restrict__ g_depth, float4* __restrict__ out_d,
+                                               float4* __restrict__ out_s, unsigned long long* __restrict__ samples)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    uint32_t steps = 0;
+    if (px < P.width && py < P.height) {
+        const size_t idx = (size_t)py * (size_t)P.width + (size_t)px;
+        float4 dO = make_float4(0.0f, 0.0f, 0.0f, 1.0f), sO = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        const float depth = g_depth[idx];
+        if (depth != 1.0f) {
+            const float tcx = ((float)px + 0.5f) / (float)P.width, tcy = ((float)py + 0.5f) / (float)P.height;
+            const float q[4] = {tcx * 2.0f - 1.0f, tcy * 2.0f - 1.0f, depth, 1.0f};
+            float wp[4];
+            for (int k = 0; k < 4; ++k)
+                wp[k] = P.vp_inv[k] * q[0] + P.vp_inv[4 + k] * q[1] + P.vp_inv[8 + k] * q[2] + P.vp_inv[12 + k] * q[3];
+            const float world[3] = {wp[0] / wp[3], wp[1] / wp[3], wp[2] / wp[3]};
+            float view[3] = {P.eye[0] - world[0], P.eye[1] - world[1], P.eye[2] - world[2]};
+            {
+                const float l = sqrtf(view[0] * view[0] + view[1] * view[1] + view[2] * view[2]);
+                view[0] /= l;
+                view[1] /= l;
+                view[2] /= l;
+            }
+            const uint32_t dt = g_diffuse[idx], st = g_specular[idx];
+            const float dc[3] = {u8_to_f(dt & 0xffu), u8_to_f((dt >> 8) & 0xffu), u8_to_f((dt >> 16) & 0xffu)};
+            const float sc[3] = {u8_to_f(st & 0xffu), u8_to_f((st >> 8) & 0xffu), u8_to_f((st >> 16) & 0xffu)};
+            const float rough = u8_to_f(dt >> 24), metallic = u8_to_f(st >> 24);
+            const uint2 nraw = *(const uint2*)(g_normal + 4 * idx);
+            const uint2 eraw = *(const uint2*)(g_emission + 4 * idx);
+            const f4 nf = unpack_half(nraw), ef = unpack_half(eraw);
+            float normal[3] = {nf.x * 2.0f - 1.0f, nf.y * 2.0f - 1.0f, nf.z * 2.0f - 1.0f};
+            {
+                const float l = sqrtf(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+                normal[0] /= l;
+                normal[1] /= l;
+                normal[2] /= l;
+            }
+            const bool has_emission = ef.x > 0.0f || ef.y > 0.0f || ef.z > 0.0f;
+
+            // calcMinLevel (voxelConeTracing.frag:394-414)
+            float min_level;
+            {
+                const float dx = P.center[0] - world[0], dy = P.center[1] - world[1], dz = P.center[2] - world[2];
+                const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+                const float min_radius = P.voxel_size * P.dim * 0.5f;
+                const float ml = fmaxf(log2f(dist / min_radius), 0.0f);
+                const float radius = min_radius * exp2f(ceilf(ml));
+                const float f = dist / radius;
+                min_level = f > 0.5f ? ceilf(ml) + (f - 0.5f) * 2.0f : ceilf(ml);
+            }
+            const float vs = P.voxel_size * exp2f(min_level);
+            float start[3];
+            for (int k = 0; k < 3; ++k) start[k] = world[k] + normal[k] * vs * P.start_offset;
+            f4 ind = {0.0f, 0.0f, 0.0f, 1.0f};
+            const int ncones = P.cones32 ? 32 : 16;
+            const float aperture = P.cones32 ? 0.628319f : 0.872665f;
+            const bool want_diffuse = (P.mode == 4u || P.mode == 5u || P.mode == 7u || P.mode == 8u);
+            for (int i = 0; want_diffuse && i < ncones; ++i) {
+                const float* d = P.cones32 ? c_dirs32[i] : c_dirs16[i];
+                const float ct = normal[0] * d[0] + normal[1] * d[1] + normal[2] * d[2];
+                if (ct < 0.0f) continue;
+                const f4 c = trace_cone<kHalf>(vol, P, start, d, aperture, 30.0f, min_level, P.step_factor, steps);
+                ind.x += c.x * ct;
+                ind.y += c.y * ct;
+                ind.z += c.z * ct;
+                ind.w += c.w * ct;
+            }
+            const float inv_n = 1.0f / (float)ncones;
+            ind.x *= inv_n;
+            ind.y *= inv_n;
+            ind.z *= inv_n;
+            ind.w *= inv_n;
+            ind.w *= P.ao_factor;
+            ind.x *= dc[0] * P.diffuse_intensity;
+            ind.y *= dc[1] * P.diffuse_intensity;
+            ind.z *= dc[2] * P.diffuse_intensity;
+
+            float spec[3] = {0.0f, 0.0f, 0.0f};
+            const bool want_spec = (P.mode == 6u || P.mode == 8u);
+            if (want_spec && (sc[0] > 1e-6f || sc[1] > 1e-6f || sc[2] > 1e-6f) && metallic > 1e-6f) {
+                const float I[3] = {-view[0], -view[1], -view[2]};
+                const float dni = normal[0] * I[0] + normal[1] * I[1] + normal[2] * I[2];
+                const float dir[3] = {I[0] - 2.0f * dni * normal[0], I[1] - 2.0f * dni * normal[1],
+                                      I[2] - 2.0f * dni * normal[2]};
+                const f4 c = trace_cone<kHalf>(vol, P, start, dir, fmaxf(rough, 0.05f), 30.0f, min_level,
+                                               P.voxel_size, steps);
+                spec[0] = c.x * sc[0] * P.specular_intensity;
+                spec[1] = c.y * sc[1] * P.specular_intensity;
+                spec[2] = c.z * sc[2] * P.specular_intensity;
+            }
+            float direct[3] = {0.0f, 0.0f, 0.0f};
+            const bool want_direct = (P.mode == 4u || P.mode == 5u || P.mode == 8u);
+            if (want_direct) {
+                if (has_emission) {
+                    direct[0] = ef.x;
+                    direct[1] = ef.y;
+                    direct[2] = ef.z;
+                } else {
+                    float b[3];
+                    brdf(normal, view, P.light_dir, dc, sc, rough, b);
+                    const float visv = trace_visibility(P, world);
+                    for (int k = 0; k < 3; ++k) direct[k] = b[k] * visv;
+                }
+            }
+            switch (P.mode) {
+            case 0: dO = make_float4(dc[0], dc[1], dc[2], 1.0f); break;
+            case 1: dO = make_float4(sc[0], sc[1], sc[2], 1.0f); break;
+            case 2: dO = make_float4(normal[0] * 0.5f + 0.5f, normal[1] * 0.5f + 0.5f, normal[2] * 0.5f + 0.5f, 1.0f); break;
+            case 3: {
+                const float colors[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+                int lower = (int)floorf(min_level);
+                lower = lower < 0 ? 0 : (lower > 5 ? 5 : lower);
+                const float fr = min_level - floorf(min_level);
+                dO = make_float4((colors[lower][0] * (1.0f - fr) + colors[lower + 1][0] * fr) * 0.5f,
+                                 (colors[lower][1] * (1.0f - fr) + colors[lower + 1][1] * fr) * 0.5f,
+                                 (colors[lower][2] * (1.0f - fr) + colors[lower + 1][2] * fr) * 0.5f, 0.5f);
+                break;
+            }
+            case 4: dO = make_float4(direct[0] * ind.w, direct[1] * ind.w, direct[2] * ind.w, 1.0f); break;
+            case 5:
+                dO = make_float4(direct[0] * ind.w + ind.x, direct[1] * ind.w + ind.y, direct[2] * ind.w + ind.z, 1.0f);
+                break;
+            case 6: sO = make_float4(spec[0], spec[1], spec[2], 1.0f); break;
+            case 7: dO = make_float4(ind.w, ind.w, ind.w, 1.0f); break;
+            case 8:
+                dO = make_float4(direct[0] * ind.w + ind.x, direct[1] * ind.w + ind.y, direct[2] * ind.w + ind.z, 1.0f);
+                sO = make_float4(spec[0], spec[1], spec[2], 1.0f);
+                break;
+            default: break;
+            }
+        }
+        out_d[idx] = dO;
+        out_s[idx] = sO;
+    }
+    if (samples) {
+        unsigned long long v = steps;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0 && v) atomicAdd(samples, v);
+    }
+}
+
+}  // namespace vct
+
+using namespace vct;
+
+extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer* g, const vct_camera_ubo* cam,
+                                const vct_cone_tracing_desc* d, const vct_shadow_map* shadow,
+                                const vct_directional_light_desc* light,
+                                const vct_directional_light_shadow_desc* light_shadow, float* diffuse_out,
+                                float* specular_out, unsigned long long* samples_out)
+{
+    if (!c || !g || !cam || !d || !diffuse_out || !specular_out) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!g->diffuse || !g->normal || !g->specular || !g->emission || !g->depth || !g->width || !g->height)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "incomplete G-buffer");
+    if (d->rendering_mode > 8u) return fail(VCT_ERR_INVALID_ARGUMENT, "rendering_mode");
+    if (d->volume_dimension != (float)c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "volume_dimension must equal the resolution");
+    if (!(d->voxel_size > 0.0f)) return fail(VCT_ERR_INVALID_ARGUMENT, "voxel_size");
+    const bool need_light = d->rendering_mode == 4u || d->rendering_mode == 5u || d->rendering_mode == 8u;
+    if (need_light && (!shadow || !shadow->depth || !shadow->width || !shadow->height || !light || !light_shadow))
+        return fail(VCT_ERR_INVALID_ARGUMENT, "light, light_shadow and shadow map are required for this mode");
+    TraceParams P{};
+    for (int k = 0; k < 3; ++k) P.center[k] = d->volume_center[k];
+    P.mode = d->rendering_mode;
+    P.voxel_size = d->voxel_size;
+    P.dim = d->volume_dimension;
+    P.start_offset = d->trace_start_offset;
+    P.diffuse_intensity = d->indirect_diffuse_intensity;
+    P.ao_factor = d->ambient_occlusion_factor;
+    P.step_factor = d->min_trace_step_factor > 0.2f ? d->min_trace_step_factor : 0.2f;
+    P.specular_intensity = d->indirect_specular_intensity;
+    P.decay = d->occlusion_decay;
+    P.cones32 = d->enable_32_cones != 0;
+    P.levels = c->levels;
+    P.res = c->res;
+    P.D = c->D;
+    for (int k = 0; k < 16; ++k) P.vp_inv[k] = cam->view_proj_inv[k];
+    for (int k = 0; k < 3; ++k) P.eye[k] = cam->eye_pos[k];
+    P.width = (int)g->width;
+    P.height = (int)g->height;
+    P.quirks = c->cfg.quirks;
+    if (light) {
+        float nd[3] = {-light->direction[0], -light->direction[1], -light->direction[2]};
+        float len = std::sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+        for (int k = 0; k < 3; ++k) P.light_dir[k] = len > 0.0f ? nd[k] / len : 0.0f;
+    }
+    if (light_shadow) {
+        for (int k = 0; k < 16; ++k) {
+            P.light_view[k] = light_shadow->view[k];
+            P.light_proj[k] = light_shadow->proj[k];
+        }
+        P.z_near = light_shadow->z_near;
+        P.z_far = light_shadow->z_far;
+    }
+    if (shadow) {
+        P.shadow = shadow->depth;
+        P.sw = (int)shadow->width;
+        P.sh = (int)shadow->height;
+    }
+    dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+    hipStream_t s = (hipStream_t)stream;
+    if (c->cfg.radiance_format == VCT_FORMAT_RGBA16F)
+        hipLaunchKernelGGL(k_trace<true>, grid, dim3(256), 0, s, (const uint2*)c->radiance.ptr, P, g->diffuse, g->normal,
+                           g->specular, g->emission, g->depth, (float4*)diffuse_out, (float4*)specular_out, samples_out);
+    else
+        hipLaunchKernelGGL(k_trace<false>, grid, dim3(256), 0, s, (const uint32_t*)c->radiance.ptr, P, g->diffuse,
+                           g->normal, g->specular, g->emission, g->depth, (float4*)diffuse_out, (float4*)specular_out,
+                           samples_out);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}

@@ -1,0 +1,473 @@
+// vct_voxelize.hip - clip-level triangle voxelization (K1) and radiance
+// injection (K2) for gfx950.
+//
+// Both passes expand triangles into (triangle, cell, sample) work items with
+// an exclusive scan over per-triangle cell counts, so every wavefront gets an
+// even share of raster work regardless of triangle size. K1 writes opacity
+// directly (the stores are idempotent). K2 reproduces the reference's
+// order-dependent imageAtomicCompSwap running average deterministically: each
+// fragment emits one record per written face texel keyed by
+// (texel address, work index), a radix sort groups the records by texel in
+// fragment order, and one lane per texel applies the running average (RGBA8)
+// or the FP32 sum/count mean (RGBA16F) in that order.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+
+#include "vct_internal.hpp"
+#include "vct_raster.hpp"
+#include "vct_texel.hpp"
+
+namespace vct {
+
+struct VoxParams {
+    float region_min[3];
+    float region_max[3];
+    float clip_max_extent;
+    float voxel_size;
+    int clip_level;
+    int res;
+    int D;
+    int lo[3];
+    int hi[3];
+    int samples;
+    uint32_t quirks;
+};
+
+// msaaVoxelizer.frag:43-55 / msaaInjectRadiance.frag:162-174
+__device__ __forceinline__ void image_coords(const VoxParams& P, const float pos[3], int ic[3])
+{
+#pragma clang fp contract(off)
+    const float c = P.voxel_size * 0.25f;
+    for (int k = 0; k < 3; ++k) {
+        const int kr = (P.quirks & VCT_QUIRK_CLAMP_X_RANGE) ? 0 : k;
+        const float lo = P.region_min[kr] + c;
+        const float hi = P.region_max[kr] - c;
+        const float w = fminf(fmaxf(pos[k], lo), hi);
+        const float q = w / P.clip_max_extent;
+        const float fr = q - floorf(q);
+        ic[k] = ((int)(fr * (float)P.res)) % P.res + 1;
+    }
+}
+
+__global__ void k_count(const float* __restrict__ pos, uint32_t ntris, VoxParams P, unsigned long long* __restrict__ counts)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntris) return;
+    float p[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) p[k] = pos[9u * t + k];
+    TriSetup s;
+    unsigned long long n = 0;
+    if (tri_setup(p, P.voxel_size, P.lo, P.hi, s)) n = (unsigned long long)s.nu * s.nv * (unsigned long long)P.samples;
+    counts[t] = n;
+}
+
+__global__ void k_total(const unsigned long long* __restrict__ counts, const unsigned long long* __restrict__ offsets,
+                        uint32_t n, unsigned long long* __restrict__ total)
+{
+    total[0] = n ? offsets[n - 1] + counts[n - 1] : 0ull;
+    total[1] = 0ull;  // record counter for K2
+}
+
+__device__ __forceinline__ uint32_t find_tri(const unsigned long long* __restrict__ offsets, uint32_t n,
+                                             unsigned long long wi)
+{
+    uint32_t lo = 0, hi = n;  // last t with offsets[t] <= wi
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (offsets[mid] <= wi) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_raster_opacity(const float* __restrict__ pos,
+                                                        const unsigned long long* __restrict__ offsets, uint32_t ntris,
+                                                        const unsigned long long* __restrict__ total, VoxParams P,
+                                                        uint32_t* __restrict__ vol)
+{
+    const unsigned long long n = total[0];
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long wi = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; wi < n; wi += stride) {
+        const uint32_t t = find_tri(offsets, ntris, wi);
+        float p[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) p[k] = pos[9u * t + k];
+        TriSetup s;
+        if (!tri_setup(p, P.voxel_size, P.lo, P.hi, s)) continue;
+        float fp[3];
+        double lam[3];
+        if (!tri_sample(p, s, P.voxel_size, P.samples, (uint32_t)(wi - offsets[t]), fp, lam)) continue;
+        // msaaVoxelizer.frag:59-60
+        if (fp[0] < P.region_min[0] || fp[1] < P.region_min[1] || fp[2] < P.region_min[2] ||
+            fp[0] > P.region_max[0] || fp[1] > P.region_max[1] || fp[2] > P.region_max[2])
+            continue;
+        int ic[3];
+        image_coords(P, fp, ic);
+        // imageStore(vec4(1.0)) into the six faces (msaaVoxelizer.frag:69-73)
+        for (int f = 0; f < kFaces; ++f) vol[vol_index(P.D, P.clip_level, f, ic[0], ic[1], ic[2])] = 0xffffffffu;
+    }
+}
+
+struct LightParams {
+    float dir[3];       // normalize(-direction)
+    float color[3];
+    float intensity;
+    float view[16];
+    float proj[16];
+    float z_near, z_far;
+    const float* depth;
+    int width, height;
+};
+
+__device__ __forceinline__ float fetch_depth(const LightParams& L, int i, int j)
+{
+    if (i < 0 || j < 0 || i >= L.width || j >= L.height) return 0.0f;  // CLAMP_TO_BORDER, opaque black
+    return L.depth[(size_t)j * (size_t)L.width + (size_t)i];
+}
+
+__device__ __forceinline__ float shadow_tap(const LightParams& L, float s, float t, bool compare, float ref)
+{
+#pragma clang fp contract(off)
+    float uu = s * (float)L.width - 0.5f;
+    float vv = t * (float)L.height - 0.5f;
+    float fu = floorf(uu), fv = floorf(vv);
+    float a = uu - fu, b = vv - fv;
+    fu = fminf(fmaxf(fu, -4.0f), (float)L.width + 4.0f);
+    fv = fminf(fmaxf(fv, -4.0f), (float)L.height + 4.0f);
+    int i0 = (int)fu, j0 = (int)fv;
+    float t00 = fetch_depth(L, i0, j0), t10 = fetch_depth(L, i0 + 1, j0);
+    float t01 = fetch_depth(L, i0, j0 + 1), t11 = fetch_depth(L, i0 + 1, j0 + 1);
+    if (compare) {
+        bool in00 = i0 >= 0 && j0 >= 0 && i0 < L.width && j0 < L.height;
+        bool in10 = i0 + 1 >= 0 && j0 >= 0 && i0 + 1 < L.width && j0 < L.height;
+        bool in01 = i0 >= 0 && j0 + 1 >= 0 && i0 < L.width && j0 + 1 < L.height;
+        bool in11 = i0 + 1 >= 0 && j0 + 1 >= 0 && i0 + 1 < L.width && j0 + 1 < L.height;
+        t00 = (!in00 || ref <= t00) ? 1.0f : 0.0f;
+        t10 = (!in10 || ref <= t10) ? 1.0f : 0.0f;
+        t01 = (!in01 || ref <= t01) ? 1.0f : 0.0f;
+        t11 = (!in11 || ref <= t11) ? 1.0f : 0.0f;
+    }
+    float r = ((1.0f - a) * (1.0f - b)) * t00;
+    r = r + (a * (1.0f - b)) * t10;
+    r = r + ((1.0f - a) * b) * t01;
+    r = r + (a * b) * t11;
+    return r;
+}
+
+// shadow.glsl:28-36 calcVisibility + PCF16 (:14-26)
+__device__ float visibility(const LightParams& L, const float pos[3], uint32_t quirks)
+{
+#pragma clang fp contract(off)
+    const float* V = L.view;
+    const float* Pm = L.proj;
+    float ls[3];
+    for (int k = 0; k < 3; ++k) ls[k] = V[0 + k] * pos[0] + V[4 + k] * pos[1] + V[8 + k] * pos[2] + V[12 + k];
+    const bool compare = !(quirks & VCT_QUIRK_SHADOW_RAW_DEPTH);
+    float ref = 0.0f;
+    if (compare) {
+        ref = Pm[2] * ls[0] + Pm[6] * ls[1] + Pm[10] * ls[2] + Pm[14];
+        ref = ref - 0.001f;
+    }
+    ls[2] = ls[2] / (L.z_far - L.z_near);
+    float xy[2];
+    for (int k = 0; k < 2; ++k) xy[k] = Pm[0 + k] * ls[0] + Pm[4 + k] * ls[1] + Pm[12 + k];
+    xy[0] = xy[0] * 0.5f + 0.5f;
+    xy[1] = xy[1] * 0.5f + 0.5f;
+    const float sx = 1.0f / (float)L.width;
+    const float sy = 1.0f / (float)L.height;
+    float sum = 0.0f;
+    for (float y = -1.5f; y <= 1.5f; y += 1.0f)
+        for (float x = -1.5f; x <= 1.5f; x += 1.0f) sum = sum + shadow_tap(L, xy[0] + x * sx, xy[1] + y * sy, compare, ref);
+    return sum * 0.0625f;
+}
+
+struct Record {
+    float r, g, b;
+};
+
+__global__ __launch_bounds__(256) void k_raster_inject(const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                       const uint32_t* __restrict__ mat,
+                                                       const vct_gltf_material* __restrict__ mats,
+                                                       const unsigned long long* __restrict__ offsets, uint32_t ntris,
+                                                       unsigned long long* __restrict__ total, VoxParams P,
+                                                       LightParams L, float box_lo0, float box_lo1, float box_lo2,
+                                                       float box_hi0, float box_hi1, float box_hi2,
+                                                       unsigned long long* __restrict__ keys,
+                                                       uint32_t* __restrict__ idx, Record* __restrict__ vals,
+                                                       uint32_t cap)
+{
+#pragma clang fp contract(off)
+    const unsigned long long n = total[0];
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    const float box_lo[3] = {box_lo0, box_lo1, box_lo2};
+    const float box_hi[3] = {box_hi0, box_hi1, box_hi2};
+    for (unsigned long long wi = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; wi < n; wi += stride) {
+        const uint32_t t = find_tri(offsets, ntris, wi);
+        float p[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) p[k] = pos[9u * t + k];
+        TriSetup s;
+        if (!tri_setup(p, P.voxel_size, P.lo, P.hi, s)) continue;
+        float fp[3];
+        double lam[3];
+        if (!tri_sample(p, s, P.voxel_size, P.samples, (uint32_t)(wi - offsets[t]), fp, lam)) continue;
+        if (fp[0] < box_lo[0] || fp[1] < box_lo[1] || fp[2] < box_lo[2] || fp[0] > box_hi[0] || fp[1] > box_hi[1] ||
+            fp[2] > box_hi[2])
+            continue;
+        const vct_gltf_material& m = mats[mat[t]];
+        int ic[3];
+        image_coords(P, fp, ic);
+        Record out[6];
+        int faces[6];
+        int nout = 0;
+        if (m.emissive_factor[0] > 0.0f || m.emissive_factor[1] > 0.0f || m.emissive_factor[2] > 0.0f) {
+            // msaaInjectRadiance.frag:76-86
+            Record e = {fminf(fmaxf(m.emissive_factor[0], 0.0f), 1.0f), fminf(fmaxf(m.emissive_factor[1], 0.0f), 1.0f),
+                        fminf(fmaxf(m.emissive_factor[2], 0.0f), 1.0f)};
+            for (int f = 0; f < 6; ++f) {
+                out[f] = e;
+                faces[f] = f;
+            }
+            nout = 6;
+        } else {
+            // msaaInjectRadiance.frag:131-153
+            const float* n0 = nrm + 9u * t;
+            float ni[3];
+            for (int k = 0; k < 3; ++k)
+                ni[k] = (float)(lam[0] * (double)n0[k] + lam[1] * (double)n0[3 + k] + lam[2] * (double)n0[6 + k]);
+            const float len = sqrtf(ni[0] * ni[0] + ni[1] * ni[1] + ni[2] * ni[2]);
+            const float nn[3] = {ni[0] / len, ni[1] / len, ni[2] / len};
+            float ndl = nn[0] * L.dir[0] + nn[1] * L.dir[1] + nn[2] * L.dir[2];
+            ndl = fminf(fmaxf(ndl, 0.001f), 1.0f);
+            const float vis = visibility(L, fp, P.quirks);
+            float lc[3];
+            for (int k = 0; k < 3; ++k) lc[k] = ndl * vis * L.color[k] * L.intensity;
+            if (lc[0] == 0.0f && lc[1] == 0.0f && lc[2] == 0.0f) continue;
+            float rad[3];
+            for (int k = 0; k < 3; ++k)
+                rad[k] = fminf(fmaxf(lc[k] * m.base_color_factor[k] * m.base_color_factor[3], 0.0f), 1.0f);
+            const int face[3] = {(-nn[0] > 0.0f) ? 0 : 1, (-nn[1] > 0.0f) ? 2 : 3, (-nn[2] > 0.0f) ? 4 : 5};
+            for (int k = 0; k < 3; ++k) {
+                const float w = fabsf(nn[k]);
+                out[k] = {rad[0] * w, rad[1] * w, rad[2] * w};
+                faces[k] = face[k];
+            }
+            nout = 3;
+        }
+        const unsigned long long base = atomicAdd(&total[1], (unsigned long long)nout);
+        for (int j = 0; j < nout; ++j) {
+            const unsigned long long r = base + (unsigned long long)j;
+            if (r >= cap) break;  // capacity is checked on the host from total[1]
+            const size_t addr = vol_index(P.D, P.clip_level, faces[j], ic[0], ic[1], ic[2]);
+            keys[r] = ((unsigned long long)addr << 32) | (wi & 0xffffffffull);
+            idx[r] = (uint32_t)r;
+            vals[r] = out[j];
+        }
+    }
+}
+
+template <bool kHalf>
+__global__ __launch_bounds__(256) void k_apply(const unsigned long long* __restrict__ keys,
+                                               const uint32_t* __restrict__ idx, const Record* __restrict__ vals,
+                                               uint32_t n, void* __restrict__ vol)
+{
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long addr = keys[i] >> 32;
+    if (i > 0 && (keys[i - 1] >> 32) == addr) return;  // not the head of its texel's run
+    if (!kHalf) {
+        uint32_t t = ((uint32_t*)vol)[addr];
+        for (uint32_t j = i; j < n && (keys[j] >> 32) == addr; ++j) {
+            const Record v = vals[idx[j]];
+            t = rgba8_avg(t, f4{v.r * 255.0f, v.g * 255.0f, v.b * 255.0f, 1.0f});
+        }
+        ((uint32_t*)vol)[addr] = t;
+    } else {
+        float sr = 0.0f, sg = 0.0f, sb = 0.0f, cnt = 0.0f;
+        for (uint32_t j = i; j < n && (keys[j] >> 32) == addr; ++j) {
+            const Record v = vals[idx[j]];
+            sr = sr + v.r;
+            sg = sg + v.g;
+            sb = sb + v.b;
+            cnt = cnt + 1.0f;
+        }
+        ((uint2*)vol)[addr] = pack_half(f4{sr / cnt, sg / cnt, sb / cnt, 1.0f});
+    }
+}
+
+static vct_status make_params(vct_ctx* c, const vct_voxelization_desc* d, uint32_t level, VoxParams* P)
+{
+    if (!d) return fail(VCT_ERR_INVALID_ARGUMENT, "null desc");
+    if (level >= (uint32_t)c->levels || d->clip_level != level)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "level mismatch (desc %u, call %u)", d->clip_level, level);
+    if (d->clipmap_resolution != c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "clipmap_resolution");
+    if (!(d->voxel_size > 0.0f) || !(d->clip_max_extent > 0.0f))
+        return fail(VCT_ERR_INVALID_ARGUMENT, "voxel_size / clip_max_extent must be positive");
+    for (int k = 0; k < 3; ++k) {
+        P->region_min[k] = d->region_min_corner[k];
+        P->region_max[k] = d->region_max_corner[k];
+        // integer region recovered from the corners written by Voxelizer::cmdVoxelize
+        const float lo = std::rint((d->region_min_corner[k] + 1e-6f) / d->voxel_size);
+        const float hi = std::rint((d->region_max_corner[k] - 1e-6f) / d->voxel_size);
+        if (!(std::fabs(lo) < 1e8f) || !(std::fabs(hi) < 1e8f) || hi < lo)
+            return fail(VCT_ERR_INVALID_ARGUMENT, "region corners");
+        P->lo[k] = (int)lo - 1;
+        P->hi[k] = (int)hi;
+    }
+    P->clip_max_extent = d->clip_max_extent;
+    P->voxel_size = d->voxel_size;
+    P->clip_level = (int)level;
+    P->res = c->res;
+    P->D = c->D;
+    P->samples = (int)c->cfg.raster_samples;
+    P->quirks = c->cfg.quirks;
+    return VCT_OK;
+}
+
+// Per-triangle counts and their exclusive scan; the total lands in totals[0].
+static vct_status expand(vct_ctx* c, hipStream_t s, const TriSoup& soup, const VoxParams& P)
+{
+    const uint32_t n = soup.count;
+    VCT_TRY(ensure(c->counts, 8u * (size_t)(n ? n : 1)));
+    VCT_TRY(ensure(c->offsets, 8u * (size_t)(n ? n : 1)));
+    VCT_TRY(ensure(c->totals, 64));
+    if (n) {
+        hipLaunchKernelGGL(k_count, dim3((n + 255) / 256), dim3(256), 0, s, soup.pos, n, P,
+                           (unsigned long long*)c->counts.ptr);
+        VCT_HIP(hipGetLastError());
+        size_t tmp = 0;
+        VCT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (unsigned long long*)c->counts.ptr,
+                                                 (unsigned long long*)c->offsets.ptr, (int)n, s));
+        VCT_TRY(ensure(c->scan_tmp, tmp));
+        VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, (unsigned long long*)c->counts.ptr,
+                                                 (unsigned long long*)c->offsets.ptr, (int)n, s));
+    }
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, s, (const unsigned long long*)c->counts.ptr,
+                       (const unsigned long long*)c->offsets.ptr, n, (unsigned long long*)c->totals.ptr);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+static int raster_grid(unsigned long long work)
+{
+    unsigned long long blocks = (work + 255) / 256;
+    if (blocks > 8192) blocks = 8192;  // 256 CUs x 32 waves of grid-stride work
+    if (blocks < 1) blocks = 1;
+    return (int)blocks;
+}
+
+}  // namespace vct
+
+using namespace vct;
+
+extern "C" {
+
+vct_status vct_voxelize_opacity(vct_ctx* c, vct_stream stream, const vct_mesh* mesh, const vct_voxelization_desc* d,
+                                uint32_t level)
+{
+    if (!c) return fail(VCT_ERR_INVALID_ARGUMENT, "null context");
+    VoxParams P;
+    VCT_TRY(make_params(c, d, level, &P));
+    hipStream_t s = (hipStream_t)stream;
+    TriSoup soup;
+    VCT_TRY(run_vertex_stage(c, s, mesh, &soup));
+    if (!soup.count) return VCT_OK;
+    VCT_TRY(expand(c, s, soup, P));
+    // grid-stride over a device-side total: no host synchronisation
+    hipLaunchKernelGGL(k_raster_opacity, dim3(8192), dim3(256), 0, s, soup.pos,
+                       (const unsigned long long*)c->offsets.ptr, soup.count,
+                       (const unsigned long long*)c->totals.ptr, P, (uint32_t*)c->opacity.ptr);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status vct_inject_radiance(vct_ctx* c, vct_stream stream, const vct_mesh* mesh, const vct_shadow_map* shadow,
+                               const vct_directional_light_desc* light,
+                               const vct_directional_light_shadow_desc* light_shadow,
+                               const vct_voxelization_desc* d, uint32_t level)
+{
+    if (!c) return fail(VCT_ERR_INVALID_ARGUMENT, "null context");
+    if (!shadow || !shadow->depth || !shadow->width || !shadow->height || !light || !light_shadow)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "light, light_shadow and shadow map are required");
+    VoxParams P;
+    VCT_TRY(make_params(c, d, level, &P));
+    hipStream_t s = (hipStream_t)stream;
+    TriSoup soup;
+    VCT_TRY(run_vertex_stage(c, s, mesh, &soup));
+    if (!soup.count) return VCT_OK;
+    VCT_TRY(expand(c, s, soup, P));
+
+    LightParams L;
+    {
+        // normalize(-uDirectionalLight.direction) with IEEE sqrt/div
+        float nd[3] = {-light->direction[0], -light->direction[1], -light->direction[2]};
+        float len = std::sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+        for (int k = 0; k < 3; ++k) L.dir[k] = nd[k] / len;
+        for (int k = 0; k < 3; ++k) L.color[k] = light->color[k];
+        L.intensity = light->intensity;
+        for (int k = 0; k < 16; ++k) {
+            L.view[k] = light_shadow->view[k];
+            L.proj[k] = light_shadow->proj[k];
+        }
+        L.z_near = light_shadow->z_near;
+        L.z_far = light_shadow->z_far;
+        L.depth = shadow->depth;
+        L.width = (int)shadow->width;
+        L.height = (int)shadow->height;
+    }
+    float box_lo[3], box_hi[3];
+    for (int k = 0; k < 3; ++k) {
+        box_lo[k] = (float)P.lo[k] * P.voxel_size;
+        box_hi[k] = (float)(P.hi[k] + 1) * P.voxel_size;
+    }
+    // The record buffer is sized from the work total: one host read-back.
+    unsigned long long tot[2] = {0, 0};
+    VCT_HIP(hipMemcpyAsync(tot, c->totals.ptr, 16, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipStreamSynchronize(s));
+    if (tot[0] == 0) return VCT_OK;
+    if (tot[0] > 0xffffffffull) return fail(VCT_ERR_CAPACITY, "raster work %llu exceeds 2^32", tot[0]);
+    const unsigned long long cap64 = tot[0] * 6ull;
+    const uint32_t cap = cap64 > 0xfffffff0ull ? 0xfffffff0u : (uint32_t)cap64;
+    for (int b = 0; b < 2; ++b) {
+        VCT_TRY(ensure(c->rec_key[b], 8u * (size_t)cap));
+        VCT_TRY(ensure(c->rec_idx[b], 4u * (size_t)cap));
+    }
+    VCT_TRY(ensure(c->rec_val, sizeof(Record) * (size_t)cap));
+    hipLaunchKernelGGL(k_raster_inject, dim3(raster_grid(tot[0])), dim3(256), 0, s, soup.pos, soup.nrm, soup.mat,
+                       mesh->materials, (const unsigned long long*)c->offsets.ptr, soup.count,
+                       (unsigned long long*)c->totals.ptr, P, L, box_lo[0], box_lo[1], box_lo[2], box_hi[0],
+                       box_hi[1], box_hi[2], (unsigned long long*)c->rec_key[0].ptr, (uint32_t*)c->rec_idx[0].ptr,
+                       (Record*)c->rec_val.ptr, cap);
+    VCT_HIP(hipGetLastError());
+    VCT_HIP(hipMemcpyAsync(tot, c->totals.ptr, 16, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipStreamSynchronize(s));
+    if (tot[1] > cap) return fail(VCT_ERR_CAPACITY, "injection records overflow (%llu > %u)", tot[1], cap);
+    const uint32_t nrec = (uint32_t)tot[1];
+    if (!nrec) return VCT_OK;
+    int end_bit = 32;
+    while ((1ull << (end_bit - 32)) < texels_per_volume(c)) ++end_bit;
+    size_t tmp = 0;
+    VCT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (unsigned long long*)c->rec_key[0].ptr,
+                                               (unsigned long long*)c->rec_key[1].ptr, (uint32_t*)c->rec_idx[0].ptr,
+                                               (uint32_t*)c->rec_idx[1].ptr, (int)nrec, 0, end_bit, s));
+    VCT_TRY(ensure(c->sort_tmp, tmp));
+    VCT_HIP(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp.ptr, tmp, (unsigned long long*)c->rec_key[0].ptr,
+                                               (unsigned long long*)c->rec_key[1].ptr, (uint32_t*)c->rec_idx[0].ptr,
+                                               (uint32_t*)c->rec_idx[1].ptr, (int)nrec, 0, end_bit, s));
+    const bool half = c->cfg.radiance_format == VCT_FORMAT_RGBA16F;
+    if (half)
+        hipLaunchKernelGGL(k_apply<true>, dim3((nrec + 255) / 256), dim3(256), 0, s,
+                           (const unsigned long long*)c->rec_key[1].ptr, (const uint32_t*)c->rec_idx[1].ptr,
+                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr);
+    else
+        hipLaunchKernelGGL(k_apply<false>, dim3((nrec + 255) / 256), dim3(256), 0, s,
+                           (const unsigned long long*)c->rec_key[1].ptr, (const uint32_t*)c->rec_idx[1].ptr,
+                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+}  // extern "C"
