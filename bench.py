@@ -1,0 +1,276 @@
+"""Benchmark: the 1080p clipmap GI frame on one MI355X.
+
+Workload (6 x 128^3 clipmap GI): the seeded procedural room scaled to 16
+units (21 260 triangles), RGBA16F radiance + RGBA8 opacity clipmaps of
+6 levels x 128^3 (6 faces, 1-texel border), a 2048^2 fp32 shadow map from a
+seeded directional light, and a 1920x1080 G-buffer ray cast from a seeded
+camera with per-pixel perceptual roughness U[0.1, 0.8] and metallic > 0 so
+that every pixel traces the specular cone.
+
+One step = one GI frame through the reference's passes: VoxelizationPass
+(clear, 6 levels of opacity voxelization, opacity mips, border wrap),
+RadianceInjectionPass with every level due (clear, injection with 16-tap PCF,
+copy-alpha, radiance mips) and VoxelConeTracingPass in CombinedGI mode
+(16 diffuse cones + 1 specular cone per pixel). The G-buffer and shadow map
+are frame inputs, produced once before timing.
+
+Prints one JSON line (see the repository's benchmark contract).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+METRIC = "1080p GI ms/frame + cone samples/s; batched & adjoint vs plain, same GPU"
+HBM_PEAK_GBS = 8000.0
+B_SAMPLE_RGBA16F = 384  # 2 levels x 3 faces x 8 texels x 8 B
+B_PIXEL = 124  # G-buffer 28 B + outputs 32 B + 16 PCF taps x 4 B
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=30, help="rows of the 1080p image traced by the CPU baseline")
+    return ap.parse_args()
+
+
+def setup():
+    import torch
+    import vk_voxel_cone_tracing_amd as V
+    from vk_voxel_cone_tracing_amd import passes, scenes
+
+    W, H, S = 1920, 1080, 16.0
+    scene = scenes.room(2, scale=S)
+    ds = V.DeviceScene(scene)
+    ctx = V.Context(resolution=128, levels=6, radiance_format=V.FORMAT_RGBA16F, quirks=V.QUIRK_ALL,
+                    raster_samples=1, downsample_region_size=10, extent_l0=16.0)
+    vp, vp_inv, eye = scenes.camera(2, W, H, scale=S)
+    ctx.update_clip_regions(eye)
+    light = scenes.light(2, scale=S)
+    ld, ls = V.light_structs(light)
+    dev = "cuda"
+    shadow_depth = torch.empty((2048, 2048), dtype=torch.float32, device=dev)
+    V._abi.call("vct_raycast_shadow", ctx._h, C.c_void_p(torch.cuda.current_stream().cuda_stream),
+                C.byref(ds.mesh), C.byref(ls), C.c_void_p(shadow_depth.data_ptr()), 2048, 2048)
+    g = dict(
+        diffuse=torch.empty((H, W), dtype=torch.int32, device=dev),
+        normal=torch.empty((H, W, 4), dtype=torch.int16, device=dev),
+        specular=torch.empty((H, W), dtype=torch.int32, device=dev),
+        emission=torch.empty((H, W, 4), dtype=torch.int16, device=dev),
+        depth=torch.empty((H, W), dtype=torch.float32, device=dev),
+    )
+    surface = torch.from_numpy(np.ascontiguousarray(scene.surface)).to(dev)
+    cam = V.camera_ubo(vp, vp_inv, eye)
+    V._abi.call("vct_raycast_gbuffer", ctx._h, C.c_void_p(torch.cuda.current_stream().cuda_stream),
+                C.byref(ds.mesh), C.c_void_p(surface.data_ptr()), C.byref(cam), C.c_void_p(g["diffuse"].data_ptr()),
+                C.c_void_p(g["normal"].data_ptr()), C.c_void_p(g["specular"].data_ptr()),
+                C.c_void_p(g["emission"].data_ptr()), C.c_void_p(g["depth"].data_ptr()), W, H)
+    # per-pixel perceptual roughness U[0.1, 0.8] in the diffuse alpha byte
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2)
+    rough = (torch.rand((H, W), generator=gen, device=dev) * 0.7 + 0.1)
+    rough_b = torch.round(rough * 255.0).to(torch.int32)
+    g["diffuse"] = (g["diffuse"] & 0x00FFFFFF) | (rough_b << 24)
+    torch.cuda.synchronize()
+    sm = V.shadow_struct(shadow_depth)
+    gb = V.gbuffer_struct(g)
+    dout = torch.zeros((H, W, 4), dtype=torch.float32, device=dev)
+    sout = torch.zeros((H, W, 4), dtype=torch.float32, device=dev)
+    mgr = passes.RenderPassManager()
+    mgr.add_render_pass("VoxelizationPass", passes.VoxelizationPass(ctx, ds))
+    inj = passes.RadianceInjectionPass(ctx, ds, sm, ld, ls)
+    mgr.add_render_pass("RadianceInjectionPass", inj)
+    vct_pass = passes.VoxelConeTracingPass(ctx, gb, cam, sm, ld, ls, dout, sout)
+    vct_pass.samples = torch.zeros(1, dtype=torch.int64, device=dev)
+    mgr.add_render_pass("VoxelConeTracingPass", vct_pass)
+    return dict(scene=scene, ds=ds, ctx=ctx, vp=vp, vp_inv=vp_inv, eye=eye, light=light, shadow=shadow_depth, g=g,
+                mgr=mgr, inj=inj, vct=vct_pass, W=W, H=H, dout=dout, sout=sout)
+
+
+def frame(st, ev=None):
+    """One GI frame; with `ev` records per-pass events."""
+    import vk_voxel_cone_tracing_amd as V
+
+    f = V.Frame()
+    st["inj"].frame_index = 0  # every clip level is due in the measured frame
+    st["vct"].samples.zero_()
+    if ev is not None:
+        ev[0].record()
+    st["mgr"].draw_single_render_pass("VoxelizationPass", f)
+    if ev is not None:
+        ev[1].record()
+    st["mgr"].draw_single_render_pass("RadianceInjectionPass", f)
+    if ev is not None:
+        ev[2].record()
+    st["mgr"].draw_single_render_pass("VoxelConeTracingPass", f)
+    if ev is not None:
+        ev[3].record()
+
+
+def cpu_baseline(st, rows: int):
+    """The oracle (the CPU restatement of the reference shaders) on the same
+    inputs: all voxel passes of the frame plus `rows` evenly spaced rows of the
+    trace; the full-frame time is the voxel passes plus the traced rows scaled
+    to 1080."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_api as O
+
+    ctx = st["ctx"]
+    scene = st["scene"]
+    pos, nrm, mat = scene.world_triangles()
+    L = O.lib()
+    RES, LV = 128, 6
+    opa = O.new_volume(RES, LV, 0)
+    rad = O.new_volume(RES, LV, 1)
+    tris = O.tris_struct(pos, nrm, mat)
+    depth = st["shadow"].cpu().numpy()
+    osm = O.Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
+    ol = O.light_struct(st["light"])
+    mats = O.materials_struct(scene)
+    regions = ctx.regions
+    g = {k: v.cpu().numpy() for k, v in st["g"].items()}
+    g = {k: (v.view(np.uint32) if v.dtype == np.int32 else (v.view(np.uint16) if v.dtype == np.int16 else v))
+         for k, v in g.items()}
+    t0 = time.perf_counter()
+    for lvl in range(LV):
+        d = ctx.voxelization_desc(lvl)
+        od = O.VoxDesc()
+        od.region_cell_min = (C.c_int * 3)(*regions[lvl][0])
+        od.region_cell_extent = (C.c_int * 3)(*regions[lvl][1])
+        od.region_min = (C.c_float * 3)(*d.region_min_corner)
+        od.region_max = (C.c_float * 3)(*d.region_max_corner)
+        od.clip_max_extent = d.clip_max_extent
+        od.voxel_size = d.voxel_size
+        od.clip_level = lvl
+        od.clipmap_resolution = RES
+        L.orc_voxelize_opacity(C.byref(O.vol_struct(opa, RES, LV)), C.byref(tris), C.byref(od), 1, 0x3F)
+        L.orc_inject_radiance(C.byref(O.vol_struct(rad, RES, LV)), C.byref(tris), mats, C.byref(osm), C.byref(ol),
+                              C.byref(od), 1, 0x3F)
+    for lvl in range(1, LV):
+        L.orc_downsample_opacity(C.byref(O.vol_struct(opa, RES, LV)), (C.c_int * 3)(*regions[lvl - 1][0]), lvl, 10)
+    L.orc_wrap_border(C.byref(O.vol_struct(opa, RES, LV)), 2, 6, LV, 1)
+    for lvl in range(LV):
+        L.orc_copy_alpha(C.byref(O.vol_struct(rad, RES, LV)), C.byref(O.vol_struct(opa, RES, LV)), lvl, 6)
+    for lvl in range(1, LV):
+        L.orc_downsample_radiance(C.byref(O.vol_struct(rad, RES, LV)), (C.c_int * 3)(*regions[lvl - 1][0]), lvl, 10)
+    t_passes = time.perf_counter() - t0
+    vdesc = st["vct"].desc()
+    fields = dict(volume_center=list(vdesc.volume_center), rendering_mode=8, voxel_size=vdesc.voxel_size,
+                  volume_dimension=128.0, trace_start_offset=1.0, indirect_diffuse_intensity=8.0,
+                  ambient_occlusion_factor=0.5, min_trace_step_factor=1.0, indirect_specular_intensity=3.0,
+                  occlusion_decay=2.0, enable_32_cones=0)
+    p = O.trace_params(fields, st["vp_inv"], st["eye"], LV)
+    W, H = st["W"], st["H"]
+    rd = np.zeros((H, W, 4), np.float32)
+    rs = np.zeros_like(rd)
+    step = max(1, H // rows)
+    row_list = list(range(step // 2, H, step))[:rows]
+    og = O.gbuf_struct(g)
+    t1 = time.perf_counter()
+    nsamp = 0
+    for r in row_list:
+        n = C.c_uint64(0)
+        L.orc_trace(C.byref(O.vol_struct(rad, RES, LV)), C.byref(og), C.byref(p), C.byref(osm), C.byref(ol), 0x3F,
+                    r, r + 1, O.vptr(rd), O.vptr(rs), C.byref(n))
+        nsamp += n.value
+    t_trace = time.perf_counter() - t1
+    full_ms = (t_passes + t_trace * H / len(row_list)) * 1e3
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=round(full_ms, 1), unit="ms/frame", cores=threads, kind="port",
+                sample=f"all 6-level voxel passes ({t_passes:.2f} s) + {len(row_list)} of {H} trace rows "
+                       f"({t_trace:.2f} s, {nsamp} cone samples), scaled to the full 1920x1080 frame",
+                cone_samples_per_s=round(nsamp / t_trace, 1) if t_trace > 0 else None)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    torch.cuda.set_device(0)
+    st = setup()
+    for _ in range(args.warmup):
+        frame(st)
+    torch.cuda.synchronize()
+    # timed region: exactly K frames, inputs resident in HBM
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame(st)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_frame = elapsed * 1e3 / args.steps
+    # per-pass device times of one more frame, and the samples it executed
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    trace_ms = []
+    for _ in range(3):
+        frame(st, ev)
+        torch.cuda.synchronize()
+        trace_ms.append(ev[2].elapsed_time(ev[3]))
+    vox_ms = ev[0].elapsed_time(ev[1])
+    inj_ms = ev[1].elapsed_time(ev[2])
+    t_trace = sorted(trace_ms)[1]
+    samples = int(st["vct"].samples.item())
+    pixels = st["W"] * st["H"]
+    alg_bytes = samples * B_SAMPLE_RGBA16F + pixels * B_PIXEL
+    achieved = alg_bytes / (t_trace * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "trace_clipmap_1080p_traffic.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": METRIC,
+        "value": round(ms_per_frame, 3),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_frame, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "clipmap_gi_1080p: 6x128^3 clipmap (RGBA16F radiance, RGBA8 opacity), 2048^2 shadow map, "
+                        "1920x1080 CombinedGI trace (16 diffuse + 1 specular cone), seeded room x16, 21260 tris",
+            "frame_passes": "voxelize+opacity mips+border, inject+copy-alpha+radiance mips (all levels), trace",
+        },
+        "cone_samples_per_frame": samples,
+        "cone_samples_per_s": round(samples / (t_trace * 1e-3), 1),
+        "passes_ms": {"voxelization": round(vox_ms, 3), "radiance_injection": round(inj_ms, 3),
+                      "cone_trace": round(t_trace, 3)},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_trace (cone trace)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": alg_bytes,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(st, args.cpu_rows)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

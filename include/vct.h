@@ -328,6 +328,20 @@ vct_status vct_read_volume(vct_ctx* ctx, vct_stream stream, vct_volume volume,
 vct_status vct_write_volume(vct_ctx* ctx, vct_stream stream, vct_volume volume,
                             const void* src, size_t src_bytes);
 
+/* ---- input producers (no reference counterpart on the hot path) ----------
+ * The reference rasterizes its G-buffer (GBufferPass.cpp:170-206) and light
+ * depth map (ReflectiveShadowMapPass.cpp:38-88) before the GI passes run;
+ * these ray-cast equivalents produce the same channel layouts on the device
+ * so that a frame can be driven without a rasterizer. `surface` holds five
+ * floats per material: perceptual roughness, metallic, specular rgb. */
+vct_status vct_raycast_gbuffer(vct_ctx* ctx, vct_stream stream, const vct_mesh* mesh, const float* surface,
+                               const vct_camera_ubo* camera, uint32_t* diffuse, uint16_t* normal,
+                               uint32_t* specular, uint16_t* emission, float* depth, uint32_t width,
+                               uint32_t height);
+vct_status vct_raycast_shadow(vct_ctx* ctx, vct_stream stream, const vct_mesh* mesh,
+                              const vct_directional_light_shadow_desc* light_shadow, float* depth,
+                              uint32_t width, uint32_t height);
+
 #ifdef __cplusplus
 }
 #endif

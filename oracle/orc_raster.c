@@ -376,3 +376,35 @@ void orc_inject_radiance(orc_volume* vol, const orc_tris* tris, const orc_materi
         free(c.acc);
     }
 }
+
+/* ---------------- test hook: list covered samples ---------------- */
+
+typedef struct {
+    float* out; /* 4 floats per sample: pos[3], triangle index */
+    uint32_t cap;
+    uint32_t n;
+} list_ctx;
+
+static void list_frag(void* user, uint32_t tri, const float pos[3], const double lam[3])
+{
+    (void)lam;
+    list_ctx* c = (list_ctx*)user;
+    if (c->n < c->cap) {
+        float* o = c->out + 4u * c->n;
+        o[0] = pos[0];
+        o[1] = pos[1];
+        o[2] = pos[2];
+        o[3] = (float)tri;
+    }
+    c->n++;
+}
+
+/* Runs the coverage rule over `count` triangles and writes every covered
+ * sample's interpolated position (and triangle index); returns the count. */
+uint32_t orc_raster_list(const float* pos, uint32_t count, float voxel_size, const int lo[3], const int hi[3],
+                         int samples, float* out, uint32_t cap)
+{
+    list_ctx c = {out, cap, 0};
+    for (uint32_t t = 0; t < count; ++t) orc_raster_triangle(pos + 9u * t, voxel_size, lo, hi, samples, list_frag, &c, t);
+    return c.n;
+}

@@ -298,3 +298,13 @@ void orc_svo_trace(const uint32_t* pool, const orc_gbuffer* g, const orc_trace_p
         }
     if (samples) *samples = total;
 }
+
+/* test hook: one sampleClipmap of voxelConeTracing.frag:313-327 */
+void orc_sample_clipmap(const orc_volume* vol, const orc_trace_params* p, const float pos[3], int level,
+                        const float dir[3], float out[4])
+{
+    float face_off[3] = {(dir[0] > 0 ? 0.0f : 1.0f) / 6.0f, (dir[1] > 0 ? 2.0f : 3.0f) / 6.0f,
+                         (dir[2] > 0 ? 4.0f : 5.0f) / 6.0f};
+    float w[3] = {dir[0] * dir[0], dir[1] * dir[1], dir[2] * dir[2]};
+    sample_clipmap_f32(vol, p, pos, level, face_off, w, out);
+}

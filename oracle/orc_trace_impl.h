@@ -335,7 +335,7 @@ static void FN(trace_rows)(const orc_volume* vol, const orc_gbuffer* g, const or
             R direct[3] = {0, 0, 0};
             if (has_emission) {
                 for (int k = 0; k < 3; ++k) direct[k] = emission[k];
-            } else {
+            } else if (sm && L) {
                 R l[3] = {-(R)L->direction[0], -(R)L->direction[1], -(R)L->direction[2]};
                 FN(normalize)(l);
                 R b[3];

@@ -165,6 +165,12 @@ void orc_brdf(const float n[3], const float v[3], const float l[3], const float 
               const float specular_color[3], float perceptual_roughness, float metallic,
               float out[3]);                                /* brdf.glsl:65-78 */
 
+/* test hooks */
+uint32_t orc_raster_list(const float* pos, uint32_t count, float voxel_size, const int lo[3], const int hi[3],
+                         int samples, float* out, uint32_t cap);
+void orc_sample_clipmap(const orc_volume* vol, const orc_trace_params* p, const float pos[3], int level,
+                        const float dir[3], float out[4]);
+
 /* ---- G-buffer / shadow producers (ray cast) ---- */
 void orc_raycast_gbuffer(const orc_tris* tris, const orc_material* mats,
                          const float* roughness_metal, /* per material: perceptual roughness, metallic, spec rgb */

@@ -20,7 +20,13 @@ FMT_RGBA16F = 1
 
 
 def build_oracle() -> Path:
-    if not ORACLE_LIB.exists():
+    # make is a no-op when the library is newer than its sources
+    if ORACLE_LIB.exists():
+        try:
+            subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True, capture_output=True)
+        except (OSError, subprocess.CalledProcessError):
+            pass
+    else:
         subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
     return ORACLE_LIB
 

@@ -248,6 +248,10 @@ _PROTOS = {
         [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(ConeTracingDesc), C.POINTER(ShadowMap),
          C.POINTER(DirectionalLightDesc), C.POINTER(DirectionalLightShadowDesc), P, P, P],
     ),
+    "vct_raycast_gbuffer": (
+        C.c_int, [P, P, C.POINTER(Mesh), P, C.POINTER(CameraUBO), P, P, P, P, P, C.c_uint32, C.c_uint32]
+    ),
+    "vct_raycast_shadow": (C.c_int, [P, P, C.POINTER(Mesh), C.POINTER(DirectionalLightShadowDesc), P, C.c_uint32, C.c_uint32]),
     "vct_volume_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_size_t)]),
     "vct_read_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),
     "vct_write_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),
