@@ -49,7 +49,8 @@ typedef enum vct_volume {
 
 typedef enum vct_format {
     VCT_FORMAT_RGBA8 = 0,   /* VK_FORMAT_R8G8B8A8_UNORM, the reference's only format */
-    VCT_FORMAT_RGBA16F = 1  /* half-float radiance (deterministic FP32 accumulate) */
+    VCT_FORMAT_RGBA16F = 1, /* half-float radiance (deterministic FP32 accumulate) */
+    VCT_FORMAT_RGBA32F = 2  /* float grids of the dense method (adjoint) */
 } vct_format;
 
 /* Reference behaviours that are defects or driver-defined. Each bit selects
@@ -79,7 +80,7 @@ enum {
 
 typedef struct vct_config {
     uint32_t voxel_resolution;       /* DEFAULT_VOXEL_RESOLUTION, EngineConfig.h:28 (power of two, 8..256) */
-    uint32_t clip_levels;            /* DEFAULT_CLIP_REGION_COUNT, EngineConfig.h:26 (1..8) */
+    uint32_t clip_levels;            /* DEFAULT_CLIP_REGION_COUNT, EngineConfig.h:26 (1..8; 0 = scratch only) */
     uint32_t radiance_format;        /* vct_format */
     uint32_t quirks;                 /* VCT_QUIRK_* bits */
     uint32_t raster_samples;         /* samples per voxel cell for triangle coverage: 1 or 4 */
@@ -327,6 +328,83 @@ vct_status vct_read_volume(vct_ctx* ctx, vct_stream stream, vct_volume volume,
                            void* dst, size_t dst_bytes);
 vct_status vct_write_volume(vct_ctx* ctx, vct_stream stream, vct_volume volume,
                             const void* src, size_t src_bytes);
+
+/* ---- sparse voxel octree path ------------------------------------------ */
+
+/* K9: SparseVoxelizer::preVoxelize + cmdVoxelize (SparseVoxelizer.h:34-53,
+ * SparseVoxelizer.cpp:248-326): fragment list of uvec2 entries
+ * (x | y<<12 | (z&0xff)<<24, (z>>8)<<28 | RGBA8 & 0x0fffffff) at resolution
+ * 2^octree_level, normalised by mesh->scene_bb_min/max. *nfrag_out receives
+ * getFragmentCount(). */
+vct_status vct_svo_voxelize(vct_ctx* ctx, vct_stream stream, const vct_mesh* mesh, const vct_shadow_map* shadow,
+                            const vct_directional_light_desc* light,
+                            const vct_directional_light_shadow_desc* light_shadow, uint32_t octree_level,
+                            uint32_t* nfrag_out);
+/* K10-K15: OctreeBuilder::cmdBuild (OctreeBuilder.h:22-41, OctreeBuilder.cpp:205-345). */
+vct_status vct_svo_build(vct_ctx* ctx, vct_stream stream);
+/* OctreeBuilder::readOctreeNodes (OctreeBuilder.cpp:359-385): host copy of the
+ * allocated nodes (2 uint32 per node); with out == NULL only *n_nodes is set. */
+vct_status vct_svo_read_nodes(vct_ctx* ctx, vct_stream stream, uint32_t* out, size_t cap_nodes, size_t* n_nodes);
+/* SparseVoxelizer::readVoxelFragments (SparseVoxelizer.cpp:328-353). */
+vct_status vct_svo_read_fragments(vct_ctx* ctx, vct_stream stream, uint32_t* out, size_t cap_fragments,
+                                  size_t* n_fragments);
+/* OctreeBuilder::getOctreeBuffer: device pointer to the node pool. */
+vct_status vct_svo_pool(vct_ctx* ctx, const uint32_t** pool, uint32_t* node_count);
+/* K16: OctreeVoxelConeTracing::onUpdate (OctreeVoxelConeTracing.cpp:74-104)
+ * over the octree built by vct_svo_build. */
+vct_status vct_svo_trace(vct_ctx* ctx, vct_stream stream, const vct_gbuffer* gbuffer, const vct_camera_ubo* camera,
+                         const vct_cone_tracing_desc* desc, const vct_shadow_map* shadow,
+                         const vct_directional_light_desc* light,
+                         const vct_directional_light_shadow_desc* light_shadow, const float scene_bb_min[3],
+                         const float scene_bb_max[3], float* diffuse_out, float* specular_out,
+                         unsigned long long* samples_out);
+
+/* ---- dense "plain" grid ------------------------------------------------
+ * No reference counterpart as a whole: the reference's voxelization,
+ * anisotropic downsample (radianceDownSample.comp:95-139) and cone march
+ * (voxelConeTracing.frag:341-392) applied to one res^3 grid with a 6-face mip
+ * pyramid down to 1^3 instead of a clipmap (level rule
+ * clamp(log2(diameter / voxelSize), 0, levels - 1), zero outside the grid). */
+typedef struct vct_dense vct_dense;
+
+typedef struct vct_dense_desc {
+    float grid_min[3];       /* world position of the grid's minimum corner */
+    float extent;            /* world edge length of the cube */
+    uint32_t resolution;     /* power of two */
+    uint32_t format;         /* VCT_FORMAT_RGBA8 or VCT_FORMAT_RGBA32F */
+    uint32_t raster_samples; /* 1 or 4 */
+    uint32_t reserved;
+} vct_dense_desc;
+
+typedef struct vct_dense_trace_desc {
+    float aperture;                   /* diffuse cone aperture (reference 0.872665 for 16 cones) */
+    float max_distance;               /* MAX_TRACE_DISTANCE */
+    float min_trace_step_factor;      /* max(0.2, .) is used, voxelConeTracing.frag:182 */
+    float trace_start_offset;
+    float occlusion_decay;
+    float ambient_occlusion_factor;
+    float indirect_diffuse_intensity;
+    int32_t enable_32_cones;
+} vct_dense_trace_desc;
+
+vct_status vct_dense_create(vct_dense** out, const vct_dense_desc* desc);
+vct_status vct_dense_destroy(vct_dense* grid);
+vct_status vct_dense_clear(vct_dense* grid, vct_stream stream);
+/* Device pointer, byte size and face count (1 for level 0, else 6) of one
+ * level: faces of (n+2)^3 texels with a zero border, x fastest. */
+vct_status vct_dense_level(vct_dense* grid, uint32_t level, void** ptr, size_t* bytes, uint32_t* faces);
+vct_status vct_dense_read_level(vct_dense* grid, vct_stream stream, uint32_t level, void* dst, size_t bytes);
+vct_status vct_dense_write_level(vct_dense* grid, vct_stream stream, uint32_t level, const void* src, size_t bytes);
+/* K1 on the dense grid: level 0 texels covered by a triangle get
+ * (base colour, 1) of the first covering triangle in draw order. */
+vct_status vct_dense_voxelize(vct_dense* grid, vct_stream stream, const vct_mesh* mesh);
+/* K5 applied level by level: levels 1..log2(res) from level 0. */
+vct_status vct_dense_mipmap(vct_dense* grid, vct_stream stream);
+/* K8's cone march over the dense pyramid: per pixel RGBA32F
+ * (indirect diffuse rgb x albedo x intensity, ambient occlusion). */
+vct_status vct_dense_trace(vct_dense* grid, vct_stream stream, const vct_gbuffer* gbuffer,
+                           const vct_camera_ubo* camera, const vct_dense_trace_desc* desc, float* out,
+                           unsigned long long* samples_out);
 
 /* ---- input producers (no reference counterpart on the hot path) ----------
  * The reference rasterizes its G-buffer (GBufferPass.cpp:170-206) and light

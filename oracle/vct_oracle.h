@@ -191,6 +191,36 @@ void orc_svo_trace(const uint32_t* pool, const orc_gbuffer* g, const orc_trace_p
                    const orc_light* light, uint32_t quirks, int row_begin, int row_end,
                    float* diffuse_out, float* specular_out, uint64_t* samples);
 
+/* ---- dense "plain" grid with a full mip pyramid (orc_dense.c) ---- */
+typedef struct orc_dense {
+    void* levels[16]; /* level 0: 1 face, level m >= 1: 6 faces; (n+2)^3 texels per face, zero border */
+    int fmt;          /* ORC_FMT_RGBA8 or ORC_FMT_RGBA32F */
+    int res;
+    int nlevels;      /* log2(res) + 1 */
+    float grid_min[3];
+    float extent;
+} orc_dense;
+
+typedef struct orc_dense_params {
+    float aperture;
+    float max_dist;
+    float step_factor;
+    float start_offset;
+    float decay;
+    float ao_factor;
+    float diffuse_intensity;
+    int cones32;
+    float view_proj_inv[16];
+} orc_dense_params;
+
+enum { ORC_FMT_RGBA32F = 2 };
+
+size_t orc_dense_level_texels(int res, int level);
+void orc_dense_voxelize(orc_dense* g, const orc_tris* tris, const orc_material* mats, int samples);
+void orc_dense_mip(orc_dense* g);
+void orc_dense_trace(const orc_dense* g, const orc_gbuffer* gb, const orc_dense_params* p, int row_begin,
+                     int row_end, double* out, uint64_t* samples);
+
 #ifdef __cplusplus
 }
 #endif

@@ -252,3 +252,71 @@ def trace_params(desc_fields: dict, vp_inv, eye, levels) -> TraceParams:
     p.view_proj_inv = (C.c_float * 16)(*glm.flat(vp_inv).tolist())
     p.eye_pos = (C.c_float * 3)(*np.asarray(eye, np.float32).tolist())
     return p
+
+
+# --------------------------------------------------------------------------
+# dense "plain" grid
+
+FMT_RGBA32F = 2
+
+
+class Dense(C.Structure):
+    _fields_ = [
+        ("levels", C.c_void_p * 16),
+        ("fmt", C.c_int),
+        ("res", C.c_int),
+        ("nlevels", C.c_int),
+        ("grid_min", C.c_float * 3),
+        ("extent", C.c_float),
+    ]
+
+
+class DenseParams(C.Structure):
+    _fields_ = [
+        ("aperture", C.c_float),
+        ("max_dist", C.c_float),
+        ("step_factor", C.c_float),
+        ("start_offset", C.c_float),
+        ("decay", C.c_float),
+        ("ao_factor", C.c_float),
+        ("diffuse_intensity", C.c_float),
+        ("cones32", C.c_int),
+        ("view_proj_inv", C.c_float * 16),
+    ]
+
+
+def new_dense(res: int, fmt: int = FMT_RGBA8, grid_min=(0.0, 0.0, 0.0), extent: float = 1.0):
+    """Returns (struct, [level arrays]) with the oracle's bordered layout."""
+    nlev = int(np.log2(res)) + 1
+    arrays = []
+    for m in range(nlev):
+        n = res >> m
+        faces = 1 if m == 0 else 6
+        shape = (faces, n + 2, n + 2, n + 2)
+        arrays.append(np.zeros(shape, np.uint32) if fmt == FMT_RGBA8 else np.zeros(shape + (4,), np.float32))
+    d = Dense()
+    for m, a in enumerate(arrays):
+        d.levels[m] = a.ctypes.data
+    d.fmt = fmt
+    d.res = res
+    d.nlevels = nlev
+    d.grid_min = (C.c_float * 3)(*grid_min)
+    d.extent = extent
+    return d, arrays
+
+
+def dense_params(vp_inv, aperture=0.872665, max_dist=1.7320508, step_factor=1.0, start_offset=1.0, decay=2.0,
+                 ao_factor=0.5, diffuse_intensity=8.0, cones32=0):
+    from vk_voxel_cone_tracing_amd import glm
+
+    p = DenseParams()
+    p.aperture = aperture
+    p.max_dist = max_dist
+    p.step_factor = max(0.2, step_factor)
+    p.start_offset = start_offset
+    p.decay = decay
+    p.ao_factor = ao_factor
+    p.diffuse_intensity = diffuse_intensity
+    p.cones32 = cones32
+    p.view_proj_inv = (C.c_float * 16)(*glm.flat(vp_inv).tolist())
+    return p

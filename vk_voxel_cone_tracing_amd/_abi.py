@@ -252,6 +252,21 @@ _PROTOS = {
         C.c_int, [P, P, C.POINTER(Mesh), P, C.POINTER(CameraUBO), P, P, P, P, P, C.c_uint32, C.c_uint32]
     ),
     "vct_raycast_shadow": (C.c_int, [P, P, C.POINTER(Mesh), C.POINTER(DirectionalLightShadowDesc), P, C.c_uint32, C.c_uint32]),
+    "vct_svo_voxelize": (
+        C.c_int,
+        [P, P, C.POINTER(Mesh), C.POINTER(ShadowMap), C.POINTER(DirectionalLightDesc),
+         C.POINTER(DirectionalLightShadowDesc), C.c_uint32, C.POINTER(C.c_uint32)],
+    ),
+    "vct_svo_build": (C.c_int, [P, P]),
+    "vct_svo_read_nodes": (C.c_int, [P, P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vct_svo_read_fragments": (C.c_int, [P, P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "vct_svo_pool": (C.c_int, [P, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
+    "vct_svo_trace": (
+        C.c_int,
+        [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(ConeTracingDesc), C.POINTER(ShadowMap),
+         C.POINTER(DirectionalLightDesc), C.POINTER(DirectionalLightShadowDesc), C.POINTER(C.c_float),
+         C.POINTER(C.c_float), P, P, P],
+    ),
     "vct_volume_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_size_t)]),
     "vct_read_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),
     "vct_write_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),

@@ -247,12 +247,12 @@ vct_status vct_create(vct_ctx** out_ctx, const vct_config* cfg)
     const uint32_t r = cfg->voxel_resolution;
     if (r < 8 || r > 256 || (r & (r - 1)))
         return fail(VCT_ERR_INVALID_ARGUMENT, "voxel_resolution must be a power of two in [8, 256]");
-    if (cfg->clip_levels < 1 || cfg->clip_levels > (uint32_t)kMaxLevels)
-        return fail(VCT_ERR_INVALID_ARGUMENT, "clip_levels must be in [1, %d]", kMaxLevels);
+    if (cfg->clip_levels > (uint32_t)kMaxLevels)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "clip_levels must be in [0, %d]", kMaxLevels);
     if (cfg->radiance_format > VCT_FORMAT_RGBA16F) return fail(VCT_ERR_INVALID_ARGUMENT, "bad radiance_format");
     if (cfg->raster_samples != 1 && cfg->raster_samples != 4)
         return fail(VCT_ERR_INVALID_ARGUMENT, "raster_samples must be 1 or 4");
-    if (cfg->downsample_region_size >= r / 4)
+    if (cfg->clip_levels && cfg->downsample_region_size >= r / 4)
         return fail(VCT_ERR_INVALID_ARGUMENT, "downsample_region_size must be below resolution / 4");
     if (!(cfg->extent_l0 > 0.0f)) return fail(VCT_ERR_INVALID_ARGUMENT, "extent_l0 must be positive");
     int dev = 0;
@@ -264,9 +264,10 @@ vct_status vct_create(vct_ctx** out_ctx, const vct_config* cfg)
     c->D = (int)r + 2;
     c->levels = (int)cfg->clip_levels;
     c->device = dev;
-    vct_status s = ensure(c->opacity, texels_per_volume(c) * 4u);
-    if (s == VCT_OK) s = ensure(c->radiance, texels_per_volume(c) * texel_bytes(c, VCT_VOLUME_RADIANCE));
-    if (s == VCT_OK) {
+    vct_status s = VCT_OK;
+    if (c->levels) s = ensure(c->opacity, texels_per_volume(c) * 4u);
+    if (s == VCT_OK && c->levels) s = ensure(c->radiance, texels_per_volume(c) * texel_bytes(c, VCT_VOLUME_RADIANCE));
+    if (s == VCT_OK && c->levels) {
         hipError_t e = hipMemset(c->opacity.ptr, 0, texels_per_volume(c) * 4u);
         if (e == hipSuccess) e = hipMemset(c->radiance.ptr, 0, texels_per_volume(c) * texel_bytes(c, VCT_VOLUME_RADIANCE));
         if (e == hipSuccess) e = hipDeviceSynchronize();

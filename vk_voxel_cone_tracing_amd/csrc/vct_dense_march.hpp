@@ -1,0 +1,268 @@
+// vct_dense_march.hpp - the cone march over the dense mip pyramid, shared by
+// the forward trace and the adjoint (which recomputes the march per cone).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "vct_dense.hpp"
+#include "vct_texel.hpp"
+
+namespace vct {
+namespace dense {
+
+constexpr int kMaxLevels = 11;
+
+struct Pyramid {
+    unsigned long long offset[kMaxLevels];  // texel offset of each level
+    unsigned long long block[kMaxLevels];   // texels per face block
+    float inv_vs[kMaxLevels];               // n_m / extent
+    int n[kMaxLevels];
+    int nlevels;
+    float vs;  // level-0 voxel size
+};
+
+struct March {
+    float coef;  // 2 tan(aperture / 2)
+    float max_dist;
+    float step_factor;
+    float start_offset;
+    float decay;
+    float ao_factor;
+    float diffuse_intensity;
+    int ncones;
+    int cones32;
+    float vp_inv[16];
+    float grid_min[3];
+};
+
+__constant__ float c_dense_dirs16[16][3] = {
+    {0.57735f, 0.57735f, 0.57735f},       {0.57735f, -0.57735f, -0.57735f},
+    {-0.57735f, 0.57735f, -0.57735f},     {-0.57735f, -0.57735f, 0.57735f},
+    {-0.903007f, -0.182696f, -0.388844f}, {-0.903007f, 0.182696f, 0.388844f},
+    {0.903007f, -0.182696f, 0.388844f},   {0.903007f, 0.182696f, -0.388844f},
+    {-0.388844f, -0.903007f, -0.182696f}, {0.388844f, -0.903007f, 0.182696f},
+    {0.388844f, 0.903007f, -0.182696f},   {-0.388844f, 0.903007f, 0.182696f},
+    {-0.182696f, -0.388844f, -0.903007f}, {0.182696f, 0.388844f, -0.903007f},
+    {-0.182696f, 0.388844f, 0.903007f},   {0.182696f, -0.388844f, 0.903007f}};
+
+__constant__ float c_dense_dirs32[32][3] = {
+    {0.898904f, 0.435512f, 0.0479745f},   {0.898904f, -0.435512f, -0.0479745f},
+    {0.898904f, 0.0479745f, -0.435512f},  {0.898904f, -0.0479745f, 0.435512f},
+    {-0.898904f, 0.435512f, -0.0479745f}, {-0.898904f, -0.435512f, 0.0479745f},
+    {-0.898904f, 0.0479745f, 0.435512f},  {-0.898904f, -0.0479745f, -0.435512f},
+    {0.0479745f, 0.898904f, 0.435512f},   {-0.0479745f, 0.898904f, -0.435512f},
+    {-0.435512f, 0.898904f, 0.0479745f},  {0.435512f, 0.898904f, -0.0479745f},
+    {-0.0479745f, -0.898904f, 0.435512f}, {0.0479745f, -0.898904f, -0.435512f},
+    {0.435512f, -0.898904f, 0.0479745f},  {-0.435512f, -0.898904f, -0.0479745f},
+    {0.435512f, 0.0479745f, 0.898904f},   {-0.435512f, -0.0479745f, 0.898904f},
+    {0.0479745f, -0.435512f, 0.898904f},  {-0.0479745f, 0.435512f, 0.898904f},
+    {0.435512f, -0.0479745f, -0.898904f}, {-0.435512f, 0.0479745f, -0.898904f},
+    {0.0479745f, 0.435512f, -0.898904f},  {-0.0479745f, -0.435512f, -0.898904f},
+    {0.57735f, 0.57735f, 0.57735f},       {0.57735f, 0.57735f, -0.57735f},
+    {0.57735f, -0.57735f, 0.57735f},      {0.57735f, -0.57735f, -0.57735f},
+    {-0.57735f, 0.57735f, 0.57735f},      {-0.57735f, 0.57735f, -0.57735f},
+    {-0.57735f, -0.57735f, 0.57735f},     {-0.57735f, -0.57735f, -0.57735f}};
+
+__device__ __forceinline__ const float* cone_dir(const March& M, int i)
+{
+    return M.cones32 ? c_dense_dirs32[i] : c_dense_dirs16[i];
+}
+
+template <bool kF32>
+__device__ __forceinline__ f4 texel(const void* __restrict__ data, unsigned long long i)
+{
+    if (kF32) {
+        const float4 t = ((const float4*)data)[i];
+        return {t.x, t.y, t.z, t.w};
+    }
+    const uint32_t t = ((const uint32_t*)data)[i];
+    const float s = 1.0f / 255.0f;
+    return {(float)(t & 0xffu) * s, (float)((t >> 8) & 0xffu) * s, (float)((t >> 16) & 0xffu) * s, (float)(t >> 24) * s};
+}
+
+// Footprint of one trilinear fetch of level m (bordered indices).
+struct Footprint {
+    unsigned long long base;  // texel index of the (0,0,0) corner inside face 0
+    unsigned long long dy, dz;
+    float fx, fy, fz;
+    bool valid;
+};
+
+__device__ __forceinline__ Footprint footprint(const Pyramid& P, int m, const float p[3])
+{
+    Footprint F;
+    const int n = P.n[m];
+    const unsigned long long D = (unsigned long long)n + 2;
+    int i0[3];
+    float fr[3];
+    F.valid = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float u = p[k] * P.inv_vs[m] - 0.5f;
+        const float f = floorf(u);
+        fr[k] = u - f;
+        if (!(f >= -1.0f) || f > (float)(n - 1)) F.valid = false;
+        int i = (int)f + 1;
+        i0[k] = i < 0 ? 0 : (i > n ? n : i);
+    }
+    F.base = P.offset[m] + ((unsigned long long)i0[2] * D + (unsigned long long)i0[1]) * D + (unsigned long long)i0[0];
+    F.dy = D;
+    F.dz = D * D;
+    F.fx = fr[0];
+    F.fy = fr[1];
+    F.fz = fr[2];
+    return F;
+}
+
+template <bool kF32>
+__device__ __forceinline__ f4 trilinear(const void* __restrict__ data, const Footprint& F, unsigned long long face_off)
+{
+    const unsigned long long b = F.base + face_off;
+    const f4 t000 = texel<kF32>(data, b), t100 = texel<kF32>(data, b + 1);
+    const f4 t010 = texel<kF32>(data, b + F.dy), t110 = texel<kF32>(data, b + F.dy + 1);
+    const f4 t001 = texel<kF32>(data, b + F.dz), t101 = texel<kF32>(data, b + F.dz + 1);
+    const f4 t011 = texel<kF32>(data, b + F.dz + F.dy), t111 = texel<kF32>(data, b + F.dz + F.dy + 1);
+    const float ax = F.fx, ay = F.fy, az = F.fz;
+    const float w000 = (1.0f - ax) * (1.0f - ay) * (1.0f - az), w100 = ax * (1.0f - ay) * (1.0f - az);
+    const float w010 = (1.0f - ax) * ay * (1.0f - az), w110 = ax * ay * (1.0f - az);
+    const float w001 = (1.0f - ax) * (1.0f - ay) * az, w101 = ax * (1.0f - ay) * az;
+    const float w011 = (1.0f - ax) * ay * az, w111 = ax * ay * az;
+    f4 r;
+#define VCT_T(c) r.c = w000 * t000.c + w100 * t100.c + w010 * t010.c + w110 * t110.c + w001 * t001.c + w101 * t101.c + \
+                       w011 * t011.c + w111 * t111.c
+    VCT_T(x);
+    VCT_T(y);
+    VCT_T(z);
+    VCT_T(w);
+#undef VCT_T
+    return r;
+}
+
+// The three-face sample of level m (one face at level 0).
+template <bool kF32>
+__device__ __forceinline__ f4 sample(const void* __restrict__ data, const Pyramid& P, int m, const float p[3],
+                                     const int face[3], const float w[3])
+{
+    const Footprint F = footprint(P, m, p);
+    if (!F.valid) return {0.0f, 0.0f, 0.0f, 0.0f};
+    if (m == 0) {
+        const f4 t = trilinear<kF32>(data, F, 0);
+        const float ws = w[0] + w[1] + w[2];
+        return {t.x * ws, t.y * ws, t.z * ws, t.w * ws};
+    }
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const f4 t = trilinear<kF32>(data, F, (unsigned long long)face[a] * P.block[m]);
+        acc.x += t.x * w[a];
+        acc.y += t.y * w[a];
+        acc.z += t.z * w[a];
+        acc.w += t.w * w[a];
+    }
+    return acc;
+}
+
+// traceCone (voxelConeTracing.frag:341-392) with the dense level rule.
+template <bool kF32>
+__device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March& M, const float start_in[3],
+                    const float dir[3], uint32_t& steps)
+{
+    const float vs = P.vs;
+    float start[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) start[k] = start_in[k] + dir[k] * vs * M.start_offset * 0.5f;
+    float step = 0.0f, diameter = vs, occlusion = 0.0f, seg = vs;
+    f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int face[3] = {dir[0] > 0.0f ? 0 : 1, dir[1] > 0.0f ? 2 : 3, dir[2] > 0.0f ? 4 : 5};
+    const float w[3] = {dir[0] * dir[0], dir[1] * dir[1], dir[2] * dir[2]};
+    const float max_level = (float)(P.nlevels - 1);
+    const float inv_vs = 1.0f / vs;
+    while (step < M.max_dist && occlusion < 1.0f) {
+        float pos[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) pos[k] = start[k] + dir[k] * step;
+        const float lvl = fminf(fmaxf(log2f(diameter * inv_vs), 0.0f), max_level);
+        const int lo = (int)floorf(lvl), hi = (int)ceilf(lvl);
+        const float fl = lvl - floorf(lvl);
+        f4 s = sample<kF32>(data, P, lo, pos, face, w);
+        if (hi != lo) {
+            const f4 b = sample<kF32>(data, P, hi, pos, face, w);
+            s.x = s.x * (1.0f - fl) + b.x * fl;
+            s.y = s.y * (1.0f - fl) + b.y * fl;
+            s.z = s.z * (1.0f - fl) + b.z * fl;
+            s.w = s.w * (1.0f - fl) + b.w * fl;
+        }
+        const float vsl = vs * exp2f(lvl);
+        const float corr = seg / vsl;
+        const float op = fminf(fmaxf(1.0f - powf(1.0f - s.w, corr), 0.0f), 1.0f);
+        const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
+        result.x += blend * (s.x * corr);
+        result.y += blend * (s.y * corr);
+        result.z += blend * (s.z * corr);
+        result.w += blend * op;
+        occlusion += (1.0f - occlusion) * op / (1.0f + (step + vsl) * M.decay);
+        const float prev = step;
+        step += fmaxf(diameter, vs) * M.step_factor;
+        seg = step - prev;
+        diameter = step * M.coef;
+        ++steps;
+    }
+    return {result.x, result.y, result.z, 1.0f - occlusion};
+}
+
+__device__ __forceinline__ void pixel_setup(const Pyramid& P, const March& M, int px, int py, int width, int height,
+                                            float depth, const uint16_t* __restrict__ nrm, uint32_t diffuse,
+                                            float start[3], float normal[3], float albedo[3])
+{
+    const float q[4] = {(((float)px + 0.5f) / (float)width) * 2.0f - 1.0f,
+                        (((float)py + 0.5f) / (float)height) * 2.0f - 1.0f, depth, 1.0f};
+    float wp[4];
+    for (int k = 0; k < 4; ++k)
+        wp[k] = M.vp_inv[k] * q[0] + M.vp_inv[4 + k] * q[1] + M.vp_inv[8 + k] * q[2] + M.vp_inv[12 + k] * q[3];
+    const f4 nf = unpack_half(*(const uint2*)nrm);
+    normal[0] = nf.x * 2.0f - 1.0f;
+    normal[1] = nf.y * 2.0f - 1.0f;
+    normal[2] = nf.z * 2.0f - 1.0f;
+    const float l = sqrtf(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+    for (int k = 0; k < 3; ++k) {
+        normal[k] /= l;
+        const float world = wp[k] / wp[3] - M.grid_min[k];
+        start[k] = world + normal[k] * P.vs * M.start_offset;
+    }
+    albedo[0] = u8_to_f(diffuse & 0xffu);
+    albedo[1] = u8_to_f((diffuse >> 8) & 0xffu);
+    albedo[2] = u8_to_f((diffuse >> 16) & 0xffu);
+}
+
+inline vct_status make_params(const vct_dense* g, const vct_camera_ubo* cam, const vct_dense_trace_desc* d,
+                              Pyramid* P, March* M)
+{
+    if (g->nlevels > kMaxLevels) return fail(VCT_ERR_UNSUPPORTED, "too many levels");
+    if (!(d->aperture > 0.0f) || !(d->max_distance > 0.0f))
+        return fail(VCT_ERR_INVALID_ARGUMENT, "aperture and max_distance must be positive");
+    for (int m = 0; m < g->nlevels; ++m) {
+        P->offset[m] = g->offset[m];
+        P->block[m] = g->block[m];
+        P->n[m] = g->res >> m;
+        P->inv_vs[m] = (float)(g->res >> m) / g->desc.extent;
+    }
+    P->nlevels = g->nlevels;
+    P->vs = g->desc.extent / (float)g->res;
+    M->coef = 2.0f * std::tan(d->aperture * 0.5f);
+    M->max_dist = d->max_distance;
+    M->step_factor = d->min_trace_step_factor > 0.2f ? d->min_trace_step_factor : 0.2f;
+    M->start_offset = d->trace_start_offset;
+    M->decay = d->occlusion_decay;
+    M->ao_factor = d->ambient_occlusion_factor;
+    M->diffuse_intensity = d->indirect_diffuse_intensity;
+    M->cones32 = d->enable_32_cones != 0;
+    M->ncones = M->cones32 ? 32 : 16;
+    for (int k = 0; k < 16; ++k) M->vp_inv[k] = cam->view_proj_inv[k];
+    for (int k = 0; k < 3; ++k) M->grid_min[k] = g->desc.grid_min[k];
+    return VCT_OK;
+}
+
+}  // namespace dense
+}  // namespace vct

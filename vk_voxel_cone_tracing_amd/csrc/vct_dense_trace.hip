@@ -1,0 +1,84 @@
+// vct_dense_trace.hip - cone tracing over the dense mip pyramid (gfx950).
+//
+// Per pixel: the reference's 16 (or 32) diffuse cones (voxelConeTracing.frag
+// :166-203) marched through the dense pyramid; output RGBA32F =
+// (indirect diffuse x albedo x intensity, ambient occlusion). Level 0 has one
+// face, so its sample is one trilinear fetch scaled by the weight sum; levels
+// >= 1 blend the three direction-selected faces. Every face block has a zero
+// border, so only positions more than one texel outside the grid need a test.
+#include <hip/hip_runtime.h>
+
+#include "vct_dense.hpp"
+#include "vct_dense_march.hpp"
+
+namespace vct {
+namespace dense {
+
+template <bool kF32>
+__global__ __launch_bounds__(256) void k_dense_trace(const void* __restrict__ data, Pyramid P, March M,
+                                                     const uint32_t* __restrict__ g_diffuse,
+                                                     const uint16_t* __restrict__ g_normal,
+                                                     const float* __restrict__ g_depth, int width, int height,
+                                                     float4* __restrict__ out, unsigned long long* __restrict__ samples)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    uint32_t steps = 0;
+    if (px < width && py < height) {
+        const size_t idx = (size_t)py * (size_t)width + (size_t)px;
+        float4 o = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        const float depth = g_depth[idx];
+        if (depth != 1.0f) {
+            float start[3], normal[3], albedo[3];
+            pixel_setup(P, M, px, py, width, height, depth, g_normal + 4 * idx, g_diffuse[idx], start, normal, albedo);
+            f4 ind = {0.0f, 0.0f, 0.0f, 1.0f};
+            for (int i = 0; i < M.ncones; ++i) {
+                const float* d = cone_dir(M, i);
+                const float ct = normal[0] * d[0] + normal[1] * d[1] + normal[2] * d[2];
+                if (ct < 0.0f) continue;
+                const f4 c = march<kF32>(data, P, M, start, d, steps);
+                ind.x += c.x * ct;
+                ind.y += c.y * ct;
+                ind.z += c.z * ct;
+                ind.w += c.w * ct;
+            }
+            const float inv = 1.0f / (float)M.ncones;
+            o = make_float4(ind.x * inv * albedo[0] * M.diffuse_intensity, ind.y * inv * albedo[1] * M.diffuse_intensity,
+                            ind.z * inv * albedo[2] * M.diffuse_intensity, ind.w * inv * M.ao_factor);
+        }
+        out[idx] = o;
+    }
+    if (samples) {
+        unsigned long long v = steps;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0 && v) atomicAdd(samples, v);
+    }
+}
+
+}  // namespace dense
+}  // namespace vct
+
+using namespace vct;
+
+extern "C" vct_status vct_dense_trace(vct_dense* g, vct_stream stream, const vct_gbuffer* gb,
+                                      const vct_camera_ubo* cam, const vct_dense_trace_desc* d, float* out,
+                                      unsigned long long* samples_out)
+{
+    if (!g || !gb || !cam || !d || !out) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!gb->diffuse || !gb->normal || !gb->depth || !gb->width || !gb->height)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "G-buffer needs diffuse, normal and depth");
+    dense::Pyramid P;
+    dense::March M;
+    VCT_TRY(dense::make_params(g, cam, d, &P, &M));
+    dim3 grid((gb->width + 15) / 16, (gb->height + 15) / 16);
+    hipStream_t s = (hipStream_t)stream;
+    if (g->texel_bytes == 16)
+        hipLaunchKernelGGL(dense::k_dense_trace<true>, grid, dim3(256), 0, s, g->data.ptr, P, M, gb->diffuse,
+                           gb->normal, gb->depth, (int)gb->width, (int)gb->height, (float4*)out, samples_out);
+    else
+        hipLaunchKernelGGL(dense::k_dense_trace<false>, grid, dim3(256), 0, s, g->data.ptr, P, M, gb->diffuse,
+                           gb->normal, gb->depth, (int)gb->width, (int)gb->height, (float4*)out, samples_out);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}

@@ -1,0 +1,536 @@
+// vct_svo.hip - sparse voxel octree path for gfx950: fragment-list
+// voxelization (K9), the per-level node init/flag/alloc build (K10-K13), leaf
+// and mipmap colour writes (K14/K15) and octree cone tracing (K16).
+//
+// Determinism. The reference hands out fragment-list slots and child blocks
+// with atomicAdd, so its layout depends on invocation order. Here:
+//  * every covered raster sample gets its slot from an exclusive scan over the
+//    (triangle, cell, sample) work items, i.e. rasterization order;
+//  * child blocks are handed to the flagged nodes of a level in pool-index
+//    order with a wavefront ballot/popcount prefix scan (three passes: wave
+//    counts, block offsets, assignment), which is the reference's outcome
+//    when invocations run in index order;
+//  * the order-dependent leaf running average is applied per leaf in
+//    fragment order after a radix sort keyed by (leaf, fragment).
+// The node pool is therefore a pure function of the fragment list.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <vector>
+
+#include "vct_internal.hpp"
+#include "vct_raster.hpp"
+#include "vct_shadow.hpp"
+#include "vct_texel.hpp"
+
+namespace vct {
+
+struct SvoRaster {
+    float vs;  // 2 / R: one raster cell per voxel of the normalised [-1, 1] box
+    int lo[3], hi[3];
+    int samples;
+    int res;
+    uint32_t quirks;
+};
+
+__global__ void k_svo_normalize(const float* __restrict__ pos, uint32_t n9, float cx, float cy, float cz, float ev,
+                                float* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n9) return;
+    const uint32_t k = i % 3u;
+    const float c = k == 0 ? cx : (k == 1 ? cy : cz);
+    out[i] = (pos[i] - c) / ev;  // voxelizer.vert:42-47
+}
+
+__global__ void k_svo_count(const float* __restrict__ npos, uint32_t ntris, SvoRaster R,
+                            unsigned long long* __restrict__ counts)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntris) return;
+    float p[9];
+    for (int k = 0; k < 9; ++k) p[k] = npos[9u * t + k];
+    TriSetup s;
+    unsigned long long n = 0;
+    if (tri_setup(p, R.vs, R.lo, R.hi, s)) n = (unsigned long long)s.nu * s.nv * (unsigned long long)R.samples;
+    counts[t] = n;
+}
+
+__device__ __forceinline__ uint32_t find_tri64(const unsigned long long* __restrict__ offsets, uint32_t n,
+                                               unsigned long long wi)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (offsets[mid] <= wi) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// covered[wi] = 1 for every covered sample: each is one fragment invocation
+// and consumes one fragment-list slot (voxelizer.frag:55).
+__global__ __launch_bounds__(256) void k_svo_cover(const float* __restrict__ npos,
+                                                   const unsigned long long* __restrict__ offsets, uint32_t ntris,
+                                                   uint32_t nwork, SvoRaster R, uint32_t* __restrict__ covered)
+{
+    const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= nwork) return;
+    const uint32_t t = find_tri64(offsets, ntris, wi);
+    float p[9];
+    for (int k = 0; k < 9; ++k) p[k] = npos[9u * t + k];
+    TriSetup s;
+    uint32_t c = 0;
+    if (tri_setup(p, R.vs, R.lo, R.hi, s)) {
+        float fp[3];
+        double lam[3];
+        c = tri_sample(p, s, R.vs, R.samples, (uint32_t)(wi - offsets[t]), fp, lam) ? 1u : 0u;
+    }
+    covered[wi] = c;
+}
+
+// voxelizer.frag:48-100 for every covered sample, written at its scan slot.
+__global__ __launch_bounds__(256) void k_svo_write(const float* __restrict__ npos, const float* __restrict__ nrm,
+                                                   const uint32_t* __restrict__ mat,
+                                                   const vct_gltf_material* __restrict__ mats,
+                                                   const unsigned long long* __restrict__ offsets, uint32_t ntris,
+                                                   uint32_t nwork, SvoRaster R, LightParams L,
+                                                   const uint32_t* __restrict__ covered,
+                                                   const uint32_t* __restrict__ slot, uint2* __restrict__ frags)
+{
+#pragma clang fp contract(off)
+    const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi >= nwork || !covered[wi]) return;
+    const uint32_t t = find_tri64(offsets, ntris, wi);
+    float p[9];
+    for (int k = 0; k < 9; ++k) p[k] = npos[9u * t + k];
+    TriSetup s;
+    if (!tri_setup(p, R.vs, R.lo, R.hi, s)) return;
+    float fp[3];
+    double lam[3];
+    if (!tri_sample(p, s, R.vs, R.samples, (uint32_t)(wi - offsets[t]), fp, lam)) return;
+    const uint32_t cur = slot[wi];
+    float q[3];
+    for (int k = 0; k < 3; ++k) q[k] = (fp[k] + 1.0f) * 0.5f;  // biasAndScale (voxelizer.geom:27-30)
+    const vct_gltf_material& m = mats[mat[t]];
+    float rad[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    if (m.emissive_factor[0] > 0.0f || m.emissive_factor[1] > 0.0f || m.emissive_factor[2] > 0.0f) {
+        for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(m.emissive_factor[k], 0.0f), 1.0f);
+    } else {
+        float color[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+        if (!(R.quirks & VCT_QUIRK_SVO_UNTEXTURED_BLACK) || m.base_color_texture > -1)
+            for (int k = 0; k < 4; ++k) color[k] = m.base_color_factor[k];
+        const float* n0 = nrm + 9u * t;
+        float ni[3];
+        for (int k = 0; k < 3; ++k)
+            ni[k] = (float)(lam[0] * (double)n0[k] + lam[1] * (double)n0[3 + k] + lam[2] * (double)n0[6 + k]);
+        const float len = sqrtf(ni[0] * ni[0] + ni[1] * ni[1] + ni[2] * ni[2]);
+        const float nn[3] = {ni[0] / len, ni[1] / len, ni[2] / len};
+        float ndl = nn[0] * L.dir[0] + nn[1] * L.dir[1] + nn[2] * L.dir[2];
+        ndl = fminf(fmaxf(ndl, 0.001f), 1.0f);
+        // calcVisibility at fs_in.position, the normalised [0,1] position (voxelizer.frag:86)
+        const float vis = visibility(L, q, R.quirks);
+        float lc[3];
+        for (int k = 0; k < 3; ++k) lc[k] = ndl * vis * L.color[k] * L.intensity;
+        if (lc[0] == 0.0f && lc[1] == 0.0f && lc[2] == 0.0f) return;  // discard: the slot stays zero
+        for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(lc[k] * color[k] * color[3], 0.0f), 1.0f);
+        rad[3] = 1.0f;
+    }
+    uint32_t vp[3];
+    for (int k = 0; k < 3; ++k) {
+        const float f = q[k] * (float)R.res;
+        uint32_t u = f > 0.0f ? (uint32_t)f : 0u;
+        vp[k] = u > (uint32_t)R.res ? (uint32_t)R.res : u;
+    }
+    const f4 v255 = {rad[0] * 255.0f, rad[1] * 255.0f, rad[2] * 255.0f, rad[3] * 255.0f};
+    frags[cur] = make_uint2(vp[0] | (vp[1] << 12u) | ((vp[2] & 0xffu) << 24u),
+                            ((vp[2] >> 8u) << 28u) | (pack_raw(v255) & 0x0fffffffu));
+}
+
+__global__ void k_fill_u2(uint2* __restrict__ p, uint32_t n, uint2 v)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ void frag_pos(uint2 f, bool half, uint32_t p[3])
+{
+    p[0] = f.x & 0xfffu;
+    p[1] = (f.x >> 12u) & 0xfffu;
+    p[2] = ((f.x >> 24u) & 0xffu) | ((f.y >> 20u) & 0xf00u);
+    if (half) {
+        p[0] >>= 1;
+        p[1] >>= 1;
+        p[2] >>= 1;
+    }
+}
+
+// The descent shared by octreeNodeFlag/LeafWrite/MipmapWrite.
+__device__ __forceinline__ uint32_t walk(const uint2* __restrict__ pool, uint32_t p[3], uint32_t res, uint32_t stop,
+                                         uint32_t& res_out)
+{
+    uint32_t idx = 0, cur = 0;
+    do {
+        res >>= 1;
+        const uint32_t cx = p[0] >= res, cy = p[1] >= res, cz = p[2] >= res;
+        idx = cur + (cz | (cx << 1) | (cy << 2));
+        cur = pool[idx].x & 0x7fffffffu;
+        p[0] -= cx * res;
+        p[1] -= cy * res;
+        p[2] -= cz * res;
+    } while (cur != 0u && res > stop);
+    res_out = res;
+    return idx;
+}
+
+// octreeNodeFlag.comp: mark the deepest existing node of every fragment.
+__global__ __launch_bounds__(256) void k_svo_flag(uint2* __restrict__ pool, const uint2* __restrict__ frags,
+                                                  uint32_t nfrag, uint32_t R, bool half)
+{
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfrag) return;
+    uint32_t p[3], r;
+    frag_pos(frags[f], half, p);
+    const uint32_t idx = walk(pool, p, R, 1u, r);
+    atomicOr(&pool[idx].x, 0x80000000u);
+}
+
+// octreeNodeAlloc.comp with ordered allocation. Pass 1: per-wave flagged counts.
+__global__ __launch_bounds__(256) void k_svo_alloc_count(const uint2* __restrict__ pool, uint32_t begin, uint32_t num,
+                                                         uint32_t* __restrict__ wave_counts)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool flagged = i < num && (pool[begin + i].x & 0x80000000u);
+    const unsigned long long ballot = __ballot(flagged);
+    if ((threadIdx.x & 63) == 0) wave_counts[i >> 6] = (uint32_t)__popcll(ballot);
+}
+
+// Pass 3: rank = wave offset + popcount of lower lanes; child block
+// (base + rank + 1) << 3 with the subdivide flag kept.
+__global__ __launch_bounds__(256) void k_svo_alloc_assign(uint2* __restrict__ pool, uint32_t begin, uint32_t num,
+                                                          const uint32_t* __restrict__ wave_offsets, uint32_t base)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool flagged = i < num && (pool[begin + i].x & 0x80000000u);
+    const unsigned long long ballot = __ballot(flagged);
+    if (!flagged) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long lower = lane ? (ballot & (~0ull >> (64 - lane))) : 0ull;
+    const uint32_t rank = wave_offsets[i >> 6] + (uint32_t)__popcll(lower);
+    pool[begin + i] = make_uint2(((base + rank + 1u) << 3u) | 0x80000000u, 0u);
+}
+
+// octreeNodeLeafWrite.comp, ordered: one (leaf, fragment) key per fragment
+// that reaches a leaf; keys of other fragments sort last.
+__global__ __launch_bounds__(256) void k_svo_leaf_keys(const uint2* __restrict__ pool, const uint2* __restrict__ frags,
+                                                       uint32_t nfrag, uint32_t R, bool half,
+                                                       unsigned long long* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals)
+{
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfrag) return;
+    uint32_t p[3], r;
+    frag_pos(frags[f], half, p);
+    const uint32_t idx = walk(pool, p, R, 1u, r);
+    keys[f] = r == 1u ? (((unsigned long long)idx << 32) | f) : ~0ull;
+    vals[f] = f;
+}
+
+__global__ __launch_bounds__(256) void k_svo_leaf_apply(uint2* __restrict__ pool, const uint2* __restrict__ frags,
+                                                        const unsigned long long* __restrict__ keys,
+                                                        const uint32_t* __restrict__ vals, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || keys[i] == ~0ull) return;
+    const unsigned long long leaf = keys[i] >> 32;
+    if (i > 0 && (keys[i - 1] >> 32) == leaf) return;
+    uint32_t t = pool[leaf].y;
+    for (uint32_t j = i; j < n && keys[j] != ~0ull && (keys[j] >> 32) == leaf; ++j)
+        t = rgba8_avg(t, unpack_raw(frags[vals[j]].y & 0x0fffffffu));
+    pool[leaf].y = t;
+}
+
+// octreeNodeMipmapWrite.comp evaluated per node: every flagged node of the
+// depth range gets the truncating average of its eight children. The
+// reference reaches exactly these nodes (the ancestors of fragments) once per
+// fragment and writes the same value each time.
+__global__ __launch_bounds__(256) void k_svo_mip(uint2* __restrict__ pool, uint32_t begin, uint32_t num)
+{
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num) return;
+    const uint2 node = pool[begin + i];
+    if (!(node.x & 0x80000000u)) return;
+    const uint32_t child = node.x & 0x7fffffffu;
+    if (child == 0u) return;
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t j = 0; j < 8; ++j) {
+        const f4 v = unpack_raw(pool[child + j].y);
+        s[0] = s[0] + v.x / 8.0f;
+        s[1] = s[1] + v.y / 8.0f;
+        s[2] = s[2] + v.z / 8.0f;
+        s[3] = s[3] + v.w / 8.0f;
+    }
+    pool[begin + i].y = pack_raw(f4{s[0], s[1], s[2], s[3]});
+}
+
+static LightParams make_light(const vct_shadow_map* shadow, const vct_directional_light_desc* light,
+                              const vct_directional_light_shadow_desc* ls)
+{
+    LightParams L{};
+    float nd[3] = {-light->direction[0], -light->direction[1], -light->direction[2]};
+    float len = std::sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+    for (int k = 0; k < 3; ++k) {
+        L.dir[k] = nd[k] / len;
+        L.color[k] = light->color[k];
+    }
+    L.intensity = light->intensity;
+    for (int k = 0; k < 16; ++k) {
+        L.view[k] = ls->view[k];
+        L.proj[k] = ls->proj[k];
+    }
+    L.z_near = ls->z_near;
+    L.z_far = ls->z_far;
+    L.depth = shadow->depth;
+    L.width = (int)shadow->width;
+    L.height = (int)shadow->height;
+    return L;
+}
+
+static unsigned blocks_for(uint64_t n, unsigned threads = 256)
+{
+    return (unsigned)((n + threads - 1) / threads);
+}
+
+}  // namespace vct
+
+using namespace vct;
+
+extern "C" {
+
+// SparseVoxelizer::preVoxelize + cmdVoxelize (SparseVoxelizer.cpp:248-326).
+vct_status vct_svo_voxelize(vct_ctx* c, vct_stream stream, const vct_mesh* mesh, const vct_shadow_map* shadow,
+                            const vct_directional_light_desc* light,
+                            const vct_directional_light_shadow_desc* light_shadow, uint32_t octree_level,
+                            uint32_t* nfrag_out)
+{
+    if (!c || !mesh || !shadow || !shadow->depth || !light || !light_shadow)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (octree_level < 1 || octree_level > 11)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "octree_level must be in [1, 11] (EngineConfig.h:20-21)");
+    hipStream_t s = (hipStream_t)stream;
+    TriSoup soup;
+    VCT_TRY(run_vertex_stage(c, s, mesh, &soup));
+    const uint32_t R = 1u << octree_level;
+    c->svo.level = octree_level;
+    c->svo.built = false;
+    c->svo.nfrag = 0;
+    if (nfrag_out) *nfrag_out = 0;
+    if (!soup.count) return VCT_OK;
+    // voxelizer.vert:42-47 normalisation
+    float ext[3], center[3];
+    for (int k = 0; k < 3; ++k) {
+        ext[k] = mesh->scene_bb_max[k] - mesh->scene_bb_min[k];
+        center[k] = (mesh->scene_bb_min[k] + mesh->scene_bb_max[k]) * 0.5f;
+    }
+    const float ev = std::fmax(ext[0], std::fmax(ext[1], ext[2])) * 0.5f;
+    if (!(ev > 0.0f)) return fail(VCT_ERR_INVALID_ARGUMENT, "empty scene bounding box");
+    const uint32_t n9 = soup.count * 9u;
+    VCT_TRY(ensure(c->rec_val, 4u * (size_t)n9));
+    float* npos = (float*)c->rec_val.ptr;
+    hipLaunchKernelGGL(k_svo_normalize, dim3(blocks_for(n9)), dim3(256), 0, s, soup.pos, n9, center[0], center[1],
+                       center[2], ev, npos);
+    SvoRaster Rr;
+    Rr.vs = 2.0f / (float)R;
+    for (int k = 0; k < 3; ++k) {
+        Rr.lo[k] = -(int)(R / 2);
+        Rr.hi[k] = (int)(R / 2) - 1;
+    }
+    Rr.samples = (int)c->cfg.raster_samples;
+    Rr.res = (int)R;
+    Rr.quirks = c->cfg.quirks;
+    VCT_TRY(ensure(c->counts, 8u * (size_t)soup.count));
+    VCT_TRY(ensure(c->offsets, 8u * (size_t)soup.count));
+    VCT_TRY(ensure(c->totals, 64));
+    hipLaunchKernelGGL(k_svo_count, dim3(blocks_for(soup.count)), dim3(256), 0, s, npos, soup.count, Rr,
+                       (unsigned long long*)c->counts.ptr);
+    size_t tmp = 0;
+    VCT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, (unsigned long long*)c->counts.ptr,
+                                             (unsigned long long*)c->offsets.ptr, (int)soup.count, s));
+    VCT_TRY(ensure(c->scan_tmp, tmp));
+    VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, (unsigned long long*)c->counts.ptr,
+                                             (unsigned long long*)c->offsets.ptr, (int)soup.count, s));
+    unsigned long long last[2];
+    VCT_HIP(hipMemcpyAsync(&last[0], (unsigned long long*)c->offsets.ptr + soup.count - 1, 8, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipMemcpyAsync(&last[1], (unsigned long long*)c->counts.ptr + soup.count - 1, 8, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipStreamSynchronize(s));
+    const unsigned long long nwork64 = last[0] + last[1];
+    if (nwork64 >= 0xffffffffull) return fail(VCT_ERR_CAPACITY, "raster work %llu exceeds 2^32", nwork64);
+    const uint32_t nwork = (uint32_t)nwork64;
+    if (!nwork) return VCT_OK;
+    VCT_TRY(ensure(c->rec_idx[0], 4u * (size_t)nwork));
+    VCT_TRY(ensure(c->rec_idx[1], 4u * (size_t)nwork));
+    uint32_t* covered = (uint32_t*)c->rec_idx[0].ptr;
+    uint32_t* slot = (uint32_t*)c->rec_idx[1].ptr;
+    hipLaunchKernelGGL(k_svo_cover, dim3(blocks_for(nwork)), dim3(256), 0, s, npos,
+                       (const unsigned long long*)c->offsets.ptr, soup.count, nwork, Rr, covered);
+    VCT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, covered, slot, (int)nwork, s));
+    VCT_TRY(ensure(c->scan_tmp, tmp));
+    VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, covered, slot, (int)nwork, s));
+    // the count pass's host read-back (SparseVoxelizer.cpp:279)
+    uint32_t lastc[2];
+    VCT_HIP(hipMemcpyAsync(&lastc[0], slot + nwork - 1, 4, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipMemcpyAsync(&lastc[1], covered + nwork - 1, 4, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipStreamSynchronize(s));
+    const uint32_t nfrag = lastc[0] + lastc[1];
+    VCT_TRY(ensure(c->svo.frags, 8u * (size_t)(nfrag ? nfrag : 1)));
+    if (nfrag) {
+        hipLaunchKernelGGL(k_fill_u2, dim3(blocks_for(nfrag)), dim3(256), 0, s, (uint2*)c->svo.frags.ptr, nfrag,
+                           make_uint2(0u, 0u));
+        const LightParams L = make_light(shadow, light, light_shadow);
+        hipLaunchKernelGGL(k_svo_write, dim3(blocks_for(nwork)), dim3(256), 0, s, npos, soup.nrm, soup.mat,
+                           mesh->materials, (const unsigned long long*)c->offsets.ptr, soup.count, nwork, Rr, L,
+                           covered, slot, (uint2*)c->svo.frags.ptr);
+        VCT_HIP(hipGetLastError());
+    }
+    c->svo.nfrag = nfrag;
+    if (nfrag_out) *nfrag_out = nfrag;
+    return VCT_OK;
+}
+
+// OctreeBuilder::cmdBuild (OctreeBuilder.cpp:205-345).
+vct_status vct_svo_build(vct_ctx* c, vct_stream stream)
+{
+    if (!c) return fail(VCT_ERR_INVALID_ARGUMENT, "null context");
+    if (!c->svo.level) return fail(VCT_ERR_NOT_READY, "vct_svo_voxelize has not run");
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t L = c->svo.level, R = 1u << L, nfrag = c->svo.nfrag;
+    const bool half = (c->cfg.quirks & VCT_QUIRK_SVO_HALF_COORDS) != 0;
+    // OctreeBuilder.cpp:110-113: clamp(8 * fragments, 1e6, 5e8) nodes
+    uint64_t cap = (uint64_t)nfrag << 3;
+    if (cap < 1000000ull) cap = 1000000ull;
+    if (cap > 500000000ull) cap = 500000000ull;
+    VCT_TRY(ensure(c->svo.pool, 8u * (size_t)cap));
+    c->svo.pool_nodes = (uint32_t)cap;
+    uint2* pool = (uint2*)c->svo.pool.ptr;
+    const uint2* frags = (const uint2*)c->svo.frags.ptr;
+    uint32_t begin = 0, num = 8, counter = 0;
+    std::vector<uint32_t> level_begin, level_num;
+    VCT_TRY(ensure(c->counts, 8u * ((cap >> 6) + 8)));
+    VCT_TRY(ensure(c->offsets, 8u * ((cap >> 6) + 8)));
+    for (uint32_t i = 1; i <= L; ++i) {
+        if ((uint64_t)begin + num > cap) return fail(VCT_ERR_CAPACITY, "octree node pool overflow (%llu nodes)", (unsigned long long)cap);
+        level_begin.push_back(begin);
+        level_num.push_back(num);
+        // octreeNodeInit.comp
+        hipLaunchKernelGGL(k_fill_u2, dim3(blocks_for(num)), dim3(256), 0, s, pool + begin, num, make_uint2(0u, 0u));
+        // octreeNodeFlag.comp
+        if (nfrag)
+            hipLaunchKernelGGL(k_svo_flag, dim3(blocks_for(nfrag)), dim3(256), 0, s, pool, frags, nfrag, R, half);
+        if (i == L) break;
+        // octreeNodeAlloc.comp + octreeNodeModifyArg.comp
+        const uint32_t waves = (num + 63) / 64;
+        uint32_t* wave_counts = (uint32_t*)c->counts.ptr;
+        uint32_t* wave_offsets = (uint32_t*)c->offsets.ptr;
+        hipLaunchKernelGGL(k_svo_alloc_count, dim3(blocks_for(num)), dim3(256), 0, s, pool, begin, num, wave_counts);
+        size_t tmp = 0;
+        VCT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, wave_counts, wave_offsets, (int)waves, s));
+        VCT_TRY(ensure(c->scan_tmp, tmp));
+        VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, wave_counts, wave_offsets, (int)waves, s));
+        hipLaunchKernelGGL(k_svo_alloc_assign, dim3(blocks_for(num)), dim3(256), 0, s, pool, begin, num, wave_offsets,
+                           counter);
+        uint32_t lw[2];
+        VCT_HIP(hipMemcpyAsync(&lw[0], wave_offsets + waves - 1, 4, hipMemcpyDeviceToHost, s));
+        VCT_HIP(hipMemcpyAsync(&lw[1], wave_counts + waves - 1, 4, hipMemcpyDeviceToHost, s));
+        VCT_HIP(hipStreamSynchronize(s));
+        counter += lw[0] + lw[1];
+        begin += num;
+        num = (counter << 3) + 8u - begin;
+    }
+    VCT_HIP(hipGetLastError());
+    const uint64_t total = ((uint64_t)counter << 3) + 8u;
+    if (total > cap) return fail(VCT_ERR_CAPACITY, "octree node pool overflow");
+    c->svo.node_count = (uint32_t)total;
+    // octreeNodeLeafWrite.comp (fragment-ordered running average)
+    if (nfrag) {
+        for (int b = 0; b < 2; ++b) {
+            VCT_TRY(ensure(c->rec_key[b], 8u * (size_t)nfrag));
+        }
+        VCT_TRY(ensure(c->rec_idx[0], 4u * (size_t)nfrag));
+        VCT_TRY(ensure(c->rec_idx[1], 4u * (size_t)nfrag));
+        hipLaunchKernelGGL(k_svo_leaf_keys, dim3(blocks_for(nfrag)), dim3(256), 0, s, pool, frags, nfrag, R, half,
+                           (unsigned long long*)c->rec_key[0].ptr, (uint32_t*)c->rec_idx[0].ptr);
+        size_t tmp = 0;
+        VCT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (unsigned long long*)c->rec_key[0].ptr,
+                                                   (unsigned long long*)c->rec_key[1].ptr,
+                                                   (uint32_t*)c->rec_idx[0].ptr, (uint32_t*)c->rec_idx[1].ptr,
+                                                   (int)nfrag, 0, 64, s));
+        VCT_TRY(ensure(c->sort_tmp, tmp));
+        VCT_HIP(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp.ptr, tmp, (unsigned long long*)c->rec_key[0].ptr,
+                                                   (unsigned long long*)c->rec_key[1].ptr,
+                                                   (uint32_t*)c->rec_idx[0].ptr, (uint32_t*)c->rec_idx[1].ptr,
+                                                   (int)nfrag, 0, 64, s));
+        hipLaunchKernelGGL(k_svo_leaf_apply, dim3(blocks_for(nfrag)), dim3(256), 0, s, pool, frags,
+                           (const unsigned long long*)c->rec_key[1].ptr, (const uint32_t*)c->rec_idx[1].ptr, nfrag);
+    }
+    // octreeNodeMipmapWrite.comp for i = 2..L (OctreeBuilder.cpp:328-344). The
+    // target of pass i is the node whose descent resolution equals i (quirk:
+    // only i in {2, 4, 8} exist) or 2^(i-1); that node sits at depth
+    // L - log2(target), whose pool range is level_begin/num of that depth.
+    const bool quirk = (c->cfg.quirks & VCT_QUIRK_SVO_MIP_LEVEL_TEST) != 0;
+    for (uint32_t i = 2; i <= L; ++i) {
+        uint32_t target = quirk ? i : (1u << (i - 1));
+        if (target & (target - 1)) continue;  // not a power of two: no node has that resolution
+        uint32_t lg = 0;
+        while ((1u << lg) < target) ++lg;
+        if (lg >= L) continue;
+        const uint32_t depth = L - lg;  // 1-based depth of the target nodes
+        if (depth < 1 || depth > level_begin.size()) continue;
+        hipLaunchKernelGGL(k_svo_mip, dim3(blocks_for(level_num[depth - 1])), dim3(256), 0, s, pool,
+                           level_begin[depth - 1], level_num[depth - 1]);
+    }
+    VCT_HIP(hipGetLastError());
+    c->svo.built = true;
+    return VCT_OK;
+}
+
+// OctreeBuilder::readOctreeNodes (OctreeBuilder.cpp:359-385): host copy of the
+// allocated part of the pool (uvec2 per node).
+vct_status vct_svo_read_nodes(vct_ctx* c, vct_stream stream, uint32_t* out, size_t cap_nodes, size_t* n_nodes)
+{
+    if (!c || !n_nodes) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->svo.built) return fail(VCT_ERR_NOT_READY, "vct_svo_build has not run");
+    *n_nodes = c->svo.node_count;
+    if (!out) return VCT_OK;
+    if (cap_nodes < c->svo.node_count) return fail(VCT_ERR_CAPACITY, "output holds %zu nodes, pool has %u", cap_nodes, c->svo.node_count);
+    hipStream_t s = (hipStream_t)stream;
+    VCT_HIP(hipMemcpyAsync(out, c->svo.pool.ptr, 8u * (size_t)c->svo.node_count, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipStreamSynchronize(s));
+    return VCT_OK;
+}
+
+// SparseVoxelizer::readVoxelFragments (SparseVoxelizer.cpp:328-353).
+vct_status vct_svo_read_fragments(vct_ctx* c, vct_stream stream, uint32_t* out, size_t cap, size_t* n)
+{
+    if (!c || !n) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    *n = c->svo.nfrag;
+    if (!out || !c->svo.nfrag) return VCT_OK;
+    if (cap < c->svo.nfrag) return fail(VCT_ERR_CAPACITY, "output holds %zu fragments, list has %u", cap, c->svo.nfrag);
+    hipStream_t s = (hipStream_t)stream;
+    VCT_HIP(hipMemcpyAsync(out, c->svo.frags.ptr, 8u * (size_t)c->svo.nfrag, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipStreamSynchronize(s));
+    return VCT_OK;
+}
+
+// Device pointer to the node pool ("OctreeBuilder::getOctreeBuffer").
+vct_status vct_svo_pool(vct_ctx* c, const uint32_t** pool, uint32_t* node_count)
+{
+    if (!c || !pool || !node_count) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->svo.built) return fail(VCT_ERR_NOT_READY, "vct_svo_build has not run");
+    *pool = (const uint32_t*)c->svo.pool.ptr;
+    *node_count = c->svo.node_count;
+    return VCT_OK;
+}
+
+}  // extern "C"

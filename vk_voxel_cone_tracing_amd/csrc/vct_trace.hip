@@ -440,19 +440,255 @@ __global__ __launch_bounds__(256) void k_trace(const typename Vol<kHalf>::T* __r
 
 }  // namespace vct
 
+
+namespace vct {
+
+struct SvoTraceParams {
+    float center[3];  // scene box centre (voxelizer.vert:42-47)
+    float ev;         // largest half extent of the scene box
+};
+
+// voxelConeTracing_Octree.frag:319-346 sampleSVO (top-down walk to the target
+// resolution; uvec3() of a negative value saturates to 0).
+__device__ __forceinline__ f4 sample_svo(const uint2* __restrict__ pool, const TraceParams& P, const SvoTraceParams& S,
+                                         const float pos[3], uint32_t level)
+{
+    uint32_t res = (uint32_t)P.dim;
+    uint32_t fp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float s = (pos[k] - S.center[k]) / S.ev;
+        const float f = ((s + 1.0f) * 0.5f) * (float)res;
+        const uint32_t u = f > 0.0f ? (f < 4294967040.0f ? (uint32_t)f : 0xffffffffu) : 0u;
+        fp[k] = u > res ? res : u;
+    }
+    const uint32_t target = 1u << level;
+    uint32_t idx = 0, cur = 0;
+    do {
+        res >>= 1;
+        const uint32_t cx = fp[0] >= res, cy = fp[1] >= res, cz = fp[2] >= res;
+        idx = cur + (cz | (cx << 1) | (cy << 2));
+        cur = pool[idx].x & 0x7fffffffu;
+        fp[0] -= cx * res;
+        fp[1] -= cy * res;
+        fp[2] -= cz * res;
+    } while (cur != 0u && res > target);
+    const uint32_t t = pool[idx].y;
+    const float sc = 1.0f / 255.0f;
+    return {(float)(t & 0xffu) * sc, (float)((t >> 8) & 0xffu) * sc, (float)((t >> 16) & 0xffu) * sc, (float)(t >> 24) * sc};
+}
+
+// voxelConeTracing_Octree.frag:359-409 traceCone
+__device__ f4 svo_trace_cone(const uint2* __restrict__ pool, const TraceParams& P, const SvoTraceParams& S,
+                             const float start_in[3], const float dir[3], float aperture, float max_dist,
+                             float start_level, float step_factor, uint32_t& steps)
+{
+    f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float cone_coef = 2.0f * tanf(aperture * 0.5f);
+    float voxel_size = P.voxel_size * exp2f(start_level);
+    float start[3];
+    for (int k = 0; k < 3; ++k) start[k] = start_in[k] + dir[k] * voxel_size * P.start_offset * 0.5f;
+    float step = 0.0f;
+    float diameter = fmaxf(step * cone_coef, P.voxel_size);
+    float occlusion = 0.0f;
+    float seg = voxel_size;
+    const float min_radius = P.voxel_size * P.dim * 0.5f;
+    const float max_level = (float)(P.levels - 1);
+    while (step < max_dist && occlusion < 1.0f) {
+        float pos[3];
+        for (int k = 0; k < 3; ++k) pos[k] = start[k] + dir[k] * step;
+        const float dx = P.center[0] - pos[0], dy = P.center[1] - pos[1], dz = P.center[2] - pos[2];
+        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float min_level = ceilf(log2f(dist / min_radius));
+        float cur = log2f(diameter / P.voxel_size);
+        cur = fminf(fmaxf(fmaxf(start_level, cur), min_level), max_level);
+        const f4 lo = sample_svo(pool, P, S, pos, (uint32_t)floorf(cur));
+        const f4 hi = sample_svo(pool, P, S, pos, (uint32_t)ceilf(cur));
+        const float fl = cur - floorf(cur);
+        const f4 s = {lo.x * (1.0f - fl) + hi.x * fl, lo.y * (1.0f - fl) + hi.y * fl, lo.z * (1.0f - fl) + hi.z * fl,
+                      lo.w * (1.0f - fl) + hi.w * fl};
+        voxel_size = P.voxel_size * exp2f(cur);
+        const float correction = seg / voxel_size;
+        const float opacity = fminf(fmaxf(1.0f - powf(1.0f - s.w, correction), 0.0f), 1.0f);
+        const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
+        result.x += blend * (s.x * correction);
+        result.y += blend * (s.y * correction);
+        result.z += blend * (s.z * correction);
+        result.w += blend * opacity;
+        occlusion += (1.0f - occlusion) * opacity / (1.0f + (step + voxel_size) * P.decay);
+        const float prev = step;
+        step += fmaxf(diameter, P.voxel_size) * step_factor;
+        seg = step - prev;
+        diameter = step * cone_coef;
+        ++steps;
+    }
+    return {fminf(fmaxf(result.x, 0.0f), 1.0f), fminf(fmaxf(result.y, 0.0f), 1.0f), fminf(fmaxf(result.z, 0.0f), 1.0f),
+            fminf(fmaxf(1.0f - occlusion, 0.0f), 1.0f)};
+}
+
+__device__ __forceinline__ float clamp01(float x)
+{
+    return fminf(fmaxf(x, 0.0f), 1.0f);
+}
+
+// voxelConeTracing_Octree.frag:148-300 main
+__global__ __launch_bounds__(256) void k_svo_trace(const uint2* __restrict__ pool, TraceParams P, SvoTraceParams S,
+                                                   const uint32_t* __restrict__ g_diffuse,
+                                                   const uint16_t* __restrict__ g_normal,
+                                                   const uint32_t* __restrict__ g_specular,
+                                                   const uint16_t* __restrict__ g_emission,
+                                                   const float* __restrict__ g_depth, float4* __restrict__ out_d,
+                                                   float4* __restrict__ out_s, unsigned long long* __restrict__ samples)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    uint32_t steps = 0;
+    if (px < P.width && py < P.height) {
+        const size_t idx = (size_t)py * (size_t)P.width + (size_t)px;
+        float4 dO = make_float4(0.0f, 0.0f, 0.0f, 1.0f), sO = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        const float depth = g_depth[idx];
+        if (depth != 1.0f) {
+            const float tcx = ((float)px + 0.5f) / (float)P.width, tcy = ((float)py + 0.5f) / (float)P.height;
+            const float q[4] = {tcx * 2.0f - 1.0f, tcy * 2.0f - 1.0f, depth, 1.0f};
+            float wp[4];
+            for (int k = 0; k < 4; ++k)
+                wp[k] = P.vp_inv[k] * q[0] + P.vp_inv[4 + k] * q[1] + P.vp_inv[8 + k] * q[2] + P.vp_inv[12 + k] * q[3];
+            const float world[3] = {wp[0] / wp[3], wp[1] / wp[3], wp[2] / wp[3]};
+            float view[3] = {P.eye[0] - world[0], P.eye[1] - world[1], P.eye[2] - world[2]};
+            {
+                const float l = sqrtf(view[0] * view[0] + view[1] * view[1] + view[2] * view[2]);
+                for (int k = 0; k < 3; ++k) view[k] /= l;
+            }
+            const uint32_t dt = g_diffuse[idx], st = g_specular[idx];
+            const float dc[3] = {u8_to_f(dt & 0xffu), u8_to_f((dt >> 8) & 0xffu), u8_to_f((dt >> 16) & 0xffu)};
+            const float sc[3] = {u8_to_f(st & 0xffu), u8_to_f((st >> 8) & 0xffu), u8_to_f((st >> 16) & 0xffu)};
+            const float rough = u8_to_f(dt >> 24), metallic = u8_to_f(st >> 24);
+            const f4 nf = unpack_half(*(const uint2*)(g_normal + 4 * idx));
+            const f4 ef = unpack_half(*(const uint2*)(g_emission + 4 * idx));
+            float normal[3] = {nf.x * 2.0f - 1.0f, nf.y * 2.0f - 1.0f, nf.z * 2.0f - 1.0f};
+            {
+                const float l = sqrtf(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+                for (int k = 0; k < 3; ++k) normal[k] /= l;
+            }
+            const bool has_emission = ef.x > 0.0f || ef.y > 0.0f || ef.z > 0.0f;
+            float min_level;
+            {
+                const float dx = P.center[0] - world[0], dy = P.center[1] - world[1], dz = P.center[2] - world[2];
+                const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+                const float min_radius = P.voxel_size * P.dim * 0.5f;
+                const float ml = fmaxf(log2f(dist / min_radius), 0.0f);
+                const float radius = min_radius * exp2f(ceilf(ml));
+                const float f = dist / radius;
+                min_level = f > 0.5f ? ceilf(ml) + (f - 0.5f) * 2.0f : ceilf(ml);
+            }
+            const float vs = P.voxel_size * exp2f(min_level);
+            float start[3];
+            for (int k = 0; k < 3; ++k) start[k] = world[k] + normal[k] * vs * P.start_offset;
+            f4 ind = {0.0f, 0.0f, 0.0f, 1.0f};
+            float valid = 0.0f;
+            const int ncones = P.cones32 ? 32 : 16;
+            const float aperture = P.cones32 ? 0.628319f : 0.872665f;
+            const bool want_diffuse = (P.mode == 4u || P.mode == 5u || P.mode == 7u || P.mode == 8u);
+            for (int i = 0; want_diffuse && i < ncones; ++i) {
+                const float* d = P.cones32 ? c_dirs32[i] : c_dirs16[i];
+                const float ct = normal[0] * d[0] + normal[1] * d[1] + normal[2] * d[2];
+                if (ct < 0.0f) continue;
+                const f4 c = svo_trace_cone(pool, P, S, start, d, aperture, 30.0f, min_level, P.step_factor, steps);
+                ind.x += c.x * ct;
+                ind.y += c.y * ct;
+                ind.z += c.z * ct;
+                ind.w += c.w * ct;
+                valid += ct;
+            }
+            ind.x /= valid;
+            ind.y /= valid;
+            ind.z /= valid;
+            ind.w /= valid;
+            ind.w *= P.ao_factor;
+            ind.x *= dc[0] * P.diffuse_intensity;
+            ind.y *= dc[1] * P.diffuse_intensity;
+            ind.z *= dc[2] * P.diffuse_intensity;
+            ind = {clamp01(ind.x), clamp01(ind.y), clamp01(ind.z), clamp01(ind.w)};
+            float spec[3] = {0.0f, 0.0f, 0.0f};
+            if ((P.mode == 6u || P.mode == 8u) && (sc[0] > 1e-6f || sc[1] > 1e-6f || sc[2] > 1e-6f) && metallic > 1e-6f) {
+                const float I[3] = {-view[0], -view[1], -view[2]};
+                const float dni = normal[0] * I[0] + normal[1] * I[1] + normal[2] * I[2];
+                float dir[3] = {I[0] - 2.0f * dni * normal[0], I[1] - 2.0f * dni * normal[1], I[2] - 2.0f * dni * normal[2]};
+                const float l = sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+                for (int k = 0; k < 3; ++k) dir[k] /= l;
+                const f4 c = svo_trace_cone(pool, P, S, start, dir, fmaxf(rough, 0.05f), 30.0f, min_level,
+                                            fmaxf(0.2f, P.voxel_size), steps);
+                spec[0] = c.x * sc[0] * P.specular_intensity;
+                spec[1] = c.y * sc[1] * P.specular_intensity;
+                spec[2] = c.z * sc[2] * P.specular_intensity;
+            }
+            float direct[3] = {0.0f, 0.0f, 0.0f};
+            if (P.mode == 4u || P.mode == 8u) {
+                if (has_emission) {
+                    direct[0] = ef.x;
+                    direct[1] = ef.y;
+                    direct[2] = ef.z;
+                } else {
+                    float b[3];
+                    brdf(normal, view, P.light_dir, dc, sc, rough, b);
+                    const float visv = trace_visibility(P, world);
+                    for (int k = 0; k < 3; ++k) direct[k] = b[k] * visv;
+                }
+                for (int k = 0; k < 3; ++k) direct[k] = clamp01(direct[k]);
+            }
+            switch (P.mode) {
+            case 0: dO = make_float4(dc[0], dc[1], dc[2], 1.0f); break;
+            case 1: dO = make_float4(sc[0], sc[1], sc[2], 1.0f); break;
+            case 2: dO = make_float4(normal[0] * 0.5f + 0.5f, normal[1] * 0.5f + 0.5f, normal[2] * 0.5f + 0.5f, 1.0f); break;
+            case 3: {
+                const float colors[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+                float o4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (min_level < 6.0f) {
+                    int lower = (int)floorf(min_level);
+                    lower = lower < 0 ? 0 : lower;
+                    const float fr = min_level - floorf(min_level);
+                    for (int k = 0; k < 3; ++k) o4[k] = colors[lower][k] * (1.0f - fr) + colors[lower + 1][k] * fr;
+                    o4[3] = 1.0f;
+                }
+                dO = make_float4(o4[0] * 0.5f, o4[1] * 0.5f, o4[2] * 0.5f, o4[3] * 0.5f);
+                break;
+            }
+            case 4: dO = make_float4(direct[0] * ind.w, direct[1] * ind.w, direct[2] * ind.w, 1.0f); break;
+            case 5: dO = make_float4(ind.x, ind.y, ind.z, 1.0f); break;
+            case 6: sO = make_float4(spec[0], spec[1], spec[2], 1.0f); break;
+            case 7: dO = make_float4(ind.w, ind.w, ind.w, 1.0f); break;
+            case 8:
+                dO = make_float4(direct[0] * ind.w + ind.x, direct[1] * ind.w + ind.y, direct[2] * ind.w + ind.z, 1.0f);
+                sO = make_float4(spec[0], spec[1], spec[2], 1.0f);
+                break;
+            default: break;
+            }
+        }
+        out_d[idx] = dO;
+        out_s[idx] = sO;
+    }
+    if (samples) {
+        unsigned long long v = steps;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0 && v) atomicAdd(samples, v);
+    }
+}
+
+}  // namespace vct
+
 using namespace vct;
 
-extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer* g, const vct_camera_ubo* cam,
-                                const vct_cone_tracing_desc* d, const vct_shadow_map* shadow,
-                                const vct_directional_light_desc* light,
-                                const vct_directional_light_shadow_desc* light_shadow, float* diffuse_out,
-                                float* specular_out, unsigned long long* samples_out)
+static vct_status make_trace_params(vct_ctx* c, const vct_gbuffer* g, const vct_camera_ubo* cam,
+                                    const vct_cone_tracing_desc* d, const vct_shadow_map* shadow,
+                                    const vct_directional_light_desc* light,
+                                    const vct_directional_light_shadow_desc* light_shadow, float* diffuse_out,
+                                    float* specular_out, int levels, TraceParams* out)
 {
     if (!c || !g || !cam || !d || !diffuse_out || !specular_out) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
     if (!g->diffuse || !g->normal || !g->specular || !g->emission || !g->depth || !g->width || !g->height)
         return fail(VCT_ERR_INVALID_ARGUMENT, "incomplete G-buffer");
     if (d->rendering_mode > 8u) return fail(VCT_ERR_INVALID_ARGUMENT, "rendering_mode");
-    if (d->volume_dimension != (float)c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "volume_dimension must equal the resolution");
     if (!(d->voxel_size > 0.0f)) return fail(VCT_ERR_INVALID_ARGUMENT, "voxel_size");
     const bool need_light = d->rendering_mode == 4u || d->rendering_mode == 5u || d->rendering_mode == 8u;
     if (need_light && (!shadow || !shadow->depth || !shadow->width || !shadow->height || !light || !light_shadow))
@@ -469,7 +705,7 @@ extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer
     P.specular_intensity = d->indirect_specular_intensity;
     P.decay = d->occlusion_decay;
     P.cones32 = d->enable_32_cones != 0;
-    P.levels = c->levels;
+    P.levels = levels;
     P.res = c->res;
     P.D = c->D;
     for (int k = 0; k < 16; ++k) P.vp_inv[k] = cam->view_proj_inv[k];
@@ -495,6 +731,21 @@ extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer
         P.sw = (int)shadow->width;
         P.sh = (int)shadow->height;
     }
+    *out = P;
+    return VCT_OK;
+}
+
+extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer* g, const vct_camera_ubo* cam,
+                                const vct_cone_tracing_desc* d, const vct_shadow_map* shadow,
+                                const vct_directional_light_desc* light,
+                                const vct_directional_light_shadow_desc* light_shadow, float* diffuse_out,
+                                float* specular_out, unsigned long long* samples_out)
+{
+    TraceParams P;
+    VCT_TRY(make_trace_params(c, g, cam, d, shadow, light, light_shadow, diffuse_out, specular_out,
+                              c ? c->levels : 0, &P));
+    if (d->volume_dimension != (float)c->res)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "volume_dimension must equal the resolution");
     dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
     hipStream_t s = (hipStream_t)stream;
     if (c->cfg.radiance_format == VCT_FORMAT_RGBA16F)
@@ -504,6 +755,40 @@ extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer
         hipLaunchKernelGGL(k_trace<false>, grid, dim3(256), 0, s, (const uint32_t*)c->radiance.ptr, P, g->diffuse,
                            g->normal, g->specular, g->emission, g->depth, (float4*)diffuse_out, (float4*)specular_out,
                            samples_out);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+// OctreeVoxelConeTracing::onUpdate (OctreeVoxelConeTracing.cpp:74-104). The
+// scene box is a parameter (the shader hard-codes Sponza's,
+// voxelConeTracing_Octree.frag:321-322); volume_dimension is the octree
+// resolution the walk starts from (the reference pushes 256, :93).
+extern "C" vct_status vct_svo_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer* g, const vct_camera_ubo* cam,
+                                    const vct_cone_tracing_desc* d, const vct_shadow_map* shadow,
+                                    const vct_directional_light_desc* light,
+                                    const vct_directional_light_shadow_desc* light_shadow,
+                                    const float scene_bb_min[3], const float scene_bb_max[3], float* diffuse_out,
+                                    float* specular_out, unsigned long long* samples_out)
+{
+    TraceParams P;
+    VCT_TRY(make_trace_params(c, g, cam, d, shadow, light, light_shadow, diffuse_out, specular_out, 6, &P));
+    if (!scene_bb_min || !scene_bb_max) return fail(VCT_ERR_INVALID_ARGUMENT, "null scene box");
+    if (!c->svo.built) return fail(VCT_ERR_NOT_READY, "vct_svo_build has not run");
+    const float dim = d->volume_dimension;
+    if (!(dim >= 2.0f) || dim > 4096.0f || (((uint32_t)dim) & ((uint32_t)dim - 1)) || (float)(uint32_t)dim != dim)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "volume_dimension must be a power of two");
+    SvoTraceParams S;
+    float ext[3];
+    for (int k = 0; k < 3; ++k) {
+        ext[k] = scene_bb_max[k] - scene_bb_min[k];
+        S.center[k] = (scene_bb_min[k] + scene_bb_max[k]) * 0.5f;
+    }
+    S.ev = std::fmax(ext[0], std::fmax(ext[1], ext[2])) * 0.5f;
+    if (!(S.ev > 0.0f)) return fail(VCT_ERR_INVALID_ARGUMENT, "empty scene box");
+    dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+    hipLaunchKernelGGL(k_svo_trace, grid, dim3(256), 0, (hipStream_t)stream, (const uint2*)c->svo.pool.ptr, P, S,
+                       g->diffuse, g->normal, g->specular, g->emission, g->depth, (float4*)diffuse_out,
+                       (float4*)specular_out, samples_out);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
