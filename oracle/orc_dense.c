@@ -83,7 +83,9 @@ static void dense_frag(void* user, uint32_t tri, const float pos[3], const doubl
     if (c->winner[v] == 0) c->winner[v] = tri + 1u;
 }
 
-/* pos must already be translated by -grid_min (float subtraction per vertex). */
+/* World-space triangles are moved into the grid frame by one float
+ * subtraction of grid_min per vertex coordinate, then rasterized with the
+ * voxel size extent / res. */
 void orc_dense_voxelize(orc_dense* g, const orc_tris* tris, const orc_material* mats, int samples)
 {
     const int n = g->res;
@@ -91,8 +93,11 @@ void orc_dense_voxelize(orc_dense* g, const orc_tris* tris, const orc_material* 
     size_t nv = (size_t)n * n * n;
     dvox_ctx c = {g, tris, mats, vs, (uint32_t*)calloc(nv, 4)};
     int lo[3] = {0, 0, 0}, hi[3] = {n - 1, n - 1, n - 1};
-    for (uint32_t t = 0; t < tris->count; ++t)
-        orc_raster_triangle(tris->pos + 9u * t, vs, lo, hi, samples, dense_frag, &c, t);
+    for (uint32_t t = 0; t < tris->count; ++t) {
+        float p[9];
+        for (int k = 0; k < 9; ++k) p[k] = tris->pos[9u * t + k] - g->grid_min[k % 3];
+        orc_raster_triangle(p, vs, lo, hi, samples, dense_frag, &c, t);
+    }
     for (int z = 0; z < n; ++z)
         for (int y = 0; y < n; ++y)
             for (int x = 0; x < n; ++x) {

@@ -60,3 +60,10 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.vct_destroy(None) == 0
     assert L.vct_trace(None, None, None, None, None, None, None, None, None, None, None) == 1
     assert b"null" in L.vct_last_error()
+
+
+def test_ctypes_mirror_covers_every_declared_function():
+    from vk_voxel_cone_tracing_amd import _abi
+
+    missing = [n for n in _declared_functions() if n not in _abi._PROTOS]
+    assert not missing, missing

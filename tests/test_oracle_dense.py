@@ -128,3 +128,22 @@ def test_opaque_grid_closed_form():
     assert abs(out[0, 0, 3] - (1 + cs * (1 - occ)) / 16 * 0.5) < 1e-4
     albedo = 0x80 / 255.0
     np.testing.assert_allclose(out[0, 0, :3], cs / 16 * albedo * 8.0, rtol=1e-4)
+
+
+def test_voxelize_is_translation_invariant():
+    # Shifting the triangles and the grid by the same dyadic offset (exact in
+    # float) must reproduce the unshifted grid texel for texel.
+    res = 16
+    rng = np.random.default_rng(9)
+    # coordinates on a 2^-16 lattice so that adding the offset is exact
+    tris = (np.round(rng.uniform(0.05, 0.95, size=(40, 9)) * 65536.0) / 65536.0).astype(np.float32)
+    t, n, m, mats = _tris(tris, [tuple(rng.uniform(0, 1, 3)) for _ in range(40)])
+    d0, lv0 = O.new_dense(res)
+    O.lib().orc_dense_voxelize(C.byref(d0), C.byref(O.tris_struct(t, n, m)), mats, 4)
+    off = np.array([3.5, -2.25, 8.0], np.float32)
+    ts = np.ascontiguousarray((t.reshape(-1, 3) + off).reshape(-1, 9))
+    assert np.array_equal((ts.reshape(-1, 3) - off).reshape(-1, 9), t)
+    d1, lv1 = O.new_dense(res, grid_min=tuple(off.tolist()))
+    O.lib().orc_dense_voxelize(C.byref(d1), C.byref(O.tris_struct(ts, n, m)), mats, 4)
+    assert (lv0[0] != 0).sum() > 100
+    np.testing.assert_array_equal(lv0[0], lv1[0])

@@ -125,16 +125,23 @@ def camera_ubo(view_proj, view_proj_inv, eye):
 
 
 def shadow_struct(depth_tensor):
+    """vct_shadow_map over a device tensor; the struct keeps the tensor alive,
+    so the memory it points to cannot be recycled while the struct exists."""
     h, w = depth_tensor.shape
-    return _abi.ShadowMap(depth_tensor.data_ptr(), w, h)
+    s = _abi.ShadowMap(depth_tensor.data_ptr(), w, h)
+    s._keep = depth_tensor
+    return s
 
 
 def gbuffer_struct(g: dict):
+    """vct_gbuffer over device tensors (kept alive by the struct)."""
     h, w = g["depth"].shape
-    return _abi.GBuffer(
+    s = _abi.GBuffer(
         g["diffuse"].data_ptr(), g["normal"].data_ptr(), g["specular"].data_ptr(), g["emission"].data_ptr(),
         g["depth"].data_ptr(), w, h,
     )
+    s._keep = dict(g)
+    return s
 
 
 class Context:

@@ -206,6 +206,30 @@ class GBuffer(C.Structure):
     ]
 
 
+class DenseDesc(C.Structure):
+    _fields_ = [
+        ("grid_min", f3),
+        ("extent", C.c_float),
+        ("resolution", C.c_uint32),
+        ("format", C.c_uint32),
+        ("raster_samples", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class DenseTraceDesc(C.Structure):
+    _fields_ = [
+        ("aperture", C.c_float),
+        ("max_distance", C.c_float),
+        ("min_trace_step_factor", C.c_float),
+        ("trace_start_offset", C.c_float),
+        ("occlusion_decay", C.c_float),
+        ("ambient_occlusion_factor", C.c_float),
+        ("indirect_diffuse_intensity", C.c_float),
+        ("enable_32_cones", C.c_int32),
+    ]
+
+
 STRUCT_SIZES = {
     "vct_config": (Config, 32),
     "vct_clipmap_region": (ClipmapRegion, 28),
@@ -219,6 +243,8 @@ STRUCT_SIZES = {
     "vct_directional_light_shadow_desc": (DirectionalLightShadowDesc, 144),
     "vct_camera_ubo": (CameraUBO, 144),
     "vct_gltf_material": (GltfMaterial, 80),
+    "vct_dense_desc": (DenseDesc, 32),
+    "vct_dense_trace_desc": (DenseTraceDesc, 32),
 }
 
 P = C.c_void_p
@@ -266,6 +292,17 @@ _PROTOS = {
         [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(ConeTracingDesc), C.POINTER(ShadowMap),
          C.POINTER(DirectionalLightDesc), C.POINTER(DirectionalLightShadowDesc), C.POINTER(C.c_float),
          C.POINTER(C.c_float), P, P, P],
+    ),
+    "vct_dense_create": (C.c_int, [C.POINTER(P), C.POINTER(DenseDesc)]),
+    "vct_dense_destroy": (C.c_int, [P]),
+    "vct_dense_clear": (C.c_int, [P, P]),
+    "vct_dense_level": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)]),
+    "vct_dense_read_level": (C.c_int, [P, P, C.c_uint32, P, C.c_size_t]),
+    "vct_dense_write_level": (C.c_int, [P, P, C.c_uint32, P, C.c_size_t]),
+    "vct_dense_voxelize": (C.c_int, [P, P, C.POINTER(Mesh)]),
+    "vct_dense_mipmap": (C.c_int, [P, P]),
+    "vct_dense_trace": (
+        C.c_int, [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(DenseTraceDesc), P, P]
     ),
     "vct_volume_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_size_t)]),
     "vct_read_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),

@@ -20,6 +20,9 @@
 #include "vct_raster.hpp"
 #include "vct_texel.hpp"
 
+static_assert(sizeof(vct_dense_desc) == 32, "vct_dense_desc layout");
+static_assert(sizeof(vct_dense_trace_desc) == 32, "vct_dense_trace_desc layout");
+
 namespace vct {
 
 vct_status dense_alloc(vct_dense* g)
