@@ -42,7 +42,23 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=30, help="rows of the 1080p image traced by the CPU baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates the HBM byte counters) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib: int) -> None:
+    """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE of
+    this kernel divided by the bytes it must move gives the counters' scale."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}), flush=True)
 
 
 def setup():
@@ -202,6 +218,9 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     st = setup()
     for _ in range(args.warmup):
         frame(st)
