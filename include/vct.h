@@ -406,6 +406,57 @@ vct_status vct_dense_trace(vct_dense* grid, vct_stream stream, const vct_gbuffer
                            const vct_camera_ubo* camera, const vct_dense_trace_desc* desc, float* out,
                            unsigned long long* samples_out);
 
+/* ---- batched dense method ------------------------------------------------
+ * No reference counterpart (the reference renders one scene): many scenes of
+ * uneven size, each voxelized into its own dense grid (vct_dense_voxelize's
+ * rule), mipped (vct_dense_mipmap) and traced (vct_dense_trace) in ONE
+ * persistent launch. The launch pulls tasks (vertex chunks, raster chunks,
+ * mip bricks, pixel tiles) from a queue ordered by phase; a task waits on its
+ * scene's completion counter of the previous phase, so small scenes trace
+ * while large ones still voxelize. The launch (counter reset + persistent
+ * kernel) is captured once in a hipGraph and replayed by vct_batch_run.
+ * Results are bit-identical to the per-scene vct_dense_* calls. */
+typedef struct vct_batch vct_batch;
+
+typedef struct vct_batch_scene {
+    vct_mesh mesh;          /* device buffers + host primitive list */
+    vct_dense_desc grid;    /* format must be VCT_FORMAT_RGBA8 */
+    vct_gbuffer gbuffer;    /* diffuse, normal and depth are read */
+    vct_camera_ubo camera;
+    float* out;             /* device RGBA32F, gbuffer.width x gbuffer.height */
+} vct_batch_scene;
+
+/* Schedule knobs picked by vct_batch_autotune. */
+typedef struct vct_batch_tuning {
+    uint32_t raster_chunk;      /* triangles per raster task: 64, 128 or 256 */
+    uint32_t mip_brick;         /* destination brick edge of a mip task: 4 or 8 */
+    uint32_t trace_schedule;    /* 0: one lane per pixel marching its cones in order;
+                                   1: lanes take the wave's (pixel, cone) pairs with
+                                      cos >= 0 in turn, summed per pixel in cone order */
+    uint32_t workgroups_per_cu; /* persistent grid = CUs x this: 1, 2 or 4 */
+} vct_batch_tuning;
+
+vct_status vct_batch_create(vct_batch** out, const vct_batch_scene* scenes, uint32_t count,
+                            const vct_dense_trace_desc* trace);
+vct_status vct_batch_destroy(vct_batch* batch);
+vct_status vct_batch_set_tuning(vct_batch* batch, const vct_batch_tuning* tuning);
+vct_status vct_batch_get_tuning(const vct_batch* batch, vct_batch_tuning* tuning);
+/* Replays the captured launch on `stream`; `samples_out` (device, may be
+ * NULL) accumulates executed cone samples and is not cleared here. */
+vct_status vct_batch_run(vct_batch* batch, vct_stream stream, unsigned long long* samples_out);
+/* Synchronizes and reports whether any task of the previous runs gave up
+ * waiting on a dependency (VCT_ERR_HIP with a message), then clears it. */
+vct_status vct_batch_status(vct_batch* batch);
+/* Times every candidate schedule `reps` times on `stream`, keeps the fastest
+ * and returns it (and its median milliseconds). */
+vct_status vct_batch_autotune(vct_batch* batch, vct_stream stream, uint32_t reps, vct_batch_tuning* best,
+                              float* best_ms);
+/* Level `level` of scene `scene` in the vct_dense_level layout. */
+vct_status vct_batch_level(vct_batch* batch, uint32_t scene, uint32_t level, void** ptr, size_t* bytes,
+                           uint32_t* faces);
+vct_status vct_batch_read_level(vct_batch* batch, vct_stream stream, uint32_t scene, uint32_t level, void* dst,
+                                size_t bytes);
+
 /* ---- input producers (no reference counterpart on the hot path) ----------
  * The reference rasterizes its G-buffer (GBufferPass.cpp:170-206) and light
  * depth map (ReflectiveShadowMapPass.cpp:38-88) before the GI passes run;

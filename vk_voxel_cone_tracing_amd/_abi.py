@@ -21,6 +21,8 @@ STATUS_NAMES = {
     5: "VCT_ERR_UNSUPPORTED",
     6: "VCT_ERR_NOT_READY",
 }
+for _code, _name in STATUS_NAMES.items():
+    globals()[_name] = _code
 
 VOLUME_OPACITY = 0
 VOLUME_RADIANCE = 1
@@ -230,6 +232,25 @@ class DenseTraceDesc(C.Structure):
     ]
 
 
+class BatchScene(C.Structure):
+    _fields_ = [
+        ("mesh", Mesh),
+        ("grid", DenseDesc),
+        ("gbuffer", GBuffer),
+        ("camera", CameraUBO),
+        ("out", C.c_void_p),
+    ]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [
+        ("raster_chunk", C.c_uint32),
+        ("mip_brick", C.c_uint32),
+        ("trace_schedule", C.c_uint32),
+        ("workgroups_per_cu", C.c_uint32),
+    ]
+
+
 STRUCT_SIZES = {
     "vct_config": (Config, 32),
     "vct_clipmap_region": (ClipmapRegion, 28),
@@ -245,6 +266,10 @@ STRUCT_SIZES = {
     "vct_gltf_material": (GltfMaterial, 80),
     "vct_dense_desc": (DenseDesc, 32),
     "vct_dense_trace_desc": (DenseTraceDesc, 32),
+    "vct_mesh": (Mesh, 104),
+    "vct_gbuffer": (GBuffer, 48),
+    "vct_batch_scene": (BatchScene, 336),
+    "vct_batch_tuning": (BatchTuning, 16),
 }
 
 P = C.c_void_p
@@ -304,6 +329,16 @@ _PROTOS = {
     "vct_dense_trace": (
         C.c_int, [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(DenseTraceDesc), P, P]
     ),
+    "vct_batch_create": (C.c_int, [C.POINTER(P), C.POINTER(BatchScene), C.c_uint32, C.POINTER(DenseTraceDesc)]),
+    "vct_batch_destroy": (C.c_int, [P]),
+    "vct_batch_set_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),
+    "vct_batch_get_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),
+    "vct_batch_run": (C.c_int, [P, P, P]),
+    "vct_batch_status": (C.c_int, [P]),
+    "vct_batch_autotune": (C.c_int, [P, P, C.c_uint32, C.POINTER(BatchTuning), C.POINTER(C.c_float)]),
+    "vct_batch_level": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
+                                  C.POINTER(C.c_uint32)]),
+    "vct_batch_read_level": (C.c_int, [P, P, C.c_uint32, C.c_uint32, P, C.c_size_t]),
     "vct_volume_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_size_t)]),
     "vct_read_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),
     "vct_write_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),

@@ -112,14 +112,6 @@ __device__ __forceinline__ float mixf(float x, float y, float a)
     return x * (1.0f - a) + y * a;
 }
 
-// Front/back texel pairs of the six faces (opacityDownSample.comp:94-138),
-// texel k at offset (k & 1, (k >> 1) & 1, k >> 2).
-__constant__ int8_t c_pairs[6][4][2] = {
-    {{0, 1}, {2, 3}, {4, 5}, {6, 7}}, {{1, 0}, {3, 2}, {5, 4}, {7, 6}},
-    {{0, 2}, {1, 3}, {4, 6}, {5, 7}}, {{2, 0}, {3, 1}, {6, 4}, {7, 5}},
-    {{0, 4}, {1, 5}, {2, 6}, {3, 7}}, {{4, 0}, {5, 1}, {6, 2}, {7, 3}},
-};
-
 template <bool kHalf, bool kRadiance>
 __global__ __launch_bounds__(256) void k_downsample(typename Texel<kHalf>::T* __restrict__ vol, int D, int res,
                                                     int level, int3 prev_min, int region_size)
@@ -152,6 +144,8 @@ __global__ __launch_bounds__(256) void k_downsample(typename Texel<kHalf>::T* __
         lerp = m - (float)((half >> 1) - region_size) + 1.0f;
         lerp = lerp * inv;
     }
+    // front/back pairs per face: opacityDownSample.comp:94-138 (face_pair_child)
+#pragma unroll
     for (int f = 0; f < kFaces; ++f) {
         f4 v[8];
 #pragma unroll
@@ -163,8 +157,8 @@ __global__ __launch_bounds__(256) void k_downsample(typename Texel<kHalf>::T* __
             float s = 0.0f;
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
-                const f4 a = v[c_pairs[f][p][0]];
-                const f4 b = v[c_pairs[f][p][1]];
+                const f4 a = v[face_pair_child(f, p, 0)];
+                const f4 b = v[face_pair_child(f, p, 1)];
                 s = s + a.w;
                 s = s + (1.0f - a.w) * b.w;
             }
@@ -176,8 +170,8 @@ __global__ __launch_bounds__(256) void k_downsample(typename Texel<kHalf>::T* __
             float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
-                const f4 a = v[c_pairs[f][p][0]];
-                const f4 b = v[c_pairs[f][p][1]];
+                const f4 a = v[face_pair_child(f, p, 0)];
+                const f4 b = v[face_pair_child(f, p, 1)];
                 const float t = 1.0f - a.w;
                 acc[0] = acc[0] + a.x;
                 acc[0] = acc[0] + t * b.x;

@@ -1,0 +1,713 @@
+// vct_batch.hip - the batched dense method: many scenes of uneven size
+// voxelized, mipped and traced in ONE persistent launch, replayed from a
+// hipGraph.
+//
+// The launch is a pool of resident workgroups pulling tasks from a queue
+// (an atomic head over a task table built on the host). Tasks are ordered by
+// phase - vertex chunks, raster chunks, mip bricks level by level, pixel
+// tiles - and every task of a phase depends only on its own scene's previous
+// phase, tracked by one completion counter per (scene, phase). A workgroup
+// takes a task only while it runs, and every task waits only on tasks earlier
+// in the queue, which were taken by running workgroups, so the waits always
+// end; a bounded spin with a shared abort flag guarantees that every wave
+// exits even if that reasoning were broken. Small scenes reach their trace
+// tiles while large scenes are still voxelizing, and a scene's tiny mip
+// levels never leave the GPU idle.
+//
+// Each task runs the same device code as the per-scene calls
+// (vct_dense_voxelize / vct_dense_mipmap / vct_dense_trace), so the grids are
+// bit-identical to theirs and the traced images equal theirs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "vct_dense.hpp"
+#include "vct_dense_march.hpp"
+#include "vct_dense_mip.hpp"
+#include "vct_internal.hpp"
+#include "vct_raster.hpp"
+#include "vct_texel.hpp"
+
+static_assert(sizeof(vct_batch_tuning) == 16, "vct_batch_tuning layout");
+static_assert(sizeof(vct_batch_scene) == 336, "vct_batch_scene layout");
+
+namespace vct {
+namespace batch {
+
+enum TaskType : uint32_t { kVertex = 0, kRaster = 1, kMip = 2, kTrace = 3 };
+
+// Phase p of a scene: 0 vertex, 1 raster, 1 + m mip level m (m >= 1).
+constexpr int kPhases = 1 + dense::kMaxLevels;
+constexpr int kTile = 16;                    // trace tile edge (pixels)
+constexpr uint32_t kSpinLimit = 1u << 22;    // about a second of polling per wait
+
+struct Task {
+    uint32_t type_scene;  // type << 24 | scene
+    uint32_t a;           // first triangle / brick index / tile index
+    uint32_t b;           // end triangle (vertex, raster)
+    uint32_t level;       // mip level (kMip)
+};
+
+struct Prim {
+    uint32_t first_index, tri_count;
+    int32_t vertex_offset;
+    uint32_t instance_index, material_index, first_tri;
+};
+
+struct SceneDev {
+    const float* positions;
+    const uint32_t* indices;
+    const float* nodes;
+    const vct_gltf_material* materials;
+    const Prim* prims;
+    uint32_t nprims;
+    uint32_t tri_count;
+    float* soup_pos;     // 9 floats per triangle, world space
+    uint32_t* soup_mat;  // material per triangle
+    float grid_min[3];
+    float vs;
+    int res;
+    int samples;  // raster samples
+    uint32_t* winner;
+    void* data;  // pyramid base (RGBA8)
+    dense::Pyramid pyr;
+    dense::March march;
+    const uint32_t* g_diffuse;
+    const uint16_t* g_normal;
+    const float* g_depth;
+    int width, height, tiles_x;
+    float4* out;
+    uint32_t need[kPhases];  // tasks of each phase
+};
+
+struct Args {
+    const Task* tasks;
+    uint32_t ntasks;
+    const SceneDev* scenes;
+    uint32_t* head;
+    uint32_t* done;  // [scene * kPhases + phase]
+    uint32_t* abort_flag;
+    uint32_t* status;
+    unsigned long long* samples;
+    uint32_t mip_brick;
+    uint32_t schedule;
+};
+
+__global__ void k_batch_reset(uint32_t* head, uint32_t* done, uint32_t ndone, uint32_t* abort_flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ndone) done[i] = 0u;
+    if (i == 0) {
+        *head = 0u;
+        *abort_flag = 0u;
+    }
+}
+
+// Thread 0 polls the counter; false if the launch is aborting.
+__device__ bool wait_count(const uint32_t* counter, uint32_t need, uint32_t* abort_flag, uint32_t* status)
+{
+    uint32_t spins = 0;
+    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > kSpinLimit) {
+            __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atomicOr(status, 1u);
+            return false;
+        }
+    }
+    return true;
+}
+
+// msaaVoxelizer.vert:31-37 for triangles [a, b) of one scene (one lane each).
+__device__ void vertex_task(const SceneDev& S, uint32_t a, uint32_t b)
+{
+#pragma clang fp contract(off)
+    const uint32_t t = a + threadIdx.x;
+    if (t >= b) return;
+    uint32_t lo = 0, hi = S.nprims;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (S.prims[mid].first_tri <= t) lo = mid; else hi = mid;
+    }
+    const Prim pr = S.prims[lo];
+    const uint32_t local = t - pr.first_tri;
+    const float* m = S.nodes + 32u * pr.instance_index;
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t vi = (uint32_t)((int64_t)S.indices[pr.first_index + 3u * local + (uint32_t)k] + pr.vertex_offset);
+        const float px = S.positions[3u * vi], py = S.positions[3u * vi + 1], pz = S.positions[3u * vi + 2];
+        for (int r = 0; r < 3; ++r)
+            S.soup_pos[9u * t + 3 * k + r] = ((m[r] * px + m[4 + r] * py) + m[8 + r] * pz) + m[12 + r];
+    }
+    S.soup_mat[t] = pr.material_index;
+}
+
+struct RasterLds {
+    TriSetup setup[256];
+    float pos[256 * 9];
+    uint32_t off[257];
+};
+
+// The dense raster (k_dense_raster) for triangles [a, b) of one scene: the
+// chunk's samples are enumerated through an LDS prefix sum so every lane of
+// the workgroup has work however uneven the triangles are.
+__device__ void raster_task(const SceneDev& S, uint32_t a, uint32_t b, RasterLds& L)
+{
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
+    const uint32_t n = b - a;
+    uint32_t cnt = 0;
+    if ((uint32_t)tid < n) {
+        float p[9];
+        for (int k = 0; k < 9; ++k) p[k] = S.soup_pos[9u * (a + tid) + k] - S.grid_min[k % 3];
+        for (int k = 0; k < 9; ++k) L.pos[9 * tid + k] = p[k];
+        const int lo[3] = {0, 0, 0}, hi[3] = {S.res - 1, S.res - 1, S.res - 1};
+        TriSetup s;
+        if (tri_setup(p, S.vs, lo, hi, s)) {
+            cnt = (uint32_t)s.nu * (uint32_t)s.nv * (uint32_t)S.samples;
+            L.setup[tid] = s;
+        }
+    }
+    // inclusive scan of the counts (Hillis-Steele over 256 entries)
+    L.off[tid + 1] = cnt;
+    if (tid == 0) L.off[0] = 0;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const uint32_t v = tid >= d ? L.off[tid + 1 - d] : 0u;
+        __syncthreads();
+        L.off[tid + 1] += v;
+        __syncthreads();
+    }
+    const uint32_t total = L.off[n];
+    for (uint32_t w = tid; w < total; w += 256) {
+        uint32_t lo = 0, hi = n;  // largest j with off[j] <= w
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (L.off[mid] <= w) lo = mid; else hi = mid;
+        }
+        const uint32_t j = lo;
+        float fp[3];
+        double lam[3];
+        if (!tri_sample(&L.pos[9 * j], L.setup[j], S.vs, S.samples, w - L.off[j], fp, lam)) continue;
+        int ic[3];
+        bool inside = true;
+        for (int k = 0; k < 3; ++k) {
+            const float f = floorf(fp[k] / S.vs);
+            if (!(f >= 0.0f) || f >= (float)S.res) inside = false;
+            ic[k] = (int)f;
+        }
+        if (!inside) continue;
+        atomicMin(&S.winner[((size_t)ic[2] * S.res + ic[1]) * S.res + ic[0]], a + j);
+    }
+}
+
+// One brick of mip level m. Level 1 resolves its 8 level-0 children from the
+// winners (k_dense_resolve), writes them to level 0 and returns the winners
+// to empty for the next launch.
+__device__ void mip_task(const SceneDev& S, uint32_t brick, int m, uint32_t B)
+{
+    const int n = S.res >> m;
+    const uint32_t be = (uint32_t)(n < (int)B ? n : (int)B);
+    const uint32_t nb = ((uint32_t)n + be - 1) / be;
+    const uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
+    const uint32_t cells = be * be * be;
+    char* base = (char*)S.data;
+    const void* src = base + 4u * S.pyr.offset[m - 1];
+    void* dst = base + 4u * S.pyr.offset[m];
+    for (uint32_t i = threadIdx.x; i < cells; i += blockDim.x) {
+        const int x = (int)(bx * be + i % be), y = (int)(by * be + (i / be) % be), z = (int)(bz * be + i / (be * be));
+        if (x >= n || y >= n || z >= n) continue;
+        if (m > 1) {
+            dense_mip_texel<false>(src, dst, n, x, y, z, false);
+            continue;
+        }
+        const size_t D0 = (size_t)S.res + 2, Dd = (size_t)n + 2, dblock = Dd * Dd * Dd;
+        f4 v[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cx = 2 * x + (c & 1), cy = 2 * y + ((c >> 1) & 1), cz = 2 * z + (c >> 2);
+            uint32_t* wp = &S.winner[((size_t)cz * S.res + cy) * S.res + cx];
+            const uint32_t t = *wp;
+            uint32_t packed = 0u;
+            if (t != 0xffffffffu) {
+                const float* bc = S.materials[S.soup_mat[t]].base_color_factor;
+                packed = pack_unorm(f4{bc[0], bc[1], bc[2], 1.0f});
+                *wp = 0xffffffffu;
+            }
+            ((uint32_t*)src)[((size_t)(cz + 1) * D0 + (size_t)(cy + 1)) * D0 + (size_t)(cx + 1)] = packed;
+            v[c] = unpack_unorm(packed);
+        }
+#pragma unroll
+        for (int f = 0; f < kFaces; ++f) {
+            const size_t di = (size_t)f * dblock + ((size_t)(z + 1) * Dd + (size_t)(y + 1)) * Dd + (size_t)(x + 1);
+            ((uint32_t*)dst)[di] = pack_unorm(dense_mip_face(v, f));
+        }
+    }
+}
+
+__device__ void trace_task(const SceneDev& S, uint32_t tile, uint32_t schedule, uint32_t& steps)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = (int)(tile % (uint32_t)S.tiles_x) * kTile + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)(tile / (uint32_t)S.tiles_x) * kTile + (wave >> 1) * 8 + (lane >> 3);
+    if (px >= S.width || py >= S.height) return;
+    float4 o;
+    if (schedule)
+        o = dense::dense_pixel<false, true>(S.data, S.pyr, S.march, px, py, S.width, S.height, S.g_diffuse,
+                                            S.g_normal, S.g_depth, steps);
+    else
+        o = dense::dense_pixel<false, false>(S.data, S.pyr, S.march, px, py, S.width, S.height, S.g_diffuse,
+                                             S.g_normal, S.g_depth, steps);
+    S.out[(size_t)py * (size_t)S.width + (size_t)px] = o;
+}
+
+__global__ __launch_bounds__(256) void k_batch(Args A)
+{
+    __shared__ RasterLds lds;
+    __shared__ uint32_t s_task;
+    __shared__ uint32_t s_go;
+    uint32_t steps = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_task = atomicAdd(A.head, 1u);
+        __syncthreads();
+        const uint32_t ti = s_task;
+        if (ti >= A.ntasks) break;
+        const Task T = A.tasks[ti];
+        const uint32_t type = T.type_scene >> 24, scene = T.type_scene & 0xffffffu;
+        const SceneDev& S = A.scenes[scene];
+        int dep = -1;
+        if (type == kRaster) dep = 0;
+        else if (type == kMip) dep = (int)T.level;  // level 1 waits on raster (phase 1), level m on level m-1
+        else if (type == kTrace) dep = S.pyr.nlevels;  // the last level, phase 1 + (nlevels - 1)
+        if (threadIdx.x == 0) {
+            bool ok = !__hip_atomic_load(A.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (ok && dep >= 0) ok = wait_count(&A.done[scene * kPhases + dep], S.need[dep], A.abort_flag, A.status);
+            s_go = ok ? 1u : 0u;
+        }
+        __syncthreads();
+        if (!s_go) break;
+        __threadfence();
+        switch (type) {
+        case kVertex: vertex_task(S, T.a, T.b); break;
+        case kRaster: raster_task(S, T.a, T.b, lds); break;
+        case kMip: mip_task(S, T.a, (int)T.level, A.mip_brick); break;
+        default: trace_task(S, T.a, A.schedule, steps); break;
+        }
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0 && type != kTrace) {
+            const int phase = type == kVertex ? 0 : (type == kRaster ? 1 : 1 + (int)T.level);
+            __hip_atomic_fetch_add(&A.done[scene * kPhases + phase], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (A.samples) {
+        unsigned long long v = steps;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(A.samples, v);
+    }
+}
+
+}  // namespace batch
+}  // namespace vct
+
+using namespace vct;
+using namespace vct::batch;
+
+struct vct_batch {
+    std::vector<vct_batch_scene> scenes;
+    std::vector<SceneDev> dev;  // host copy of the scene table
+    vct_dense_trace_desc trace{};
+    vct_batch_tuning tuning{256, 8, 0, 2};
+    int cus = 0;
+    DeviceBuffer scene_table, prims, tasks, counters, soup_pos, soup_mat, data, winner;
+    uint32_t ntasks = 0;
+    uint32_t ndone = 0;
+    std::vector<size_t> data_off, winner_off;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long* graph_samples = nullptr;  // samples pointer the graph was captured with
+    bool graph_valid = false;
+};
+
+static void drop_graph(vct_batch* b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_valid = false;
+}
+
+// Task table for the current tuning. Within a phase, larger work first for
+// voxelization and mips; trace tiles of the scenes whose mips finish first
+// (smaller grids) first.
+static vct_status build_tasks(vct_batch* b)
+{
+    const uint32_t ns = (uint32_t)b->scenes.size();
+    std::vector<uint32_t> by_tris(ns), by_res_desc(ns), by_res_asc(ns);
+    for (uint32_t i = 0; i < ns; ++i) by_tris[i] = by_res_desc[i] = by_res_asc[i] = i;
+    std::stable_sort(by_tris.begin(), by_tris.end(),
+                     [&](uint32_t x, uint32_t y) { return b->dev[x].tri_count > b->dev[y].tri_count; });
+    std::stable_sort(by_res_desc.begin(), by_res_desc.end(),
+                     [&](uint32_t x, uint32_t y) { return b->dev[x].res > b->dev[y].res; });
+    std::stable_sort(by_res_asc.begin(), by_res_asc.end(), [&](uint32_t x, uint32_t y) {
+        return b->dev[x].res != b->dev[y].res ? b->dev[x].res < b->dev[y].res
+                                               : b->dev[x].tri_count < b->dev[y].tri_count;
+    });
+    std::vector<Task> t;
+    for (auto& d : b->dev) std::fill(std::begin(d.need), std::end(d.need), 0u);
+    const uint32_t rc = b->tuning.raster_chunk, B = b->tuning.mip_brick;
+    for (uint32_t s : by_tris)
+        for (uint32_t a = 0; a < b->dev[s].tri_count; a += 256, ++b->dev[s].need[0])
+            t.push_back({(kVertex << 24) | s, a, std::min(a + 256u, b->dev[s].tri_count), 0});
+    for (uint32_t s : by_tris)
+        for (uint32_t a = 0; a < b->dev[s].tri_count; a += rc, ++b->dev[s].need[1])
+            t.push_back({(kRaster << 24) | s, a, std::min(a + rc, b->dev[s].tri_count), 0});
+    int max_levels = 0;
+    for (auto& d : b->dev) max_levels = std::max(max_levels, d.pyr.nlevels);
+    for (int m = 1; m < max_levels; ++m)
+        for (uint32_t s : by_res_desc) {
+            SceneDev& d = b->dev[s];
+            if (m >= d.pyr.nlevels) continue;
+            const uint32_t n = (uint32_t)(d.res >> m), be = std::min(n, B), nb = (n + be - 1) / be;
+            for (uint32_t k = 0; k < nb * nb * nb; ++k, ++d.need[1 + m]) t.push_back({(kMip << 24) | s, k, 0, (uint32_t)m});
+        }
+    for (uint32_t s : by_res_asc) {
+        const SceneDev& d = b->dev[s];
+        const uint32_t tiles = (uint32_t)d.tiles_x * (uint32_t)((d.height + kTile - 1) / kTile);
+        for (uint32_t k = 0; k < tiles; ++k) t.push_back({(kTrace << 24) | s, k, 0, 0});
+    }
+    b->ntasks = (uint32_t)t.size();
+    VCT_TRY(ensure(b->tasks, sizeof(Task) * t.size()));
+    VCT_HIP(hipMemcpy(b->tasks.ptr, t.data(), sizeof(Task) * t.size(), hipMemcpyHostToDevice));
+    VCT_HIP(hipMemcpy(b->scene_table.ptr, b->dev.data(), sizeof(SceneDev) * ns, hipMemcpyHostToDevice));
+    return VCT_OK;
+}
+
+static vct_status capture_graph(vct_batch* b, unsigned long long* samples)
+{
+    drop_graph(b);
+    uint32_t* ctr = (uint32_t*)b->counters.ptr;
+    Args A;
+    A.tasks = (const Task*)b->tasks.ptr;
+    A.ntasks = b->ntasks;
+    A.scenes = (const SceneDev*)b->scene_table.ptr;
+    A.head = ctr;
+    A.abort_flag = ctr + 1;
+    A.status = ctr + 2;
+    A.done = ctr + 4;
+    A.samples = samples;
+    A.mip_brick = b->tuning.mip_brick;
+    A.schedule = b->tuning.trace_schedule;
+    const unsigned grid = (unsigned)(b->cus * (int)b->tuning.workgroups_per_cu);
+    VCT_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_batch_reset, dim3((b->ndone + 255) / 256), dim3(256), 0, b->capture, A.head, A.done, b->ndone,
+                       A.abort_flag);
+    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), 0, b->capture, A);
+    hipError_t e = hipStreamEndCapture(b->capture, &b->graph);
+    if (e != hipSuccess) {
+        b->graph = nullptr;
+        return fail(VCT_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    VCT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_samples = samples;
+    b->graph_valid = true;
+    return VCT_OK;
+}
+
+extern "C" {
+
+static vct_status batch_init(vct_batch* b, const vct_batch_scene* scenes, uint32_t count,
+                             const vct_dense_trace_desc* trace);
+
+vct_status vct_batch_create(vct_batch** out, const vct_batch_scene* scenes, uint32_t count,
+                            const vct_dense_trace_desc* trace)
+{
+    if (!out || !scenes || !trace || !count) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument or empty batch");
+    *out = nullptr;
+    if (count > 0xffffffu) return fail(VCT_ERR_INVALID_ARGUMENT, "too many scenes");
+    vct_batch* b = new (std::nothrow) vct_batch();
+    if (!b) return fail(VCT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    const vct_status s = batch_init(b, scenes, count, trace);
+    if (s != VCT_OK) {
+        vct_batch_destroy(b);
+        return s;
+    }
+    *out = b;
+    return VCT_OK;
+}
+
+}  // extern "C"
+
+// Validates the scenes, lays out every scene's soup, pyramid and winners in
+// shared allocations, and builds the task table for the default tuning.
+static vct_status batch_init(vct_batch* b, const vct_batch_scene* scenes, uint32_t count,
+                             const vct_dense_trace_desc* trace)
+{
+    b->scenes.assign(scenes, scenes + count);
+    b->trace = *trace;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return fail(VCT_ERR_HIP, "device query failed");
+    // per-scene validation and sizes
+    std::vector<Prim> prims;
+    size_t tris_total = 0, data_total = 0, winner_total = 0;
+    b->dev.resize(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const vct_batch_scene& sc = scenes[i];
+        const vct_mesh& m = sc.mesh;
+        if (!m.positions || !m.indices || !m.node_matrices || !m.materials || (!m.primitives && m.primitive_count))
+            return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u: mesh buffers must be non-null", i);
+        const uint32_t r = sc.grid.resolution;
+        if (r < 2 || r > 1024 || (r & (r - 1)))
+            return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u: resolution must be a power of two in [2, 1024]", i);
+        if (sc.grid.format != VCT_FORMAT_RGBA8) return fail(VCT_ERR_UNSUPPORTED, "scene %u: batches are RGBA8", i);
+        if (sc.grid.raster_samples != 1 && sc.grid.raster_samples != 4)
+            return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u: raster_samples", i);
+        if (!(sc.grid.extent > 0.0f)) return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u: extent", i);
+        if (!sc.gbuffer.diffuse || !sc.gbuffer.normal || !sc.gbuffer.depth || !sc.gbuffer.width || !sc.gbuffer.height ||
+            !sc.out)
+            return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u: G-buffer and output are required", i);
+        SceneDev& d = b->dev[i];
+        std::memset(&d, 0, sizeof(d));
+        d.positions = m.positions;
+        d.indices = m.indices;
+        d.nodes = m.node_matrices;
+        d.materials = m.materials;
+        const uint32_t pbase = (uint32_t)prims.size();
+        uint64_t nt = 0;
+        for (uint32_t k = 0; k < m.primitive_count; ++k) {
+            const vct_primitive& p = m.primitives[k];
+            if (p.index_count % 3u || (uint64_t)p.first_index + p.index_count > m.index_count ||
+                p.instance_index >= m.instance_count || p.material_index >= m.material_count)
+                return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u primitive %u out of range", i, k);
+            if (!p.index_count) continue;
+            prims.push_back({p.first_index, p.index_count / 3u, p.vertex_offset, p.instance_index, p.material_index,
+                             (uint32_t)nt});
+            nt += p.index_count / 3u;
+        }
+        if (nt > 0xffffffffull / 9u) return fail(VCT_ERR_CAPACITY, "scene %u: too many triangles", i);
+        d.nprims = (uint32_t)prims.size() - pbase;
+        d.prims = (const Prim*)(uintptr_t)pbase;  // rebased below
+        d.tri_count = (uint32_t)nt;
+        d.soup_pos = (float*)(uintptr_t)tris_total;
+        tris_total += nt;
+        d.res = (int)r;
+        d.samples = (int)sc.grid.raster_samples;
+        d.vs = sc.grid.extent / (float)r;
+        for (int k = 0; k < 3; ++k) d.grid_min[k] = sc.grid.grid_min[k];
+        int nl = 0;
+        while ((1u << nl) < r) ++nl;
+        d.pyr.nlevels = nl + 1;
+        if (d.pyr.nlevels > dense::kMaxLevels) return fail(VCT_ERR_UNSUPPORTED, "scene %u: too many levels", i);
+        size_t tex = 0;
+        for (int lv = 0; lv < d.pyr.nlevels; ++lv) {
+            const int n = (int)r >> lv;
+            d.pyr.offset[lv] = tex;
+            d.pyr.block[lv] = (unsigned long long)(n + 2) * (n + 2) * (n + 2);
+            d.pyr.n[lv] = n;
+            d.pyr.inv_vs[lv] = (float)n / sc.grid.extent;
+            tex += d.pyr.block[lv] * (lv == 0 ? 1u : 6u);
+        }
+        d.pyr.vs = d.vs;
+        b->data_off.push_back(data_total);
+        data_total += tex;
+        b->winner_off.push_back(winner_total);
+        winner_total += (size_t)r * r * r;
+        vct_dense_trace_desc td = *trace;
+        vct_dense tmp;  // only the fields make_params reads
+        tmp.desc = sc.grid;
+        tmp.res = (int)r;
+        tmp.nlevels = d.pyr.nlevels;
+        for (int lv = 0; lv < d.pyr.nlevels; ++lv) {
+            tmp.offset[lv] = d.pyr.offset[lv];
+            tmp.block[lv] = d.pyr.block[lv];
+        }
+        VCT_TRY(dense::make_params(&tmp, &sc.camera, &td, &d.pyr, &d.march));
+        d.g_diffuse = sc.gbuffer.diffuse;
+        d.g_normal = sc.gbuffer.normal;
+        d.g_depth = sc.gbuffer.depth;
+        d.width = (int)sc.gbuffer.width;
+        d.height = (int)sc.gbuffer.height;
+        d.tiles_x = (d.width + kTile - 1) / kTile;
+        d.out = (float4*)sc.out;
+    }
+    VCT_TRY(ensure(b->prims, sizeof(Prim) * (prims.size() ? prims.size() : 1)));
+    VCT_TRY(ensure(b->soup_pos, 36u * (tris_total ? tris_total : 1)));
+    VCT_TRY(ensure(b->soup_mat, 4u * (tris_total ? tris_total : 1)));
+    VCT_TRY(ensure(b->data, 4u * data_total));
+    VCT_TRY(ensure(b->winner, 4u * winner_total));
+    b->ndone = count * kPhases;
+    VCT_TRY(ensure(b->counters, 4u * (4 + (size_t)b->ndone)));
+    VCT_TRY(ensure(b->scene_table, sizeof(SceneDev) * count));
+    if (!prims.empty())
+        VCT_HIP(hipMemcpy(b->prims.ptr, prims.data(), sizeof(Prim) * prims.size(), hipMemcpyHostToDevice));
+    for (uint32_t i = 0; i < count; ++i) {
+        SceneDev& d = b->dev[i];
+        d.prims = (const Prim*)b->prims.ptr + (uintptr_t)d.prims;
+        const size_t tb = (size_t)(uintptr_t)d.soup_pos;
+        d.soup_pos = (float*)b->soup_pos.ptr + 9u * tb;
+        d.soup_mat = (uint32_t*)b->soup_mat.ptr + tb;
+        d.data = (uint32_t*)b->data.ptr + b->data_off[i];
+        d.winner = (uint32_t*)b->winner.ptr + b->winner_off[i];
+    }
+    // borders stay zero forever; winners start empty and every launch leaves them empty
+    VCT_HIP(hipMemset(b->data.ptr, 0, 4u * data_total));
+    VCT_HIP(hipMemset(b->winner.ptr, 0xff, 4u * winner_total));
+    VCT_HIP(hipMemset(b->counters.ptr, 0, 4u * (4 + (size_t)b->ndone)));
+    VCT_HIP(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+    VCT_TRY(build_tasks(b));
+    VCT_HIP(hipDeviceSynchronize());
+    return VCT_OK;
+}
+
+extern "C" {
+
+vct_status vct_batch_destroy(vct_batch* b)
+{
+    if (!b) return VCT_OK;
+    hipDeviceSynchronize();
+    drop_graph(b);
+    if (b->capture) hipStreamDestroy(b->capture);
+    DeviceBuffer* bufs[] = {&b->scene_table, &b->prims, &b->tasks, &b->counters, &b->soup_pos,
+                            &b->soup_mat,    &b->data,  &b->winner};
+    for (DeviceBuffer* p : bufs) release(*p);
+    delete b;
+    return VCT_OK;
+}
+
+vct_status vct_batch_set_tuning(vct_batch* b, const vct_batch_tuning* t)
+{
+    if (!b || !t) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (t->raster_chunk != 64 && t->raster_chunk != 128 && t->raster_chunk != 256)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "raster_chunk must be 64, 128 or 256");
+    if (t->mip_brick != 4 && t->mip_brick != 8) return fail(VCT_ERR_INVALID_ARGUMENT, "mip_brick must be 4 or 8");
+    if (t->trace_schedule > 1) return fail(VCT_ERR_INVALID_ARGUMENT, "trace_schedule must be 0 or 1");
+    if (t->workgroups_per_cu < 1 || t->workgroups_per_cu > 4)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "workgroups_per_cu must be 1..4");
+    VCT_HIP(hipDeviceSynchronize());
+    const bool retask = t->raster_chunk != b->tuning.raster_chunk || t->mip_brick != b->tuning.mip_brick;
+    b->tuning = *t;
+    drop_graph(b);
+    if (retask) VCT_TRY(build_tasks(b));
+    return VCT_OK;
+}
+
+vct_status vct_batch_get_tuning(const vct_batch* b, vct_batch_tuning* t)
+{
+    if (!b || !t) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    *t = b->tuning;
+    return VCT_OK;
+}
+
+vct_status vct_batch_run(vct_batch* b, vct_stream stream, unsigned long long* samples_out)
+{
+    if (!b) return fail(VCT_ERR_INVALID_ARGUMENT, "null batch");
+    if (!b->graph_valid || b->graph_samples != samples_out) VCT_TRY(capture_graph(b, samples_out));
+    VCT_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return VCT_OK;
+}
+
+vct_status vct_batch_status(vct_batch* b)
+{
+    if (!b) return fail(VCT_ERR_INVALID_ARGUMENT, "null batch");
+    VCT_HIP(hipDeviceSynchronize());
+    uint32_t st = 0;
+    VCT_HIP(hipMemcpy(&st, (uint32_t*)b->counters.ptr + 2, 4, hipMemcpyDeviceToHost));
+    if (st) {
+        const uint32_t zero = 0;
+        VCT_HIP(hipMemcpy((uint32_t*)b->counters.ptr + 2, &zero, 4, hipMemcpyHostToDevice));
+        return fail(VCT_ERR_HIP, "a batched task gave up waiting on its dependency; results are incomplete");
+    }
+    return VCT_OK;
+}
+
+vct_status vct_batch_autotune(vct_batch* b, vct_stream stream, uint32_t reps, vct_batch_tuning* best,
+                              float* best_ms)
+{
+    if (!b) return fail(VCT_ERR_INVALID_ARGUMENT, "null batch");
+    if (reps == 0) reps = 3;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    VCT_HIP(hipEventCreate(&e0));
+    VCT_HIP(hipEventCreate(&e1));
+    auto time_it = [&](const vct_batch_tuning& t, float* ms) -> vct_status {
+        VCT_TRY(vct_batch_set_tuning(b, &t));
+        VCT_TRY(vct_batch_run(b, stream, nullptr));  // capture + warm-up
+        std::vector<float> v;
+        for (uint32_t r = 0; r < reps; ++r) {
+            VCT_HIP(hipEventRecord(e0, s));
+            VCT_TRY(vct_batch_run(b, stream, nullptr));
+            VCT_HIP(hipEventRecord(e1, s));
+            VCT_HIP(hipEventSynchronize(e1));
+            float x = 0.0f;
+            VCT_HIP(hipEventElapsedTime(&x, e0, e1));
+            v.push_back(x);
+        }
+        std::sort(v.begin(), v.end());
+        *ms = v[v.size() / 2];
+        return vct_batch_status(b);
+    };
+    // coordinate descent over the four knobs, two sweeps
+    static const uint32_t chunks[] = {64, 128, 256}, bricks[] = {4, 8}, scheds[] = {0, 1}, occ[] = {1, 2, 4};
+    vct_batch_tuning cur = b->tuning;
+    float cur_ms = 0.0f;
+    vct_status st = time_it(cur, &cur_ms);
+    for (int sweep = 0; st == VCT_OK && sweep < 2; ++sweep) {
+        for (int knob = 0; st == VCT_OK && knob < 4; ++knob) {
+            const uint32_t* vals = knob == 0 ? chunks : knob == 1 ? bricks : knob == 2 ? scheds : occ;
+            const int nv = knob == 0 ? 3 : knob == 1 ? 2 : knob == 2 ? 2 : 3;
+            for (int k = 0; st == VCT_OK && k < nv; ++k) {
+                vct_batch_tuning t = cur;
+                uint32_t* field = knob == 0 ? &t.raster_chunk : knob == 1 ? &t.mip_brick
+                                : knob == 2 ? &t.trace_schedule : &t.workgroups_per_cu;
+                if (*field == vals[k]) continue;
+                *field = vals[k];
+                float ms = 0.0f;
+                st = time_it(t, &ms);
+                if (st == VCT_OK && ms < cur_ms) {
+                    cur = t;
+                    cur_ms = ms;
+                }
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (st != VCT_OK) return st;
+    VCT_TRY(vct_batch_set_tuning(b, &cur));
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return VCT_OK;
+}
+
+vct_status vct_batch_level(vct_batch* b, uint32_t scene, uint32_t level, void** ptr, size_t* bytes, uint32_t* faces)
+{
+    if (!b || !ptr) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene >= b->dev.size()) return fail(VCT_ERR_INVALID_ARGUMENT, "scene out of range");
+    const SceneDev& d = b->dev[scene];
+    if (level >= (uint32_t)d.pyr.nlevels) return fail(VCT_ERR_INVALID_ARGUMENT, "level out of range");
+    *ptr = (uint32_t*)d.data + d.pyr.offset[level];
+    if (bytes) *bytes = 4u * d.pyr.block[level] * (level == 0 ? 1u : 6u);
+    if (faces) *faces = level == 0 ? 1u : 6u;
+    return VCT_OK;
+}
+
+vct_status vct_batch_read_level(vct_batch* b, vct_stream stream, uint32_t scene, uint32_t level, void* dst,
+                                size_t bytes)
+{
+    void* p;
+    size_t n;
+    VCT_TRY(vct_batch_level(b, scene, level, &p, &n, nullptr));
+    if (!dst || bytes != n) return fail(VCT_ERR_INVALID_ARGUMENT, "buffer is %zu bytes, level is %zu", bytes, n);
+    VCT_HIP(hipMemcpyAsync(dst, p, n, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    VCT_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return VCT_OK;
+}
+
+}  // extern "C"

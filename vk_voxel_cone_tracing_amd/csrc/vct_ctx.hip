@@ -298,7 +298,7 @@ vct_status vct_destroy(vct_ctx* c)
     DeviceBuffer* bufs[] = {&c->soup_pos, &c->soup_nrm, &c->soup_mat, &c->prims, &c->prim_first, &c->counts,
                             &c->offsets, &c->scan_tmp, &c->totals, &c->rec_key[0], &c->rec_key[1], &c->rec_idx[0],
                             &c->rec_idx[1], &c->rec_val, &c->sort_tmp, &c->staging, &c->materials_flat,
-                            &c->svo.frags, &c->svo.pool, &c->svo.counters};
+                            &c->svo.frags, &c->svo.pool, &c->svo.counters, &c->spec_perm};
     for (DeviceBuffer* b : bufs) release(*b);
     if (c->pinned) hipHostFree(c->pinned);
     delete c;

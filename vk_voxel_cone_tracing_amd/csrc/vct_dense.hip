@@ -16,6 +16,7 @@
 #include <new>
 
 #include "vct_dense.hpp"
+#include "vct_dense_mip.hpp"
 #include "vct_internal.hpp"
 #include "vct_raster.hpp"
 #include "vct_texel.hpp"
@@ -133,69 +134,16 @@ __global__ void k_dense_resolve(const uint32_t* __restrict__ winner, const uint3
     else ((uint32_t*)level0)[i] = pack_unorm(f4{bc[0], bc[1], bc[2], 1.0f});
 }
 
-__constant__ int8_t c_dpairs[6][4][2] = {
-    {{0, 1}, {2, 3}, {4, 5}, {6, 7}}, {{1, 0}, {3, 2}, {5, 4}, {7, 6}},
-    {{0, 2}, {1, 3}, {4, 6}, {5, 7}}, {{2, 0}, {3, 1}, {6, 4}, {7, 5}},
-    {{0, 4}, {1, 5}, {2, 6}, {3, 7}}, {{4, 0}, {5, 1}, {6, 2}, {7, 3}},
-};
-
-template <bool kF32>
-__device__ __forceinline__ f4 dense_load(const void* base, size_t i)
-{
-    if (kF32) {
-        const float4 t = ((const float4*)base)[i];
-        return {t.x, t.y, t.z, t.w};
-    }
-    return unpack_unorm(((const uint32_t*)base)[i]);
-}
-
-template <bool kF32>
-__device__ __forceinline__ void dense_store(void* base, size_t i, f4 v)
-{
-    if (kF32) ((float4*)base)[i] = make_float4(v.x, v.y, v.z, v.w);
-    else ((uint32_t*)base)[i] = pack_unorm(v);
-}
-
 // radianceDownSample.comp:95-139 over the whole grid, one level.
 template <bool kF32>
 __global__ __launch_bounds__(256) void k_dense_mip(const void* __restrict__ src, void* __restrict__ dst, int n,
                                                    bool src_single_face)
 {
-#pragma clang fp contract(off)
     const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     const int z = (int)(blockIdx.z * blockDim.z + threadIdx.z);
     if (x >= n || y >= n || z >= n) return;
-    const int np = 2 * n;
-    const size_t Ds = (size_t)np + 2, Dd = (size_t)n + 2;
-    const size_t sblock = Ds * Ds * Ds, dblock = Dd * Dd * Dd;
-    for (int f = 0; f < kFaces; ++f) {
-        const size_t sface = src_single_face ? 0 : (size_t)f * sblock;
-        f4 v[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const size_t si = sface + ((size_t)(2 * z + (i >> 2) + 1) * Ds + (size_t)(2 * y + ((i >> 1) & 1) + 1)) * Ds +
-                              (size_t)(2 * x + (i & 1) + 1);
-            v[i] = dense_load<kF32>(src, si);
-        }
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const f4 a = v[c_dpairs[f][p][0]];
-            const f4 b = v[c_dpairs[f][p][1]];
-            const float t = 1.0f - a.w;
-            acc[0] = acc[0] + a.x;
-            acc[0] = acc[0] + t * b.x;
-            acc[1] = acc[1] + a.y;
-            acc[1] = acc[1] + t * b.y;
-            acc[2] = acc[2] + a.z;
-            acc[2] = acc[2] + t * b.z;
-            acc[3] = acc[3] + a.w;
-            acc[3] = acc[3] + t * b.w;
-        }
-        const size_t di = (size_t)f * dblock + ((size_t)(z + 1) * Dd + (size_t)(y + 1)) * Dd + (size_t)(x + 1);
-        dense_store<kF32>(dst, di, f4{acc[0] * 0.25f, acc[1] * 0.25f, acc[2] * 0.25f, acc[3] * 0.25f});
-    }
+    dense_mip_texel<kF32>(src, dst, n, x, y, z, src_single_face);
 }
 
 }  // namespace vct

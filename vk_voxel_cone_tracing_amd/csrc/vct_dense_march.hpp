@@ -164,8 +164,47 @@ __device__ __forceinline__ f4 sample(const void* __restrict__ data, const Pyrami
     return acc;
 }
 
-// traceCone (voxelConeTracing.frag:341-392) with the dense level rule.
+// One sample of the march: the level's three-face trilinear sample, lerped
+// with the next level when the level is fractional.
 template <bool kF32>
+__device__ __forceinline__ f4 level_sample(const void* __restrict__ data, const Pyramid& P, const float pos[3],
+                                           float lvl, const int face[3], const float w[3])
+{
+    const int lo = (int)floorf(lvl), hi = (int)ceilf(lvl);
+    const float fl = lvl - floorf(lvl);
+    f4 s = sample<kF32>(data, P, lo, pos, face, w);
+    if (hi != lo) {
+        const f4 b = sample<kF32>(data, P, hi, pos, face, w);
+        s.x = s.x * (1.0f - fl) + b.x * fl;
+        s.y = s.y * (1.0f - fl) + b.y * fl;
+        s.z = s.z * (1.0f - fl) + b.z * fl;
+        s.w = s.w * (1.0f - fl) + b.w * fl;
+    }
+    return s;
+}
+
+// Front-to-back accumulation of one sample (voxelConeTracing.frag:370-386).
+__device__ __forceinline__ void accumulate(f4& result, float& occlusion, const f4& s, float lvl, float step, float seg,
+                                           float vs, float decay)
+{
+    const float vsl = vs * exp2f(lvl);
+    const float corr = seg / vsl;
+    const float op = fminf(fmaxf(1.0f - powf(1.0f - s.w, corr), 0.0f), 1.0f);
+    const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
+    result.x += blend * (s.x * corr);
+    result.y += blend * (s.y * corr);
+    result.z += blend * (s.z * corr);
+    result.w += blend * op;
+    occlusion += (1.0f - occlusion) * op / (1.0f + (step + vsl) * decay);
+}
+
+// traceCone (voxelConeTracing.frag:341-392) with the dense level rule.
+// The step sequence t_k does not depend on the data (only the loop exit on
+// full occlusion does), so with kAhead the samples of steps k and k+1 are
+// fetched together and step k+1 is accumulated only if the loop would have
+// reached it: the same arithmetic in the same order, with twice the loads in
+// flight per lane.
+template <bool kF32, bool kAhead = false>
 __device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March& M, const float start_in[3],
                     const float dir[3], uint32_t& steps)
 {
@@ -184,30 +223,37 @@ __device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March
 #pragma unroll
         for (int k = 0; k < 3; ++k) pos[k] = start[k] + dir[k] * step;
         const float lvl = fminf(fmaxf(log2f(diameter * inv_vs), 0.0f), max_level);
-        const int lo = (int)floorf(lvl), hi = (int)ceilf(lvl);
-        const float fl = lvl - floorf(lvl);
-        f4 s = sample<kF32>(data, P, lo, pos, face, w);
-        if (hi != lo) {
-            const f4 b = sample<kF32>(data, P, hi, pos, face, w);
-            s.x = s.x * (1.0f - fl) + b.x * fl;
-            s.y = s.y * (1.0f - fl) + b.y * fl;
-            s.z = s.z * (1.0f - fl) + b.z * fl;
-            s.w = s.w * (1.0f - fl) + b.w * fl;
+        if (!kAhead) {
+            const f4 s = level_sample<kF32>(data, P, pos, lvl, face, w);
+            accumulate(result, occlusion, s, lvl, step, seg, vs, M.decay);
+            const float prev = step;
+            step += fmaxf(diameter, vs) * M.step_factor;
+            seg = step - prev;
+            diameter = step * M.coef;
+            ++steps;
+        } else {
+            const float step_b = step + fmaxf(diameter, vs) * M.step_factor;
+            const float seg_b = step_b - step;
+            const float diam_b = step_b * M.coef;
+            float pos_b[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pos_b[k] = start[k] + dir[k] * step_b;
+            const float lvl_b = fminf(fmaxf(log2f(diam_b * inv_vs), 0.0f), max_level);
+            const f4 sa = level_sample<kF32>(data, P, pos, lvl, face, w);
+            const f4 sb = level_sample<kF32>(data, P, pos_b, lvl_b, face, w);
+            accumulate(result, occlusion, sa, lvl, step, seg, vs, M.decay);
+            ++steps;
+            step = step_b;
+            seg = seg_b;
+            diameter = diam_b;
+            if (!(step < M.max_dist && occlusion < 1.0f)) break;
+            accumulate(result, occlusion, sb, lvl_b, step, seg, vs, M.decay);
+            ++steps;
+            const float prev = step;
+            step += fmaxf(diameter, vs) * M.step_factor;
+            seg = step - prev;
+            diameter = step * M.coef;
         }
-        const float vsl = vs * exp2f(lvl);
-        const float corr = seg / vsl;
-        const float op = fminf(fmaxf(1.0f - powf(1.0f - s.w, corr), 0.0f), 1.0f);
-        const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
-        result.x += blend * (s.x * corr);
-        result.y += blend * (s.y * corr);
-        result.z += blend * (s.z * corr);
-        result.w += blend * op;
-        occlusion += (1.0f - occlusion) * op / (1.0f + (step + vsl) * M.decay);
-        const float prev = step;
-        step += fmaxf(diameter, vs) * M.step_factor;
-        seg = step - prev;
-        diameter = step * M.coef;
-        ++steps;
     }
     return {result.x, result.y, result.z, 1.0f - occlusion};
 }
@@ -234,6 +280,35 @@ __device__ __forceinline__ void pixel_setup(const Pyramid& P, const March& M, in
     albedo[0] = u8_to_f(diffuse & 0xffu);
     albedo[1] = u8_to_f((diffuse >> 8) & 0xffu);
     albedo[2] = u8_to_f((diffuse >> 16) & 0xffu);
+}
+
+// One pixel of the dense trace: 16 (or 32) cones weighted by cos, output
+// (indirect diffuse x albedo x intensity, ambient occlusion).
+template <bool kF32, bool kAhead>
+__device__ __forceinline__ float4 dense_pixel(const void* __restrict__ data, const Pyramid& P, const March& M, int px,
+                                              int py, int width, int height, const uint32_t* __restrict__ g_diffuse,
+                                              const uint16_t* __restrict__ g_normal,
+                                              const float* __restrict__ g_depth, uint32_t& steps)
+{
+    const size_t idx = (size_t)py * (size_t)width + (size_t)px;
+    const float depth = g_depth[idx];
+    if (depth == 1.0f) return make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    float start[3], normal[3], albedo[3];
+    pixel_setup(P, M, px, py, width, height, depth, g_normal + 4 * idx, g_diffuse[idx], start, normal, albedo);
+    f4 ind = {0.0f, 0.0f, 0.0f, 1.0f};
+    for (int i = 0; i < M.ncones; ++i) {
+        const float* d = cone_dir(M, i);
+        const float ct = normal[0] * d[0] + normal[1] * d[1] + normal[2] * d[2];
+        if (ct < 0.0f) continue;
+        const f4 c = march<kF32, kAhead>(data, P, M, start, d, steps);
+        ind.x += c.x * ct;
+        ind.y += c.y * ct;
+        ind.z += c.z * ct;
+        ind.w += c.w * ct;
+    }
+    const float inv = 1.0f / (float)M.ncones;
+    return make_float4(ind.x * inv * albedo[0] * M.diffuse_intensity, ind.y * inv * albedo[1] * M.diffuse_intensity,
+                       ind.z * inv * albedo[2] * M.diffuse_intensity, ind.w * inv * M.ao_factor);
 }
 
 inline vct_status make_params(const vct_dense* g, const vct_camera_ubo* cam, const vct_dense_trace_desc* d,

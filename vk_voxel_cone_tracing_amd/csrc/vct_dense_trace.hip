@@ -25,30 +25,9 @@ __global__ __launch_bounds__(256) void k_dense_trace(const void* __restrict__ da
     const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
     uint32_t steps = 0;
-    if (px < width && py < height) {
-        const size_t idx = (size_t)py * (size_t)width + (size_t)px;
-        float4 o = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-        const float depth = g_depth[idx];
-        if (depth != 1.0f) {
-            float start[3], normal[3], albedo[3];
-            pixel_setup(P, M, px, py, width, height, depth, g_normal + 4 * idx, g_diffuse[idx], start, normal, albedo);
-            f4 ind = {0.0f, 0.0f, 0.0f, 1.0f};
-            for (int i = 0; i < M.ncones; ++i) {
-                const float* d = cone_dir(M, i);
-                const float ct = normal[0] * d[0] + normal[1] * d[1] + normal[2] * d[2];
-                if (ct < 0.0f) continue;
-                const f4 c = march<kF32>(data, P, M, start, d, steps);
-                ind.x += c.x * ct;
-                ind.y += c.y * ct;
-                ind.z += c.z * ct;
-                ind.w += c.w * ct;
-            }
-            const float inv = 1.0f / (float)M.ncones;
-            o = make_float4(ind.x * inv * albedo[0] * M.diffuse_intensity, ind.y * inv * albedo[1] * M.diffuse_intensity,
-                            ind.z * inv * albedo[2] * M.diffuse_intensity, ind.w * inv * M.ao_factor);
-        }
-        out[idx] = o;
-    }
+    if (px < width && py < height)
+        out[(size_t)py * (size_t)width + (size_t)px] =
+            dense_pixel<kF32, false>(data, P, M, px, py, width, height, g_diffuse, g_normal, g_depth, steps);
     if (samples) {
         unsigned long long v = steps;
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

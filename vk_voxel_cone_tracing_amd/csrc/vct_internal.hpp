@@ -89,6 +89,7 @@ struct vct_ctx {
     vct::DeviceBuffer rec_key[2], rec_idx[2], rec_val, sort_tmp;
     vct::DeviceBuffer staging;
     vct::DeviceBuffer materials_flat;
+    vct::DeviceBuffer spec_perm;  // roughness-sorted pixel order of the specular cones
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
 
@@ -110,6 +111,19 @@ __host__ __device__ inline size_t vol_index(int D, int level, int face, int x, i
 {
     return ((((size_t)level * kFaces + (size_t)face) * (size_t)D + (size_t)z) * (size_t)D + (size_t)y) * (size_t)D +
            (size_t)x;
+}
+
+// Children are indexed i = x | y << 1 | z << 2. Face f looks along axis
+// f >> 1 (x, y, z), towards + for even f; its 4 front/back pairs are the child
+// pairs that differ only in that axis' bit, listed by increasing index of the
+// child with the bit clear, the front child being the one with the bit clear
+// for + faces and set for - faces (radianceDownSample.comp:95-139).
+__host__ __device__ constexpr int face_pair_child(int f, int p, int back)
+{
+    const int axis = f >> 1;
+    const int c = ((p >> axis) << (axis + 1)) | (p & ((1 << axis) - 1));
+    const int set = (f & 1) ? !back : back;
+    return set ? c | (1 << axis) : c;
 }
 
 // Mesh vertex stage (msaaVoxelizer.vert:31-37): world-space triangles.

@@ -289,7 +289,100 @@ __device__ __forceinline__ float half_to_float(uint16_t h)
     return (float)f;
 }
 
+// Per-pixel inputs of voxelConeTracing.frag main (:143-200): world position
+// from depth, G-buffer decode, calcMinLevel and the cone start point.
+struct PixelIn {
+    float world[3], view[3], normal[3], start[3];
+    float dc[3], sc[3], ef[3];
+    float rough, metallic, min_level;
+    bool has_emission;
+};
+
+__device__ __forceinline__ void pixel_in(const TraceParams& P, size_t idx, int px, int py, float depth,
+                                         const uint32_t* __restrict__ g_diffuse, const uint16_t* __restrict__ g_normal,
+                                         const uint32_t* __restrict__ g_specular,
+                                         const uint16_t* __restrict__ g_emission, PixelIn& I)
+{
+    const float tcx = ((float)px + 0.5f) / (float)P.width, tcy = ((float)py + 0.5f) / (float)P.height;
+    const float q[4] = {tcx * 2.0f - 1.0f, tcy * 2.0f - 1.0f, depth, 1.0f};
+    float wp[4];
+    for (int k = 0; k < 4; ++k)
+        wp[k] = P.vp_inv[k] * q[0] + P.vp_inv[4 + k] * q[1] + P.vp_inv[8 + k] * q[2] + P.vp_inv[12 + k] * q[3];
+    for (int k = 0; k < 3; ++k) I.world[k] = wp[k] / wp[3];
+    for (int k = 0; k < 3; ++k) I.view[k] = P.eye[k] - I.world[k];
+    {
+        const float l = sqrtf(I.view[0] * I.view[0] + I.view[1] * I.view[1] + I.view[2] * I.view[2]);
+        for (int k = 0; k < 3; ++k) I.view[k] /= l;
+    }
+    const uint32_t dt = g_diffuse[idx], st = g_specular[idx];
+    I.dc[0] = u8_to_f(dt & 0xffu);
+    I.dc[1] = u8_to_f((dt >> 8) & 0xffu);
+    I.dc[2] = u8_to_f((dt >> 16) & 0xffu);
+    I.sc[0] = u8_to_f(st & 0xffu);
+    I.sc[1] = u8_to_f((st >> 8) & 0xffu);
+    I.sc[2] = u8_to_f((st >> 16) & 0xffu);
+    I.rough = u8_to_f(dt >> 24);
+    I.metallic = u8_to_f(st >> 24);
+    const f4 nf = unpack_half(*(const uint2*)(g_normal + 4 * idx));
+    const f4 ef = unpack_half(*(const uint2*)(g_emission + 4 * idx));
+    I.normal[0] = nf.x * 2.0f - 1.0f;
+    I.normal[1] = nf.y * 2.0f - 1.0f;
+    I.normal[2] = nf.z * 2.0f - 1.0f;
+    {
+        const float l = sqrtf(I.normal[0] * I.normal[0] + I.normal[1] * I.normal[1] + I.normal[2] * I.normal[2]);
+        for (int k = 0; k < 3; ++k) I.normal[k] /= l;
+    }
+    I.ef[0] = ef.x;
+    I.ef[1] = ef.y;
+    I.ef[2] = ef.z;
+    I.has_emission = ef.x > 0.0f || ef.y > 0.0f || ef.z > 0.0f;
+    // calcMinLevel (voxelConeTracing.frag:394-414)
+    {
+        const float dx = P.center[0] - I.world[0], dy = P.center[1] - I.world[1], dz = P.center[2] - I.world[2];
+        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float min_radius = P.voxel_size * P.dim * 0.5f;
+        const float ml = fmaxf(log2f(dist / min_radius), 0.0f);
+        const float radius = min_radius * exp2f(ceilf(ml));
+        const float f = dist / radius;
+        I.min_level = f > 0.5f ? ceilf(ml) + (f - 0.5f) * 2.0f : ceilf(ml);
+    }
+    const float vs = P.voxel_size * exp2f(I.min_level);
+    for (int k = 0; k < 3; ++k) I.start[k] = I.world[k] + I.normal[k] * vs * P.start_offset;
+}
+
+__device__ __forceinline__ bool wants_specular(const TraceParams& P, const PixelIn& I)
+{
+    return (P.mode == 6u || P.mode == 8u) && (I.sc[0] > 1e-6f || I.sc[1] > 1e-6f || I.sc[2] > 1e-6f) &&
+           I.metallic > 1e-6f;
+}
+
+// The specular cone (voxelConeTracing.frag:205-216): reflect, aperture
+// max(roughness, 0.05), step factor = voxel size.
 template <bool kHalf>
+__device__ __forceinline__ float4 specular_cone(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P,
+                                                const PixelIn& I, uint32_t& steps)
+{
+    const float in[3] = {-I.view[0], -I.view[1], -I.view[2]};
+    const float dni = I.normal[0] * in[0] + I.normal[1] * in[1] + I.normal[2] * in[2];
+    const float dir[3] = {in[0] - 2.0f * dni * I.normal[0], in[1] - 2.0f * dni * I.normal[1],
+                          in[2] - 2.0f * dni * I.normal[2]};
+    const f4 c = trace_cone<kHalf>(vol, P, I.start, dir, fmaxf(I.rough, 0.05f), 30.0f, I.min_level, P.voxel_size, steps);
+    return make_float4(c.x * I.sc[0] * P.specular_intensity, c.y * I.sc[1] * P.specular_intensity,
+                       c.z * I.sc[2] * P.specular_intensity, 1.0f);
+}
+
+__device__ __forceinline__ void add_samples(unsigned long long* samples, uint32_t steps)
+{
+    if (!samples) return;
+    unsigned long long v = steps;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(samples, v);
+}
+
+// Diffuse cones, direct term and mode composition, one lane per pixel in 8x8
+// wave tiles. With kSplitSpec the specular cone is left to k_trace_specular
+// (which writes out_s); otherwise it is traced here.
+template <bool kHalf, bool kSplitSpec>
 __global__ __launch_bounds__(256) void k_trace(const typename Vol<kHalf>::T* __restrict__ vol, TraceParams P,
                                                const uint32_t* __restrict__ g_diffuse,
                                                const uint16_t* __restrict__ g_normal,
@@ -307,58 +400,17 @@ __global__ __launch_bounds__(256) void k_trace(const typename Vol<kHalf>::T* __r
         float4 dO = make_float4(0.0f, 0.0f, 0.0f, 1.0f), sO = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         const float depth = g_depth[idx];
         if (depth != 1.0f) {
-            const float tcx = ((float)px + 0.5f) / (float)P.width, tcy = ((float)py + 0.5f) / (float)P.height;
-            const float q[4] = {tcx * 2.0f - 1.0f, tcy * 2.0f - 1.0f, depth, 1.0f};
-            float wp[4];
-            for (int k = 0; k < 4; ++k)
-                wp[k] = P.vp_inv[k] * q[0] + P.vp_inv[4 + k] * q[1] + P.vp_inv[8 + k] * q[2] + P.vp_inv[12 + k] * q[3];
-            const float world[3] = {wp[0] / wp[3], wp[1] / wp[3], wp[2] / wp[3]};
-            float view[3] = {P.eye[0] - world[0], P.eye[1] - world[1], P.eye[2] - world[2]};
-            {
-                const float l = sqrtf(view[0] * view[0] + view[1] * view[1] + view[2] * view[2]);
-                view[0] /= l;
-                view[1] /= l;
-                view[2] /= l;
-            }
-            const uint32_t dt = g_diffuse[idx], st = g_specular[idx];
-            const float dc[3] = {u8_to_f(dt & 0xffu), u8_to_f((dt >> 8) & 0xffu), u8_to_f((dt >> 16) & 0xffu)};
-            const float sc[3] = {u8_to_f(st & 0xffu), u8_to_f((st >> 8) & 0xffu), u8_to_f((st >> 16) & 0xffu)};
-            const float rough = u8_to_f(dt >> 24), metallic = u8_to_f(st >> 24);
-            const uint2 nraw = *(const uint2*)(g_normal + 4 * idx);
-            const uint2 eraw = *(const uint2*)(g_emission + 4 * idx);
-            const f4 nf = unpack_half(nraw), ef = unpack_half(eraw);
-            float normal[3] = {nf.x * 2.0f - 1.0f, nf.y * 2.0f - 1.0f, nf.z * 2.0f - 1.0f};
-            {
-                const float l = sqrtf(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
-                normal[0] /= l;
-                normal[1] /= l;
-                normal[2] /= l;
-            }
-            const bool has_emission = ef.x > 0.0f || ef.y > 0.0f || ef.z > 0.0f;
-
-            // calcMinLevel (voxelConeTracing.frag:394-414)
-            float min_level;
-            {
-                const float dx = P.center[0] - world[0], dy = P.center[1] - world[1], dz = P.center[2] - world[2];
-                const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-                const float min_radius = P.voxel_size * P.dim * 0.5f;
-                const float ml = fmaxf(log2f(dist / min_radius), 0.0f);
-                const float radius = min_radius * exp2f(ceilf(ml));
-                const float f = dist / radius;
-                min_level = f > 0.5f ? ceilf(ml) + (f - 0.5f) * 2.0f : ceilf(ml);
-            }
-            const float vs = P.voxel_size * exp2f(min_level);
-            float start[3];
-            for (int k = 0; k < 3; ++k) start[k] = world[k] + normal[k] * vs * P.start_offset;
+            PixelIn I;
+            pixel_in(P, idx, px, py, depth, g_diffuse, g_normal, g_specular, g_emission, I);
             f4 ind = {0.0f, 0.0f, 0.0f, 1.0f};
             const int ncones = P.cones32 ? 32 : 16;
             const float aperture = P.cones32 ? 0.628319f : 0.872665f;
             const bool want_diffuse = (P.mode == 4u || P.mode == 5u || P.mode == 7u || P.mode == 8u);
             for (int i = 0; want_diffuse && i < ncones; ++i) {
                 const float* d = P.cones32 ? c_dirs32[i] : c_dirs16[i];
-                const float ct = normal[0] * d[0] + normal[1] * d[1] + normal[2] * d[2];
+                const float ct = I.normal[0] * d[0] + I.normal[1] * d[1] + I.normal[2] * d[2];
                 if (ct < 0.0f) continue;
-                const f4 c = trace_cone<kHalf>(vol, P, start, d, aperture, 30.0f, min_level, P.step_factor, steps);
+                const f4 c = trace_cone<kHalf>(vol, P, I.start, d, aperture, 30.0f, I.min_level, P.step_factor, steps);
                 ind.x += c.x * ct;
                 ind.y += c.y * ct;
                 ind.z += c.z * ct;
@@ -370,46 +422,33 @@ __global__ __launch_bounds__(256) void k_trace(const typename Vol<kHalf>::T* __r
             ind.z *= inv_n;
             ind.w *= inv_n;
             ind.w *= P.ao_factor;
-            ind.x *= dc[0] * P.diffuse_intensity;
-            ind.y *= dc[1] * P.diffuse_intensity;
-            ind.z *= dc[2] * P.diffuse_intensity;
-
-            float spec[3] = {0.0f, 0.0f, 0.0f};
-            const bool want_spec = (P.mode == 6u || P.mode == 8u);
-            if (want_spec && (sc[0] > 1e-6f || sc[1] > 1e-6f || sc[2] > 1e-6f) && metallic > 1e-6f) {
-                const float I[3] = {-view[0], -view[1], -view[2]};
-                const float dni = normal[0] * I[0] + normal[1] * I[1] + normal[2] * I[2];
-                const float dir[3] = {I[0] - 2.0f * dni * normal[0], I[1] - 2.0f * dni * normal[1],
-                                      I[2] - 2.0f * dni * normal[2]};
-                const f4 c = trace_cone<kHalf>(vol, P, start, dir, fmaxf(rough, 0.05f), 30.0f, min_level,
-                                               P.voxel_size, steps);
-                spec[0] = c.x * sc[0] * P.specular_intensity;
-                spec[1] = c.y * sc[1] * P.specular_intensity;
-                spec[2] = c.z * sc[2] * P.specular_intensity;
-            }
+            ind.x *= I.dc[0] * P.diffuse_intensity;
+            ind.y *= I.dc[1] * P.diffuse_intensity;
+            ind.z *= I.dc[2] * P.diffuse_intensity;
+            if (!kSplitSpec && wants_specular(P, I)) sO = specular_cone<kHalf>(vol, P, I, steps);
             float direct[3] = {0.0f, 0.0f, 0.0f};
             const bool want_direct = (P.mode == 4u || P.mode == 5u || P.mode == 8u);
             if (want_direct) {
-                if (has_emission) {
-                    direct[0] = ef.x;
-                    direct[1] = ef.y;
-                    direct[2] = ef.z;
+                if (I.has_emission) {
+                    for (int k = 0; k < 3; ++k) direct[k] = I.ef[k];
                 } else {
                     float b[3];
-                    brdf(normal, view, P.light_dir, dc, sc, rough, b);
-                    const float visv = trace_visibility(P, world);
+                    brdf(I.normal, I.view, P.light_dir, I.dc, I.sc, I.rough, b);
+                    const float visv = trace_visibility(P, I.world);
                     for (int k = 0; k < 3; ++k) direct[k] = b[k] * visv;
                 }
             }
             switch (P.mode) {
-            case 0: dO = make_float4(dc[0], dc[1], dc[2], 1.0f); break;
-            case 1: dO = make_float4(sc[0], sc[1], sc[2], 1.0f); break;
-            case 2: dO = make_float4(normal[0] * 0.5f + 0.5f, normal[1] * 0.5f + 0.5f, normal[2] * 0.5f + 0.5f, 1.0f); break;
+            case 0: dO = make_float4(I.dc[0], I.dc[1], I.dc[2], 1.0f); break;
+            case 1: dO = make_float4(I.sc[0], I.sc[1], I.sc[2], 1.0f); break;
+            case 2:
+                dO = make_float4(I.normal[0] * 0.5f + 0.5f, I.normal[1] * 0.5f + 0.5f, I.normal[2] * 0.5f + 0.5f, 1.0f);
+                break;
             case 3: {
                 const float colors[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
-                int lower = (int)floorf(min_level);
+                int lower = (int)floorf(I.min_level);
                 lower = lower < 0 ? 0 : (lower > 5 ? 5 : lower);
-                const float fr = min_level - floorf(min_level);
+                const float fr = I.min_level - floorf(I.min_level);
                 dO = make_float4((colors[lower][0] * (1.0f - fr) + colors[lower + 1][0] * fr) * 0.5f,
                                  (colors[lower][1] * (1.0f - fr) + colors[lower + 1][1] * fr) * 0.5f,
                                  (colors[lower][2] * (1.0f - fr) + colors[lower + 1][2] * fr) * 0.5f, 0.5f);
@@ -417,25 +456,112 @@ __global__ __launch_bounds__(256) void k_trace(const typename Vol<kHalf>::T* __r
             }
             case 4: dO = make_float4(direct[0] * ind.w, direct[1] * ind.w, direct[2] * ind.w, 1.0f); break;
             case 5:
-                dO = make_float4(direct[0] * ind.w + ind.x, direct[1] * ind.w + ind.y, direct[2] * ind.w + ind.z, 1.0f);
-                break;
-            case 6: sO = make_float4(spec[0], spec[1], spec[2], 1.0f); break;
-            case 7: dO = make_float4(ind.w, ind.w, ind.w, 1.0f); break;
             case 8:
                 dO = make_float4(direct[0] * ind.w + ind.x, direct[1] * ind.w + ind.y, direct[2] * ind.w + ind.z, 1.0f);
-                sO = make_float4(spec[0], spec[1], spec[2], 1.0f);
                 break;
+            case 7: dO = make_float4(ind.w, ind.w, ind.w, 1.0f); break;
             default: break;
             }
         }
         out_d[idx] = dO;
-        out_s[idx] = sO;
+        if (!kSplitSpec) out_s[idx] = sO;
     }
-    if (samples) {
-        unsigned long long v = steps;
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0 && v) atomicAdd(samples, v);
+    add_samples(samples, steps);
+}
+
+// Specular work ordering. The specular cone's step count is set by the
+// roughness byte alone (aperture max(r, 0.05), step factor = voxel size, 63 to
+// 336 steps over r in [0.1, 0.8]), so lanes of a wave that mixes roughness
+// idle until its smoothest lane finishes. Each 64x64 pixel tile is
+// counting-sorted by roughness in LDS and the permutation written to `perm`
+// (tile t owns perm[t*4096, (t+1)*4096)); pixels without a specular cone go
+// last, unused slots hold 0xffffffff. The order inside one roughness bin
+// depends on LDS atomic order, which changes no pixel's value.
+constexpr int kSpecTile = 64;
+constexpr int kSpecTilePixels = kSpecTile * kSpecTile;
+constexpr int kSpecBins = 257;
+
+__global__ __launch_bounds__(256) void k_spec_bin(TraceParams P, const uint32_t* __restrict__ g_diffuse,
+                                                  const uint32_t* __restrict__ g_specular,
+                                                  const float* __restrict__ g_depth, int tiles_x,
+                                                  uint32_t* __restrict__ perm)
+{
+    __shared__ uint32_t hist[kSpecBins];
+    __shared__ uint32_t cursor[kSpecBins];
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int tx0 = (tile % tiles_x) * kSpecTile, ty0 = (tile / tiles_x) * kSpecTile;
+    for (int b = tid; b < kSpecBins; b += 256) hist[b] = 0u;
+    __syncthreads();
+    uint16_t key[kSpecTilePixels / 256];
+#pragma unroll
+    for (int j = 0; j < kSpecTilePixels / 256; ++j) {
+        const int local = j * 256 + tid;
+        const int px = tx0 + (local % kSpecTile), py = ty0 + (local / kSpecTile);
+        uint16_t k = 0xffffu;
+        if (px < P.width && py < P.height) {
+            const size_t idx = (size_t)py * (size_t)P.width + (size_t)px;
+            k = kSpecBins - 1;
+            if (g_depth[idx] != 1.0f) {
+                const uint32_t st = g_specular[idx];
+                const bool spec = ((st & 0xffu) || ((st >> 8) & 0xffu) || ((st >> 16) & 0xffu)) && (st >> 24);
+                if (spec && (P.mode == 6u || P.mode == 8u)) k = (uint16_t)(g_diffuse[idx] >> 24);
+            }
+            atomicAdd(&hist[k], 1u);
+        }
+        key[j] = k;
     }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int b = 0; b < kSpecBins; ++b) {
+            cursor[b] = run;
+            run += hist[b];
+        }
+    }
+    __syncthreads();
+    uint32_t* out = perm + (size_t)tile * kSpecTilePixels;
+#pragma unroll
+    for (int j = 0; j < kSpecTilePixels / 256; ++j) {
+        const int local = j * 256 + tid;
+        if (key[j] == 0xffffu) continue;
+        const int px = tx0 + (local % kSpecTile), py = ty0 + (local / kSpecTile);
+        const uint32_t pos = atomicAdd(&cursor[key[j]], 1u);
+        out[pos] = (uint32_t)py * (uint32_t)P.width + (uint32_t)px;
+    }
+    // unused slots of an edge tile: after the scatter the last bin's cursor
+    // is the number of pixels the tile holds
+    __syncthreads();
+    const uint32_t used = cursor[kSpecBins - 1];
+    for (uint32_t i = used + tid; i < (uint32_t)kSpecTilePixels; i += 256) out[i] = 0xffffffffu;
+}
+
+// One lane per entry of `perm`: the specular cone of that pixel (or the
+// reference's (0,0,0,1) where none is traced).
+template <bool kHalf>
+__global__ __launch_bounds__(256) void k_trace_specular(const typename Vol<kHalf>::T* __restrict__ vol, TraceParams P,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const uint32_t* __restrict__ g_diffuse,
+                                                        const uint16_t* __restrict__ g_normal,
+                                                        const uint32_t* __restrict__ g_specular,
+                                                        const uint16_t* __restrict__ g_emission,
+                                                        const float* __restrict__ g_depth, float4* __restrict__ out_s,
+                                                        unsigned long long* __restrict__ samples)
+{
+    const uint32_t pi = perm[(size_t)blockIdx.x * 256 + threadIdx.x];
+    uint32_t steps = 0;
+    if (pi != 0xffffffffu) {
+        const int px = (int)(pi % (uint32_t)P.width), py = (int)(pi / (uint32_t)P.width);
+        float4 sO = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        const float depth = g_depth[pi];
+        if (depth != 1.0f) {
+            PixelIn I;
+            pixel_in(P, pi, px, py, depth, g_diffuse, g_normal, g_specular, g_emission, I);
+            if (wants_specular(P, I)) sO = specular_cone<kHalf>(vol, P, I, steps);
+        }
+        out_s[pi] = sO;
+    }
+    add_samples(samples, steps);
 }
 
 }  // namespace vct
@@ -735,6 +861,35 @@ static vct_status make_trace_params(vct_ctx* c, const vct_gbuffer* g, const vct_
     return VCT_OK;
 }
 
+// The specular cone runs in its own roughness-binned pass whenever the mode
+// traces it (modes 6 and 8); the diffuse pass then leaves out_s alone.
+template <bool kHalf>
+static vct_status launch_trace(vct_ctx* c, hipStream_t s, const typename Vol<kHalf>::T* vol, const TraceParams& P,
+                               const vct_gbuffer* g, float4* out_d, float4* out_s, unsigned long long* samples)
+{
+    const dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+    if (P.mode != 6u && P.mode != 8u) {
+        hipLaunchKernelGGL((k_trace<kHalf, false>), grid, dim3(256), 0, s, vol, P, g->diffuse, g->normal, g->specular,
+                           g->emission, g->depth, out_d, out_s, samples);
+        VCT_HIP(hipGetLastError());
+        return VCT_OK;
+    }
+    const int tiles_x = (int)((g->width + kSpecTile - 1) / kSpecTile);
+    const int tiles_y = (int)((g->height + kSpecTile - 1) / kSpecTile);
+    const size_t tiles = (size_t)tiles_x * (size_t)tiles_y;
+    VCT_TRY(ensure(c->spec_perm, 4u * tiles * kSpecTilePixels));
+    uint32_t* perm = (uint32_t*)c->spec_perm.ptr;
+    hipLaunchKernelGGL(k_spec_bin, dim3((unsigned)tiles), dim3(256), 0, s, P, g->diffuse, g->specular, g->depth,
+                       tiles_x, perm);
+    hipLaunchKernelGGL((k_trace<kHalf, true>), grid, dim3(256), 0, s, vol, P, g->diffuse, g->normal, g->specular,
+                       g->emission, g->depth, out_d, out_s, samples);
+    hipLaunchKernelGGL(k_trace_specular<kHalf>, dim3((unsigned)(tiles * (kSpecTilePixels / 256))), dim3(256), 0, s,
+                       vol, P, (const uint32_t*)perm, g->diffuse, g->normal, g->specular, g->emission, g->depth, out_s,
+                       samples);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
 extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer* g, const vct_camera_ubo* cam,
                                 const vct_cone_tracing_desc* d, const vct_shadow_map* shadow,
                                 const vct_directional_light_desc* light,
@@ -746,17 +901,11 @@ extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer
                               c ? c->levels : 0, &P));
     if (d->volume_dimension != (float)c->res)
         return fail(VCT_ERR_INVALID_ARGUMENT, "volume_dimension must equal the resolution");
-    dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
-    hipStream_t s = (hipStream_t)stream;
     if (c->cfg.radiance_format == VCT_FORMAT_RGBA16F)
-        hipLaunchKernelGGL(k_trace<true>, grid, dim3(256), 0, s, (const uint2*)c->radiance.ptr, P, g->diffuse, g->normal,
-                           g->specular, g->emission, g->depth, (float4*)diffuse_out, (float4*)specular_out, samples_out);
-    else
-        hipLaunchKernelGGL(k_trace<false>, grid, dim3(256), 0, s, (const uint32_t*)c->radiance.ptr, P, g->diffuse,
-                           g->normal, g->specular, g->emission, g->depth, (float4*)diffuse_out, (float4*)specular_out,
-                           samples_out);
-    VCT_HIP(hipGetLastError());
-    return VCT_OK;
+        return launch_trace<true>(c, (hipStream_t)stream, (const uint2*)c->radiance.ptr, P, g, (float4*)diffuse_out,
+                                  (float4*)specular_out, samples_out);
+    return launch_trace<false>(c, (hipStream_t)stream, (const uint32_t*)c->radiance.ptr, P, g, (float4*)diffuse_out,
+                               (float4*)specular_out, samples_out);
 }
 
 // OctreeVoxelConeTracing::onUpdate (OctreeVoxelConeTracing.cpp:74-104). The

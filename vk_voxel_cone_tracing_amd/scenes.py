@@ -305,3 +305,22 @@ def light(seed: int, scale: float = 1.0):
     proj = glm.ortho_lh_zo(-half, half, -half, half, 0.1, 3.0 * scale)
     return dict(direction=d.astype(np.float32), color=np.ones(3, np.float32), intensity=np.float32(1.0),
                 view=view, proj=proj, z_near=np.float32(0.1), z_far=np.float32(3.0 * scale))
+
+
+def batch(seed: int = 3, count: int = 64, tri_min: int = 5_000, tri_max: int = 500_000):
+    """Scenes of uneven size for the batched workload: triangle counts
+    log-uniform in [tri_min, tri_max] and dense grid resolutions log-uniform
+    over 64^3..256^3 (the exponent drawn from U[6, 8] and rounded, since the
+    mip pyramid needs a power of two). Each scene is the room with as many of
+    its 16 spheres as fit the budget, topped up with thin triangles.
+    Returns a list of (Scene, resolution)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    room_fixed = 12 + 64 * 12
+    for i in range(count):
+        tris = int(round(np.exp(rng.uniform(np.log(tri_min), np.log(tri_max)))))
+        res = 1 << int(round(rng.uniform(6.0, 8.0)))
+        spheres = int(min(16, max(0, (tris - room_fixed) // 1280)))
+        scene = room(10_000 + 100 * seed + i, scale=1.0, n_boxes=64, n_spheres=spheres, thin_total=tris)
+        out.append((scene, res))
+    return out
