@@ -263,6 +263,9 @@ static const double ORC_DIRS32_F64[32][3] = {
     {0.57735, -0.57735, -0.57735},     {-0.57735, 0.57735, 0.57735},      {-0.57735, 0.57735, -0.57735},
     {-0.57735, -0.57735, 0.57735},     {-0.57735, -0.57735, -0.57735}};
 
+const double* orc_dense_dir16(int i) { return ORC_DIRS16_F64[i]; }
+const double* orc_dense_dir32(int i) { return ORC_DIRS32_F64[i]; }
+
 /* Per pixel: indirect diffuse rgb (x albedo x intensity) and ambient
  * occlusion in alpha, as voxelConeTracing.frag:166-203 with the dense march.
  * out: 4 doubles per pixel. */

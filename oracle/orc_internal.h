@@ -19,4 +19,8 @@ typedef void (*orc_frag_fn)(void* user, uint32_t tri, const float pos[3], const 
 void orc_raster_triangle(const float* p, float voxel_size, const int lo[3], const int hi[3],
                          int samples, orc_frag_fn fn, void* user, uint32_t tri);
 
+/* voxelConeTracing.frag:78-135 cone directions in double (orc_dense.c). */
+const double* orc_dense_dir16(int i);
+const double* orc_dense_dir32(int i);
+
 #endif

@@ -221,6 +221,16 @@ void orc_dense_mip(orc_dense* g);
 void orc_dense_trace(const orc_dense* g, const orc_gbuffer* gb, const orc_dense_params* p, int row_begin,
                      int row_end, double* out, uint64_t* samples);
 
+/* ---- FP64 adjoint of the dense method (orc_adjoint.c) ---- */
+typedef struct orc_adj orc_adj;
+orc_adj* orc_adj_create(int res, float extent, const float grid_min[3]);
+void orc_adj_free(orc_adj* a);
+void orc_adj_set_level0(orc_adj* a, const double* rgba);
+void orc_adj_mip(orc_adj* a);
+void orc_adj_render(orc_adj* a, const orc_gbuffer* gb, const orc_dense_params* P, double* out);
+void orc_adj_backward(orc_adj* a, const orc_gbuffer* gb, const orc_dense_params* P, const double* grad_out,
+                      double* grad_level0);
+
 #ifdef __cplusplus
 }
 #endif

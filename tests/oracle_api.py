@@ -127,6 +127,13 @@ def lib() -> C.CDLL:
         _L.orc_calc_min_level.restype = C.c_float
         _L.orc_svo_fragments.restype = C.c_uint32
         _L.orc_svo_build.restype = C.c_int
+        _L.orc_adj_create.restype = C.c_void_p
+        _L.orc_adj_create.argtypes = [C.c_int, C.c_float, C.POINTER(C.c_float)]
+        for fn in ("orc_adj_free", "orc_adj_mip"):
+            getattr(_L, fn).argtypes = [C.c_void_p]
+        _L.orc_adj_set_level0.argtypes = [C.c_void_p, C.c_void_p]
+        _L.orc_adj_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _L.orc_adj_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     return _L
 
 
@@ -320,3 +327,40 @@ def dense_params(vp_inv, aperture=0.872665, max_dist=1.7320508, step_factor=1.0,
     p.cones32 = cones32
     p.view_proj_inv = (C.c_float * 16)(*glm.flat(vp_inv).tolist())
     return p
+
+
+# --------------------------------------------------------------------------
+# FP64 adjoint of the dense method
+
+
+class Adjoint:
+    """orc_adj: a double-precision level-0 RGBA grid, its mips, render and
+    reverse-mode gradient."""
+
+    def __init__(self, res, extent=1.0, grid_min=(0.0, 0.0, 0.0)):
+        self.res = res
+        self.h = lib().orc_adj_create(res, extent, (C.c_float * 3)(*grid_min))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_adj_free(self.h)
+            self.h = None
+
+    def set_level0(self, rgba):
+        a = np.ascontiguousarray(rgba, np.float64).reshape(self.res, self.res, self.res, 4)
+        lib().orc_adj_set_level0(self.h, a.ctypes.data)
+        lib().orc_adj_mip(self.h)
+
+    def render(self, g: dict, params) -> np.ndarray:
+        H, W = g["depth"].shape
+        out = np.zeros((H, W, 4), np.float64)
+        gs = gbuf_struct(g)
+        lib().orc_adj_render(self.h, C.byref(gs), C.byref(params), out.ctypes.data)
+        return out
+
+    def backward(self, g: dict, params, grad_out: np.ndarray) -> np.ndarray:
+        go = np.ascontiguousarray(grad_out, np.float64)
+        grad = np.zeros((self.res, self.res, self.res, 4), np.float64)
+        gs = gbuf_struct(g)
+        lib().orc_adj_backward(self.h, C.byref(gs), C.byref(params), go.ctypes.data, grad.ctypes.data)
+        return grad

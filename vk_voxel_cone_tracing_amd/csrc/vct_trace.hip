@@ -69,6 +69,10 @@ struct TraceParams {
     float z_near, z_far;
     const float* shadow;
     int sw, sh;
+    // reciprocals of the march's constant divisors
+    float inv_extent0;     // 1 / (voxel_size * dim): level-0 clip extent
+    float inv_voxel_size;  // 1 / voxel_size
+    float inv_min_radius;  // 1 / (voxel_size * dim / 2)
 };
 
 template <bool kHalf>
@@ -97,8 +101,7 @@ template <bool kHalf>
 __device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P,
                                            const float pos[3], int level, const int face[3], const float w[3])
 {
-    const float vs = P.voxel_size * exp2f((float)level);
-    const float inv_extent = 1.0f / (vs * P.dim);
+    const float inv_extent = ldexpf(P.inv_extent0, -level);  // 1 / (voxel_size * 2^level * dim), exact
     int i0[3];
     float fr[3];
 #pragma unroll
@@ -141,7 +144,59 @@ __device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restr
     return acc;
 }
 
-// voxelConeTracing.frag:341-392 traceCone
+// The per-step level of traceCone (voxelConeTracing.frag:357-363):
+// clamp(max(startLevel, log2(d / vs), ceil(log2(|p - c| / r0))), ., L - 1).
+__device__ __forceinline__ float cone_level(const TraceParams& P, const float pos[3], float diameter,
+                                            float start_level, float max_level)
+{
+    const float dx = P.center[0] - pos[0], dy = P.center[1] - pos[1], dz = P.center[2] - pos[2];
+    const float dist = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+    const float min_level = ceilf(log2f(dist * P.inv_min_radius));
+    const float cur = log2f(diameter * P.inv_voxel_size);
+    return fminf(fmaxf(fmaxf(start_level, cur), min_level), max_level);
+}
+
+template <bool kHalf>
+__device__ __forceinline__ f4 cone_sample(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P,
+                                          const float pos[3], float cur, const int face[3], const float w[3])
+{
+    const int lower = (int)floorf(cur);
+    const int upper = (int)ceilf(cur);
+    const float fl = cur - floorf(cur);
+    f4 s = sample_level<kHalf>(vol, P, pos, lower, face, w);
+    if (upper != lower) {
+        const f4 hi = sample_level<kHalf>(vol, P, pos, upper, face, w);
+        s.x = s.x * (1.0f - fl) + hi.x * fl;
+        s.y = s.y * (1.0f - fl) + hi.y * fl;
+        s.z = s.z * (1.0f - fl) + hi.z * fl;
+        s.w = s.w * (1.0f - fl) + hi.w * fl;
+    }
+    return s;
+}
+
+// Front-to-back accumulation of one sample (voxelConeTracing.frag:370-386).
+// pow(1 - a, c) is evaluated as exp2(c * log2(1 - a)) on the hardware
+// transcendental units (a = 1 gives log2(0) = -inf and exp2(-inf) = 0).
+__device__ __forceinline__ void cone_accumulate(const TraceParams& P, f4& result, float& occlusion, const f4& s,
+                                                float cur, float step, float seg)
+{
+    const float voxel_size = P.voxel_size * exp2f(cur);
+    const float correction = seg * (P.inv_voxel_size * exp2f(-cur));
+    float opacity = 1.0f - __builtin_amdgcn_exp2f(correction * __builtin_amdgcn_logf(1.0f - s.w));
+    opacity = fminf(fmaxf(opacity, 0.0f), 1.0f);
+    const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
+    result.x += blend * (s.x * correction);
+    result.y += blend * (s.y * correction);
+    result.z += blend * (s.z * correction);
+    result.w += blend * opacity;
+    occlusion += (1.0f - occlusion) * opacity * __builtin_amdgcn_rcpf(1.0f + (step + voxel_size) * P.decay);
+}
+
+// voxelConeTracing.frag:341-392 traceCone. The step sequence does not depend
+// on the data (only the exit on full occlusion does), so the samples of two
+// consecutive steps are fetched together and the second is accumulated only
+// if the loop would have reached it: the arithmetic and its order are those
+// of the one-step loop, with twice the loads in flight per lane.
 template <bool kHalf>
 __device__ f4 trace_cone(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P, const float start_in[3],
                          const float dir[3], float aperture, float max_dist, float start_level, float step_factor,
@@ -149,54 +204,42 @@ __device__ f4 trace_cone(const typename Vol<kHalf>::T* __restrict__ vol, const T
 {
     f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
     const float cone_coef = 2.0f * tanf(aperture * 0.5f);
-    float voxel_size = P.voxel_size * exp2f(start_level);
+    const float voxel_size0 = P.voxel_size * exp2f(start_level);
     float start[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) start[k] = start_in[k] + dir[k] * voxel_size * P.start_offset * 0.5f;
+    for (int k = 0; k < 3; ++k) start[k] = start_in[k] + dir[k] * voxel_size0 * P.start_offset * 0.5f;
     float step = 0.0f;
     float diameter = fmaxf(step * cone_coef, P.voxel_size);
     float occlusion = 0.0f;
     const int face[3] = {dir[0] > 0.0f ? 0 : 1, dir[1] > 0.0f ? 2 : 3, dir[2] > 0.0f ? 4 : 5};
     const float w[3] = {dir[0] * dir[0], dir[1] * dir[1], dir[2] * dir[2]};
-    float seg = voxel_size;
-    const float min_radius = P.voxel_size * P.dim * 0.5f;
+    float seg = voxel_size0;
     const float max_level = (float)(P.levels - 1);
-    const float inv_vs = 1.0f / P.voxel_size;
     while (step < max_dist && occlusion < 1.0f) {
-        float pos[3];
+        float pos_a[3], pos_b[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) pos[k] = start[k] + dir[k] * step;
-        const float dx = P.center[0] - pos[0], dy = P.center[1] - pos[1], dz = P.center[2] - pos[2];
-        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float min_level = ceilf(log2f(dist / min_radius));
-        float cur = log2f(diameter * inv_vs);
-        cur = fminf(fmaxf(fmaxf(start_level, cur), min_level), max_level);
-        const int lower = (int)floorf(cur);
-        const int upper = (int)ceilf(cur);
-        const float fl = cur - floorf(cur);
-        f4 s = sample_level<kHalf>(vol, P, pos, lower, face, w);
-        if (upper != lower) {
-            const f4 hi = sample_level<kHalf>(vol, P, pos, upper, face, w);
-            s.x = s.x * (1.0f - fl) + hi.x * fl;
-            s.y = s.y * (1.0f - fl) + hi.y * fl;
-            s.z = s.z * (1.0f - fl) + hi.z * fl;
-            s.w = s.w * (1.0f - fl) + hi.w * fl;
-        }
-        voxel_size = P.voxel_size * exp2f(cur);
-        const float correction = seg / voxel_size;
-        float opacity = 1.0f - powf(1.0f - s.w, correction);
-        opacity = fminf(fmaxf(opacity, 0.0f), 1.0f);
-        const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
-        result.x += blend * (s.x * correction);
-        result.y += blend * (s.y * correction);
-        result.z += blend * (s.z * correction);
-        result.w += blend * opacity;
-        occlusion += (1.0f - occlusion) * opacity / (1.0f + (step + voxel_size) * P.decay);
+        for (int k = 0; k < 3; ++k) pos_a[k] = start[k] + dir[k] * step;
+        const float cur_a = cone_level(P, pos_a, diameter, start_level, max_level);
+        const float step_b = step + fmaxf(diameter, P.voxel_size) * step_factor;
+        const float seg_b = step_b - step;
+        const float diam_b = step_b * cone_coef;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) pos_b[k] = start[k] + dir[k] * step_b;
+        const float cur_b = cone_level(P, pos_b, diam_b, start_level, max_level);
+        const f4 sa = cone_sample<kHalf>(vol, P, pos_a, cur_a, face, w);
+        const f4 sb = cone_sample<kHalf>(vol, P, pos_b, cur_b, face, w);
+        cone_accumulate(P, result, occlusion, sa, cur_a, step, seg);
+        ++steps;
+        step = step_b;
+        seg = seg_b;
+        diameter = diam_b;
+        if (!(step < max_dist && occlusion < 1.0f)) break;
+        cone_accumulate(P, result, occlusion, sb, cur_b, step, seg);
+        ++steps;
         const float prev = step;
         step += fmaxf(diameter, P.voxel_size) * step_factor;
         seg = step - prev;
         diameter = step * cone_coef;
-        ++steps;
     }
     return {result.x, result.y, result.z, 1.0f - occlusion};
 }
@@ -834,6 +877,9 @@ static vct_status make_trace_params(vct_ctx* c, const vct_gbuffer* g, const vct_
     P.levels = levels;
     P.res = c->res;
     P.D = c->D;
+    P.inv_extent0 = 1.0f / (d->voxel_size * d->volume_dimension);
+    P.inv_voxel_size = 1.0f / d->voxel_size;
+    P.inv_min_radius = 1.0f / (d->voxel_size * d->volume_dimension * 0.5f);
     for (int k = 0; k < 16; ++k) P.vp_inv[k] = cam->view_proj_inv[k];
     for (int k = 0; k < 3; ++k) P.eye[k] = cam->eye_pos[k];
     P.width = (int)g->width;
