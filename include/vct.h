@@ -406,6 +406,31 @@ vct_status vct_dense_trace(vct_dense* grid, vct_stream stream, const vct_gbuffer
                            const vct_camera_ubo* camera, const vct_dense_trace_desc* desc, float* out,
                            unsigned long long* samples_out);
 
+/* ---- adjoint of the dense method ------------------------------------------
+ * No reference counterpart: gradients of an image loss with respect to the
+ * level-0 albedo (rgb) and opacity (a) of an RGBA32F dense grid, through
+ * vct_dense_mipmap and vct_dense_trace. The gradient pyramid has the grid's
+ * layout (vct_dense_texels texels of 4 floats); the caller zeroes it, the
+ * trace adjoint accumulates into it (float atomics, so the last bits depend
+ * on summation order), the mip adjoint carries levels >= 1 down to level 0.
+ * Typical use: set_level0, mipmap, trace (forward); zero grad; per view
+ * trace_backward; mipmap_backward; level0_grad. */
+enum {
+    VCT_ADJOINT_RECOMPUTE = 1u << 0 /* fetch samples again in the reverse sweep instead of keeping them */
+};
+vct_status vct_dense_texels(vct_dense* grid, size_t* texels);
+/* albedo: res^3 x 3 floats, opacity: res^3 floats (device, x fastest). */
+vct_status vct_dense_set_level0(vct_dense* grid, vct_stream stream, const float* albedo, const float* opacity);
+/* grad_out: d loss / d out of vct_dense_trace (RGBA32F per pixel). `status`
+ * (device uint32, zeroed by the caller) is set non-zero if a march outgrew
+ * the per-lane state (only possible for step counts the call rejects). */
+vct_status vct_dense_trace_backward(vct_dense* grid, vct_stream stream, const vct_gbuffer* gbuffer,
+                                    const vct_camera_ubo* camera, const vct_dense_trace_desc* desc,
+                                    const float* grad_out, float* grad_pyramid, uint32_t flags, uint32_t* status);
+vct_status vct_dense_mipmap_backward(vct_dense* grid, vct_stream stream, float* grad_pyramid);
+vct_status vct_dense_level0_grad(vct_dense* grid, vct_stream stream, const float* grad_pyramid, float* grad_albedo,
+                                 float* grad_opacity);
+
 /* ---- batched dense method ------------------------------------------------
  * No reference counterpart (the reference renders one scene): many scenes of
  * uneven size, each voxelized into its own dense grid (vct_dense_voxelize's

@@ -36,6 +36,7 @@ QUIRK_SVO_HALF_COORDS = 1 << 3
 QUIRK_SVO_MIP_LEVEL_TEST = 1 << 4
 QUIRK_SVO_UNTEXTURED_BLACK = 1 << 5
 QUIRK_ALL = 0x3F
+ADJOINT_RECOMPUTE = 1
 
 MODE_DIFFUSE = 0
 MODE_SPECULAR = 1
@@ -329,6 +330,13 @@ _PROTOS = {
     "vct_dense_trace": (
         C.c_int, [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(DenseTraceDesc), P, P]
     ),
+    "vct_dense_texels": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+    "vct_dense_set_level0": (C.c_int, [P, P, P, P]),
+    "vct_dense_trace_backward": (
+        C.c_int, [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(DenseTraceDesc), P, P, C.c_uint32, P]
+    ),
+    "vct_dense_mipmap_backward": (C.c_int, [P, P, P]),
+    "vct_dense_level0_grad": (C.c_int, [P, P, P, P, P]),
     "vct_batch_create": (C.c_int, [C.POINTER(P), C.POINTER(BatchScene), C.c_uint32, C.POINTER(DenseTraceDesc)]),
     "vct_batch_destroy": (C.c_int, [P]),
     "vct_batch_set_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),

@@ -1,8 +1,11 @@
-"""Builds libvct.so (the gfx950 HIP library) in-tree with hipcc.
+"""Builds libvct.so (the gfx950 HIP library) in-tree with hipcc, and the
+libtorch binding _vct_torch*.so of the dense adjoint on top of it.
 
 Usage: python -m vk_voxel_cone_tracing_amd.build  (or build_all() from Python).
 Every .hip file under csrc/ is compiled to an object in parallel and linked
-into vk_voxel_cone_tracing_amd/libvct.so. The build always recompiles.
+into vk_voxel_cone_tracing_amd/libvct.so; csrc_torch/vct_torch.cpp is
+compiled against PyTorch's headers and linked to libvct.so (rpath $ORIGIN).
+The build always recompiles.
 """
 from __future__ import annotations
 
@@ -67,5 +70,46 @@ def build_all(verbose: bool = True) -> Path:
     return LIB
 
 
+TORCH_SRC = PKG / "csrc_torch" / "vct_torch.cpp"
+
+
+def torch_ext_path() -> Path:
+    import sysconfig
+
+    return PKG / ("_vct_torch" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_torch_ext(verbose: bool = True) -> Path:
+    """The autograd binding: host C++ only (no device code), compiled with
+    hipcc so the HIP runtime headers that ATen's HIP context uses resolve."""
+    import sysconfig
+
+    import torch
+    import torch.utils.cpp_extension as ce
+
+    out = torch_ext_path()
+    tmp = out.with_name(out.name + ".tmp")
+    incs = [f"-I{p}" for p in ce.include_paths(device_type="cuda")] + [f"-I{sysconfig.get_paths()['include']}"]
+    libdirs = ce.library_paths(device_type="cuda")
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = [
+        _hipcc(), "-O2", "-std=c++17", "-fPIC", "-shared", "-w",
+        "-DTORCH_EXTENSION_NAME=_vct_torch", "-DTORCH_API_INCLUDE_EXTENSION_H",
+        f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1",
+        *incs, str(TORCH_SRC), "-o", str(tmp),
+        *[f"-L{d}" for d in libdirs], "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_hip",
+        "-ltorch_python", f"-L{PKG}", "-lvct", "-Wl,-rpath,$ORIGIN",
+        *[f"-Wl,-rpath,{d}" for d in libdirs],
+    ]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"torch binding build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}", file=sys.stderr)
+    return out
+
+
 if __name__ == "__main__":
     build_all()
+    build_torch_ext()
