@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=30, help="rows of the 1080p image traced by the CPU baseline")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the plain-VXAO, SVO, batched and adjoint workloads reported beside the frame")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates the HBM byte counters) and exit")
     return ap.parse_args()
@@ -132,6 +134,25 @@ def frame(st, ev=None):
     st["mgr"].draw_single_render_pass("VoxelConeTracingPass", f)
     if ev is not None:
         ev[3].record()
+
+
+def extras() -> dict:
+    """The other measured workloads (plain dense VXAO 640x360, the 512^3
+    octree from 1M triangles, 64 batched scenes vs their per-scene loop, the
+    256^3 adjoint vs its forward), each on its own synthetic inputs."""
+    from vk_voxel_cone_tracing_amd import workloads as wl
+
+    res = {}
+    for name, fn in (("plain_vxao_640x360", wl.plain_vxao), ("svo_512", wl.svo_build), ("batched_64", wl.batched),
+                     ("adjoint_256", wl.adjoint)):
+        t0 = time.perf_counter()
+        try:
+            res[name] = fn()
+        except Exception as e:  # reported, not fatal: the frame metric above stands on its own
+            res[name] = {"error": f"{type(e).__name__}: {e}"}
+        res[name]["wall_s"] = round(time.perf_counter() - t0, 1)
+        print(f"[bench] {name}: {json.dumps(res[name])}", file=sys.stderr, flush=True)
+    return res
 
 
 def cpu_baseline(st, rows: int):
@@ -286,6 +307,8 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes,
         },
     }
+    if not args.no_extras:
+        out["workloads"] = extras()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(st, args.cpu_rows)
     print(json.dumps(out), flush=True)

@@ -1,0 +1,275 @@
+"""Device set-up of the measured workloads (synthetic, seeded) beside the
+1080p clipmap frame: the plain dense VXAO frame, the 512^3 octree build, the
+batched dense method against its per-scene loop, and the dense adjoint
+against its forward. Every input is produced on the GPU by this package
+(procedural scenes, ray-cast G-buffers); timings use HIP events on the
+stream the work runs on.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import statistics
+
+import numpy as np
+import torch
+
+from . import DeviceScene, _abi, camera_ubo, gbuffer_struct, scenes
+
+A = _abi
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def gbuffer(ctx_h, ds: DeviceScene, scene, cam_triple, W: int, H: int) -> dict:
+    """A G-buffer ray cast on the device (vct_raycast_gbuffer)."""
+    vp, vp_inv, eye = cam_triple
+    g = dict(
+        diffuse=torch.empty((H, W), dtype=torch.int32, device="cuda"),
+        normal=torch.empty((H, W, 4), dtype=torch.int16, device="cuda"),
+        specular=torch.empty((H, W), dtype=torch.int32, device="cuda"),
+        emission=torch.empty((H, W, 4), dtype=torch.int16, device="cuda"),
+        depth=torch.empty((H, W), dtype=torch.float32, device="cuda"),
+    )
+    surface = torch.from_numpy(np.ascontiguousarray(scene.surface)).cuda()
+    cam = camera_ubo(vp, vp_inv, eye)
+    A.call("vct_raycast_gbuffer", ctx_h, _stream(), C.byref(ds.mesh), C.c_void_p(surface.data_ptr()), C.byref(cam),
+           C.c_void_p(g["diffuse"].data_ptr()), C.c_void_p(g["normal"].data_ptr()),
+           C.c_void_p(g["specular"].data_ptr()), C.c_void_p(g["emission"].data_ptr()),
+           C.c_void_p(g["depth"].data_ptr()), W, H)
+    torch.cuda.synchronize()
+    return g
+
+
+def scratch_ctx():
+    h = C.c_void_p()
+    cfg = A.Config(8, 0, 0, A.QUIRK_ALL, 1, 1, 1.0, 0)
+    A.call("vct_create", C.byref(h), C.byref(cfg))
+    return h
+
+
+def time_ms(fn, reps: int = 5, warmup: int = 1) -> float:
+    """Median of `reps` event-timed calls on the current stream."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        t.append(e0.elapsed_time(e1))
+    return statistics.median(t)
+
+
+def grid_box(scene):
+    lo = np.asarray(scene.bb_min, np.float32)
+    ext = float(np.float32((np.asarray(scene.bb_max) - lo).max() * 1.02))
+    return [float(v) for v in (lo - np.float32(0.01 * ext))], ext
+
+
+TRACE_DEFAULTS = dict(aperture=0.872665, min_trace_step_factor=1.0, trace_start_offset=1.0, occlusion_decay=2.0,
+                      ambient_occlusion_factor=0.5, indirect_diffuse_intensity=8.0, enable_32_cones=0)
+
+
+def dense_trace_desc(max_distance: float):
+    d = TRACE_DEFAULTS
+    return A.DenseTraceDesc(d["aperture"], max_distance, d["min_trace_step_factor"], d["trace_start_offset"],
+                            d["occlusion_decay"], d["ambient_occlusion_factor"], d["indirect_diffuse_intensity"],
+                            d["enable_32_cones"])
+
+
+# --------------------------------------------------------------------------
+def plain_vxao(seed: int = 0, res: int = 128, W: int = 640, H: int = 360, reps: int = 5) -> dict:
+    """The plain method: the room (about 21k triangles) voxelized into a
+    res^3 RGBA8 grid, mipped to 1^3, VXAO with 16 cones over a ray-cast W x H
+    G-buffer. One frame = clear + voxelize + mips + trace."""
+    scene = scenes.room(seed, scale=1.0)
+    ds = DeviceScene(scene)
+    sc = scratch_ctx()
+    try:
+        g = gbuffer(sc, ds, scene, scenes.camera(seed, W, H), W, H)
+    finally:
+        A.call("vct_destroy", sc)
+    gb = gbuffer_struct(g)
+    vp, vp_inv, eye = scenes.camera(seed, W, H)
+    cam = camera_ubo(vp, vp_inv, eye)
+    gmin, ext = grid_box(scene)
+    h = C.c_void_p()
+    A.call("vct_dense_create", C.byref(h), C.byref(A.DenseDesc(A.f3(*gmin), ext, res, 0, 1, 0)))
+    out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    samples = torch.zeros(1, dtype=torch.int64, device="cuda")
+    td = dense_trace_desc(float(np.sqrt(3.0)) * ext)
+    s = _stream()
+
+    def frame():
+        A.call("vct_dense_clear", h, s)
+        A.call("vct_dense_voxelize", h, s, C.byref(ds.mesh))
+        A.call("vct_dense_mipmap", h, s)
+        A.call("vct_dense_trace", h, s, C.byref(gb), C.byref(cam), C.byref(td), C.c_void_p(out.data_ptr()),
+               C.c_void_p(samples.data_ptr()))
+
+    try:
+        ms = time_ms(frame, reps)
+        samples.zero_()
+        frame()
+        torch.cuda.synchronize()
+        n = int(samples.item())
+        trace_ms = time_ms(lambda: A.call("vct_dense_trace", h, s, C.byref(gb), C.byref(cam), C.byref(td),
+                                          C.c_void_p(out.data_ptr()), None), reps)
+    finally:
+        A.call("vct_dense_destroy", h)
+    return dict(workload=f"plain VXAO: room seed {seed} ({scene.triangle_count} tris), {res}^3 RGBA8 dense grid "
+                         f"mipped to 1^3, 16 cones, {W}x{H}", ms_per_frame=round(ms, 3), trace_ms=round(trace_ms, 3),
+                cone_samples=n, cone_samples_per_s=round(n / (trace_ms * 1e-3), 1))
+
+
+# --------------------------------------------------------------------------
+def svo_build(seed: int = 1, level: int = 9, total_tris: int = 1_000_000, reps: int = 3) -> dict:
+    """The 512^3 sparse voxel octree: the room plus thin triangles up to
+    `total_tris`, fragments (K9) then flag / allocate / init per level and the
+    leaf and mip writes (K10-K15)."""
+    from . import Context, light_structs, shadow_struct
+
+    scene = scenes.room(seed, scale=1.0, thin_total=total_tris)
+    ds = DeviceScene(scene)
+    ctx = Context(resolution=16, levels=1, raster_samples=1, downsample_region_size=1, extent_l0=1.0)
+    light = scenes.light(seed, scale=1.0)
+    ld, ls = light_structs(light)
+    depth = torch.empty((2048, 2048), dtype=torch.float32, device="cuda")
+    s = _stream()
+    A.call("vct_raycast_shadow", ctx._h, s, C.byref(ds.mesh), C.byref(ls), C.c_void_p(depth.data_ptr()), 2048, 2048)
+    sm = shadow_struct(depth)
+    nfrag = C.c_uint32()
+
+    def build():
+        A.call("vct_svo_voxelize", ctx._h, s, C.byref(ds.mesh), C.byref(sm), C.byref(ld), C.byref(ls), level,
+               C.byref(nfrag))
+        A.call("vct_svo_build", ctx._h, s)
+
+    ms = time_ms(build, reps)
+    n = C.c_size_t()
+    A.call("vct_svo_read_nodes", ctx._h, s, None, 0, C.byref(n))
+    return dict(workload=f"SVO {1 << level}^3: room seed {seed} + thin triangles, {scene.triangle_count} tris",
+                ms_per_build=round(ms, 3), fragments=int(nfrag.value), nodes=int(n.value),
+                nodes_per_s=round(n.value / (ms * 1e-3), 1))
+
+
+# --------------------------------------------------------------------------
+def batched(seed: int = 3, count: int = 64, W: int = 512, H: int = 512, reps: int = 3) -> dict:
+    """64 scenes of uneven size (5k-500k triangles, 64^3-256^3 grids,
+    log-uniform) voxelized, mipped and VXAO-traced at W x H: the per-scene
+    loop of vct_dense_* calls against one replay of the batched launch
+    (after vct_batch_autotune)."""
+    items = []
+    sc = scratch_ctx()
+    try:
+        for i, (scene, res) in enumerate(scenes.batch(seed, count)):
+            ds = DeviceScene(scene)
+            cam3 = scenes.camera(3000 + i, W, H)
+            g = gbuffer(sc, ds, scene, cam3, W, H)
+            gmin, ext = grid_box(scene)
+            items.append(dict(scene=scene, ds=ds, res=res, g=g, gb=gbuffer_struct(g), cam=camera_ubo(*cam3),
+                              gmin=gmin, ext=ext, out=torch.empty((H, W, 4), dtype=torch.float32, device="cuda")))
+    finally:
+        A.call("vct_destroy", sc)
+    td = dense_trace_desc(float(np.sqrt(3.0)))  # the scenes are unit rooms
+    s = _stream()
+    # per-scene loop (grids created once, outside the timing)
+    grids = []
+    for it in items:
+        h = C.c_void_p()
+        A.call("vct_dense_create", C.byref(h), C.byref(A.DenseDesc(A.f3(*it["gmin"]), it["ext"], it["res"], 0, 1, 0)))
+        grids.append(h)
+
+    def loop():
+        for h, it in zip(grids, items):
+            A.call("vct_dense_clear", h, s)
+            A.call("vct_dense_voxelize", h, s, C.byref(it["ds"].mesh))
+            A.call("vct_dense_mipmap", h, s)
+            A.call("vct_dense_trace", h, s, C.byref(it["gb"]), C.byref(it["cam"]), C.byref(td),
+                   C.c_void_p(it["out"].data_ptr()), None)
+
+    try:
+        loop_ms = time_ms(loop, reps)
+    finally:
+        for h in grids:
+            A.call("vct_dense_destroy", h)
+    arr = (A.BatchScene * len(items))()
+    for k, it in enumerate(items):
+        arr[k].mesh = it["ds"].mesh
+        arr[k].grid = A.DenseDesc(A.f3(*it["gmin"]), it["ext"], it["res"], 0, 1, 0)
+        arr[k].gbuffer = it["gb"]
+        arr[k].camera = it["cam"]
+        arr[k].out = it["out"].data_ptr()
+    b = C.c_void_p()
+    A.call("vct_batch_create", C.byref(b), arr, len(items), C.byref(td))
+    try:
+        default_ms = time_ms(lambda: A.call("vct_batch_run", b, s, None), reps)
+        best = A.BatchTuning()
+        tuned = C.c_float()
+        A.call("vct_batch_autotune", b, s, 2, C.byref(best), C.byref(tuned))
+        batch_ms = time_ms(lambda: A.call("vct_batch_run", b, s, None), reps)
+        A.call("vct_batch_status", b)
+    finally:
+        A.call("vct_batch_destroy", b)
+    tris = sum(it["scene"].triangle_count for it in items)
+    return dict(workload=f"{count} scenes, {tris} tris in all, grids 64^3-256^3, VXAO {W}x{H} each",
+                per_scene_loop_ms=round(loop_ms, 3), batched_default_ms=round(default_ms, 3),
+                batched_ms=round(batch_ms, 3), speedup=round(loop_ms / batch_ms, 3),
+                tuning=dict(raster_chunk=best.raster_chunk, mip_brick=best.mip_brick,
+                            trace_schedule=best.trace_schedule, workgroups_per_cu=best.workgroups_per_cu))
+
+
+# --------------------------------------------------------------------------
+def adjoint(seed: int = 4, res: int = 256, views: int = 16, W: int = 512, H: int = 512, reps: int = 3) -> dict:
+    """fp32 res^3 albedo + opacity grid, `views` seeded views: forward render
+    and the L2 loss against targets rendered from a perturbed grid
+    (sigma 0.05), then the backward through cone marching and mips."""
+    from . import adjoint as adj
+
+    scene = scenes.room(seed, scale=1.0)
+    ds = DeviceScene(scene)
+    sc = scratch_ctx()
+    cams = [scenes.camera(4000 + v, W, H) for v in range(views)]
+    try:
+        gs = [gbuffer(sc, ds, scene, c, W, H) for c in cams]
+    finally:
+        A.call("vct_destroy", sc)
+    stack = lambda k: torch.stack([g[k] for g in gs]).contiguous()  # noqa: E731
+    diffuse, normal, depth = stack("diffuse"), stack("normal"), stack("depth")
+    cam_t = adj.cameras_tensor(cams)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    albedo = torch.rand((res, res, res, 3), generator=gen, device="cuda") * 0.6 + 0.2
+    opacity = torch.rand((res, res, res), generator=gen, device="cuda") * 0.5 + 0.05
+    p = adj.TraceParams(max_distance=float(np.sqrt(3.0)))
+    with torch.no_grad():
+        ta = (albedo + 0.05 * torch.randn(albedo.shape, generator=gen, device="cuda")).clamp(0.0, 1.0)
+        to = (opacity + 0.05 * torch.randn(opacity.shape, generator=gen, device="cuda")).clamp(0.0, 0.95)
+        target = adj.render(ta, to, diffuse, normal, depth, cam_t, (0.0, 0.0, 0.0), 1.0, p)
+    a = albedo.clone().requires_grad_(True)
+    o = opacity.clone().requires_grad_(True)
+
+    def fwd():
+        with torch.no_grad():
+            out = adj.render(a, o, diffuse, normal, depth, cam_t, (0.0, 0.0, 0.0), 1.0, p)
+            return ((out - target) ** 2).sum()
+
+    res_out = {}
+    fwd_ms = time_ms(fwd, reps)
+    for recompute in (False, True):
+        def step():
+            a.grad = None
+            o.grad = None
+            out = adj.render(a, o, diffuse, normal, depth, cam_t, (0.0, 0.0, 0.0), 1.0, p, recompute=recompute)
+            ((out - target) ** 2).sum().backward()
+
+        res_out["recompute" if recompute else "store"] = round(time_ms(step, reps), 3)
+    best = min(res_out.values())
+    return dict(workload=f"adjoint: fp32 {res}^3 albedo+opacity, {views} views {W}x{H}, L2 vs targets from a "
+                         f"perturbed grid (sigma 0.05)", forward_ms=round(fwd_ms, 3),
+                forward_backward_ms=res_out, backward_over_forward=round((best - fwd_ms) / fwd_ms, 3))
