@@ -171,8 +171,10 @@ __global__ __launch_bounds__(256) void k_dense_trace_bwd(const void* __restrict_
                                                          float* __restrict__ grad, uint32_t* __restrict__ status)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const unsigned tiles_x = ((unsigned)width + 15u) / 16u;
+    const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
     if (px >= width || py >= height) return;
     const size_t idx = (size_t)py * (size_t)width + (size_t)px;
     const float depth = g_depth[idx];
@@ -303,7 +305,7 @@ vct_status vct_dense_trace_backward(vct_dense* g, vct_stream stream, const vct_g
     VCT_TRY(dense::make_params(g, cam, d, &P, &M));
     if (march_steps(P, M) + 1 > dense::kAdjMaxSteps)
         return fail(VCT_ERR_UNSUPPORTED, "the march takes more than %d steps", dense::kAdjMaxSteps - 1);
-    dim3 grid((gb->width + 15) / 16, (gb->height + 15) / 16);
+    dim3 grid(((gb->width + 15) / 16) * ((gb->height + 15) / 16));
     hipStream_t s = (hipStream_t)stream;
     if (flags & VCT_ADJOINT_RECOMPUTE)
         hipLaunchKernelGGL(dense::k_dense_trace_bwd<false>, grid, dim3(256), 0, s, g->data.ptr, P, M, gb->diffuse,

@@ -22,8 +22,10 @@ __global__ __launch_bounds__(256) void k_dense_trace(const void* __restrict__ da
                                                      float4* __restrict__ out, unsigned long long* __restrict__ samples)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const unsigned tiles_x = ((unsigned)width + 15u) / 16u;
+    const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
     uint32_t steps = 0;
     if (px < width && py < height)
         out[(size_t)py * (size_t)width + (size_t)px] =
@@ -50,7 +52,7 @@ extern "C" vct_status vct_dense_trace(vct_dense* g, vct_stream stream, const vct
     dense::Pyramid P;
     dense::March M;
     VCT_TRY(dense::make_params(g, cam, d, &P, &M));
-    dim3 grid((gb->width + 15) / 16, (gb->height + 15) / 16);
+    dim3 grid(((gb->width + 15) / 16) * ((gb->height + 15) / 16));
     hipStream_t s = (hipStream_t)stream;
     if (g->texel_bytes == 16)
         hipLaunchKernelGGL(dense::k_dense_trace<true>, grid, dim3(256), 0, s, g->data.ptr, P, M, gb->diffuse,

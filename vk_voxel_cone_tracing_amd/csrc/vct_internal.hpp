@@ -126,6 +126,17 @@ __host__ __device__ constexpr int face_pair_child(int f, int p, int back)
     return set ? c | (1 << axis) : c;
 }
 
+// Workgroups of a launch are dispatched to the device's 8 XCDs round-robin
+// (workgroup b runs on XCD b % 8) and every XCD has its own L2. This maps the
+// dispatch index b of n workgroups to a work index such that each XCD walks
+// one contiguous eighth of the work: neighbouring pixel tiles, whose cones
+// fetch overlapping voxels, then share an L2. It is a bijection on [0, n).
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n)
+{
+    const unsigned xcd = b & 7u, k = b >> 3, per = n >> 3, rem = n & 7u;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
 // Mesh vertex stage (msaaVoxelizer.vert:31-37): world-space triangles.
 vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, TriSoup* out);
 

@@ -435,8 +435,10 @@ __global__ __launch_bounds__(256) void k_trace(const typename Vol<kHalf>::T* __r
                                                float4* __restrict__ out_s, unsigned long long* __restrict__ samples)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const unsigned tiles_x = ((unsigned)P.width + 15u) / 16u;
+    const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
     uint32_t steps = 0;
     if (px < P.width && py < P.height) {
         const size_t idx = (size_t)py * (size_t)P.width + (size_t)px;
@@ -591,7 +593,7 @@ __global__ __launch_bounds__(256) void k_trace_specular(const typename Vol<kHalf
                                                         const float* __restrict__ g_depth, float4* __restrict__ out_s,
                                                         unsigned long long* __restrict__ samples)
 {
-    const uint32_t pi = perm[(size_t)blockIdx.x * 256 + threadIdx.x];
+    const uint32_t pi = perm[(size_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x];
     uint32_t steps = 0;
     if (pi != 0xffffffffu) {
         const int px = (int)(pi % (uint32_t)P.width), py = (int)(pi / (uint32_t)P.width);
@@ -710,8 +712,10 @@ __global__ __launch_bounds__(256) void k_svo_trace(const uint2* __restrict__ poo
                                                    float4* __restrict__ out_s, unsigned long long* __restrict__ samples)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const unsigned tiles_x = ((unsigned)P.width + 15u) / 16u;
+    const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
     uint32_t steps = 0;
     if (px < P.width && py < P.height) {
         const size_t idx = (size_t)py * (size_t)P.width + (size_t)px;
@@ -913,7 +917,7 @@ template <bool kHalf>
 static vct_status launch_trace(vct_ctx* c, hipStream_t s, const typename Vol<kHalf>::T* vol, const TraceParams& P,
                                const vct_gbuffer* g, float4* out_d, float4* out_s, unsigned long long* samples)
 {
-    const dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+    const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
     if (P.mode != 6u && P.mode != 8u) {
         hipLaunchKernelGGL((k_trace<kHalf, false>), grid, dim3(256), 0, s, vol, P, g->diffuse, g->normal, g->specular,
                            g->emission, g->depth, out_d, out_s, samples);
@@ -980,7 +984,7 @@ extern "C" vct_status vct_svo_trace(vct_ctx* c, vct_stream stream, const vct_gbu
     }
     S.ev = std::fmax(ext[0], std::fmax(ext[1], ext[2])) * 0.5f;
     if (!(S.ev > 0.0f)) return fail(VCT_ERR_INVALID_ARGUMENT, "empty scene box");
-    dim3 grid((g->width + 15) / 16, (g->height + 15) / 16);
+    dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
     hipLaunchKernelGGL(k_svo_trace, grid, dim3(256), 0, (hipStream_t)stream, (const uint2*)c->svo.pool.ptr, P, S,
                        g->diffuse, g->normal, g->specular, g->emission, g->depth, (float4*)diffuse_out,
                        (float4*)specular_out, samples_out);
