@@ -268,6 +268,16 @@ vct_status vct_get_clip_regions(const vct_ctx* ctx, vct_clipmap_region* regions,
  * VoxelizationPass::calculateChangeDelta (VoxelizationPass.cpp:438-449) from the
  * initial regions of createVoxelClipmap. Writes the regions into ctx. */
 vct_status vct_update_clip_regions(vct_ctx* ctx, const float camera_pos[3]);
+/* Incremental (toroidal) update for a moving camera:
+ * VoxelizationPass::fillRevoxelizationRegions (VoxelizationPass.cpp:450-494)
+ * with calculateChangeDelta (:438-449) taken from the CURRENT regions. Moves
+ * every level's region in ctx and writes the regions that must be cleared and
+ * voxelized again: up to 3 slabs per level (one per axis that moved by at
+ * least the level's minimum change) or the whole region when a move reaches
+ * the extent. revox[level * 3 + i], i < counts[level]; both arrays hold
+ * `levels` entries (revox: 3 per level). */
+vct_status vct_move_clip_regions(vct_ctx* ctx, const float camera_pos[3], vct_clipmap_region* revox,
+                                 uint32_t* counts);
 /* Host half of Voxelizer::cmdVoxelize (Voxelizer.cpp:225-262). */
 vct_status vct_make_voxelization_desc(const vct_ctx* ctx, uint32_t level,
                                       const vct_clipmap_region* region,

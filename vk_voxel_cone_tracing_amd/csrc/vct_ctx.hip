@@ -356,6 +356,43 @@ vct_status vct_update_clip_regions(vct_ctx* c, const float cam[3])
     return VCT_OK;
 }
 
+vct_status vct_move_clip_regions(vct_ctx* c, const float cam[3], vct_clipmap_region* revox, uint32_t* counts)
+{
+    if (!c || !cam || !revox || !counts) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->regions_set) return fail(VCT_ERR_NOT_READY, "the regions have not been set");
+    static const int32_t kMinChange[8] = {2, 2, 2, 2, 2, 1, 1, 1};  // VoxelizationPass.h:57
+    for (int i = 0; i < c->levels; ++i) {
+        vct_clipmap_region& reg = c->regions[i];
+        const float vs = reg.voxel_size;
+        // camera box of level i (Application.cpp:116-128)
+        const float half = (float)((((uint32_t)c->cfg.extent_l0) >> 1) * (1u << i));
+        int32_t delta[3];
+        bool full = false;
+        for (int k = 0; k < 3; ++k) {
+            const float delta_w = (cam[k] - half) - (float)reg.min_corner[k] * vs;
+            const float min_change = vs * (float)kMinChange[i];
+            delta[k] = (int32_t)std::trunc(delta_w / min_change) * kMinChange[i];
+            reg.min_corner[k] += delta[k];
+            if ((uint32_t)std::abs(delta[k]) >= reg.extent[k]) full = true;
+        }
+        uint32_t n = 0;
+        if (full) {
+            revox[3 * i + n++] = reg;
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                const int32_t ad = std::abs(delta[k]);
+                if (ad < kMinChange[i]) continue;
+                vct_clipmap_region slab = reg;
+                slab.extent[k] = (uint32_t)ad;
+                slab.min_corner[k] = delta[k] > 0 ? reg.min_corner[k] + (int32_t)reg.extent[k] - ad : reg.min_corner[k];
+                revox[3 * i + n++] = slab;
+            }
+        }
+        counts[i] = n;
+    }
+    return VCT_OK;
+}
+
 vct_status vct_make_voxelization_desc(const vct_ctx* c, uint32_t level, const vct_clipmap_region* region,
                                       vct_voxelization_desc* d)
 {

@@ -64,18 +64,40 @@ class RenderPassManager:
 class VoxelizationPass(RenderPassBase):
     """VoxelizationPass (VoxelizationPass.cpp:74-212): full revoxelization of
     every clip level (the reference forces it, :86), then the opacity mips of
-    levels 1..N-1 and the opacity border wrap."""
+    levels 1..N-1 and the opacity border wrap.
 
-    def __init__(self, ctx: Context, scene):
+    With `incremental` the pass follows a moving camera the way the
+    reference's disabled path does (:89-99, 129-141, 438-494): the first frame
+    voxelizes everything; later frames move the regions (`camera_pos` of the
+    frame), clear only the slabs that scrolled in (ClipmapCleaner, toroidal)
+    and voxelize only those."""
+
+    def __init__(self, ctx: Context, scene, incremental: bool = False):
         super().__init__()
         self.ctx = ctx
         self.scene = scene
+        self.incremental = incremental
+        self.camera_pos = None
+        self._revox = None
 
     def on_begin_render_pass(self, frame):
+        if self.incremental and self._revox is not None and self.camera_pos is not None:
+            self._revox = self.ctx.move_clip_regions(self.camera_pos)
+            for level, slabs in enumerate(self._revox):
+                for mn, ext, _ in slabs:
+                    self.ctx.clear_region(VOLUME_OPACITY, mn, ext, level, stream=frame.stream)
+            return
         # vkCmdClearColorImage of the opacity atlas (:104-125)
         self.ctx.clear_volume(VOLUME_OPACITY, frame.stream)
+        self._revox = [[r] for r in self.ctx.regions]
 
     def on_update(self, frame):
+        if self.incremental:
+            for level, slabs in enumerate(self._revox):
+                for region in slabs:
+                    desc = self.ctx.voxelization_desc(level, region)
+                    self.ctx.voxelize_opacity(self.scene, desc, level, frame.stream)
+            return
         for level in range(self.ctx.levels):
             desc = self.ctx.voxelization_desc(level)
             self.ctx.voxelize_opacity(self.scene, desc, level, frame.stream)
