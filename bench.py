@@ -298,7 +298,7 @@ def main():
                       "cone_trace": round(t_trace, 3)},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_trace (cone trace)",
+            "kernel": "cone-trace pass: k_spec_bin + k_trace<half,split> + k_trace_specular<half>",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -492,6 +492,23 @@ vct_status vct_batch_level(vct_batch* batch, uint32_t scene, uint32_t level, voi
 vct_status vct_batch_read_level(vct_batch* batch, vct_stream stream, uint32_t scene, uint32_t level, void* dst,
                                 size_t bytes);
 
+/* ---- the step after the trace ---------------------------------------------
+ * SpecularFilterPass (SpecularFilterPass.cpp:75-90, specularFilter.frag:25-53):
+ * out = diffuse + filtered specular (filter_method 1: the 32-direction x
+ * 8-ring blur of filter.glsl:9-24; otherwise the 15x15 bilateral filter of
+ * filter.glsl:26-63), then the Uncharted2 tonemap (tonemapping.glsl:9-26)
+ * when tonemap_enable == 1. Images are RGBA32F width x height (device).
+ * Defaults of the reference: gamma 2.2, exposure 0.1, tonemap off, filter 1
+ * (SpecularFilterPass.h:48-51). */
+typedef struct vct_specular_filter_desc { /* SpecularFilterPass.h:31-37 (push constant) */
+    float tonemap_gamma;
+    float tonemap_exposure;
+    int32_t tonemap_enable;
+    int32_t filter_method;
+} vct_specular_filter_desc;
+vct_status vct_specular_filter(vct_stream stream, const float* diffuse, const float* specular, uint32_t width,
+                               uint32_t height, const vct_specular_filter_desc* desc, float* out);
+
 /* ---- input producers (no reference counterpart on the hot path) ----------
  * The reference rasterizes its G-buffer (GBufferPass.cpp:170-206) and light
  * depth map (ReflectiveShadowMapPass.cpp:38-88) before the GI passes run;

@@ -231,6 +231,10 @@ void orc_adj_render(orc_adj* a, const orc_gbuffer* gb, const orc_dense_params* P
 void orc_adj_backward(orc_adj* a, const orc_gbuffer* gb, const orc_dense_params* P, const double* grad_out,
                       double* grad_level0);
 
+/* ---- SpecularFilterPass (orc_filter.c) ---- */
+void orc_specular_filter(const float* diffuse, const float* specular, int W, int H, float gamma, float exposure,
+                         int tonemap_enable, int filter_method, float* out);
+
 #ifdef __cplusplus
 }
 #endif

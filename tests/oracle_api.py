@@ -127,6 +127,8 @@ def lib() -> C.CDLL:
         _L.orc_calc_min_level.restype = C.c_float
         _L.orc_svo_fragments.restype = C.c_uint32
         _L.orc_svo_build.restype = C.c_int
+        _L.orc_specular_filter.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
+                                           C.c_int, C.c_void_p]
         _L.orc_adj_create.restype = C.c_void_p
         _L.orc_adj_create.argtypes = [C.c_int, C.c_float, C.POINTER(C.c_float)]
         for fn in ("orc_adj_free", "orc_adj_mip"):
@@ -364,3 +366,12 @@ class Adjoint:
         gs = gbuf_struct(g)
         lib().orc_adj_backward(self.h, C.byref(gs), C.byref(params), go.ctypes.data, grad.ctypes.data)
         return grad
+
+
+def specular_filter(diffuse, specular, gamma=2.2, exposure=0.1, tonemap=0, method=1):
+    H, W = diffuse.shape[:2]
+    d = np.ascontiguousarray(diffuse, np.float32)
+    s = np.ascontiguousarray(specular, np.float32)
+    out = np.zeros((H, W, 4), np.float32)
+    lib().orc_specular_filter(d.ctypes.data, s.ctypes.data, W, H, gamma, exposure, tonemap, method, out.ctypes.data)
+    return out

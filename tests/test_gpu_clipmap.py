@@ -258,3 +258,81 @@ def test_trace_matches_oracle(fmt, mode, cones32, W, H):
         n_gpu, n_orc = int(samples.item()), int(n_ref.value)
         print(f"samples gpu {n_gpu} oracle {n_orc}")
         assert abs(n_gpu - n_orc) <= max(2, n_orc // 100000)
+
+
+def _moving_frames(ctx, ds, cams, incremental=True):
+    """VoxelizationPass over a camera path; returns the opacity volume and the
+    regions after each frame."""
+    import vk_voxel_cone_tracing_amd as V
+    from vk_voxel_cone_tracing_amd import passes
+
+    vp = passes.VoxelizationPass(ctx, ds, incremental=incremental)
+    out = []
+    for k, cam in enumerate(cams):
+        if k == 0:
+            ctx.update_clip_regions(cam)
+        vp.camera_pos = cam
+        vp.render(V.Frame())
+        out.append((ctx.read_volume(V.VOLUME_OPACITY), ctx.regions))
+    return out
+
+
+@pytest.mark.parametrize("samples", [1, 4])
+def test_incremental_revoxelization_equals_full(samples):
+    # Intent mode (no float(vec3) clamp): scrolling the toroidal clipmap and
+    # voxelizing only the slabs that scrolled in gives the same volume as
+    # voxelizing every level of the moved regions from scratch.
+    import vk_voxel_cone_tracing_amd as V
+
+    quirks = 0x3F & ~V._abi.QUIRK_CLAMP_X_RANGE
+    scene, ctx, ds = _setup_case(quirks, samples)
+    base = np.full(3, 0.5 * SCALE)
+    cams = [base, base + [0.61, -0.27, 0.93], base + [0.65, 0.9, 0.2], base + [-1.7, 0.95, -0.4], base + [30.0, 0.0, 0.0]]
+    inc = _moving_frames(ctx, ds, cams)
+    moved = 0
+    for k, (vol, regions) in enumerate(inc):
+        ref_ctx = V.Context(resolution=RES, levels=LEVELS, quirks=quirks, raster_samples=samples,
+                            downsample_region_size=2, extent_l0=EXTENT_L0)
+        ref_ctx.set_regions(regions)
+        from vk_voxel_cone_tracing_amd import passes
+
+        full = passes.VoxelizationPass(ref_ctx, ds)
+        full.render(V.Frame())
+        ref = ref_ctx.read_volume(V.VOLUME_OPACITY)
+        if k:
+            moved += int(regions != inc[k - 1][1])
+        print(f"frame {k}: regions {regions[0][0]} .. occupied {(ref != 0).sum()}")
+        np.testing.assert_array_equal(vol, ref, err_msg=f"frame {k}")
+    assert moved >= 3
+
+
+def test_incremental_revoxelization_parity_mode_matches_oracle():
+    # Parity mode keeps the reference's float(vec3) clamp, which the slab
+    # regions make visible; the GPU must still follow the oracle bit for bit.
+    import vk_voxel_cone_tracing_amd as V
+
+    scene, ctx, ds = _setup_case(0x3F)
+    pos, nrm, mat = scene.world_triangles()
+    base = np.full(3, 0.5 * SCALE)
+    ctx.update_clip_regions(base)
+    ctx.clear_volume(V.VOLUME_OPACITY)
+    L = O.lib()
+    ref = O.new_volume(RES, LEVELS)
+    tris = O.tris_struct(pos, nrm, mat)
+    for level in range(LEVELS):
+        ctx.voxelize_opacity(ds, ctx.voxelization_desc(level), level)
+        L.orc_voxelize_opacity(C.byref(O.vol_struct(ref, RES, LEVELS)), C.byref(tris),
+                               C.byref(_desc_to_oracle(ctx.voxelization_desc(level), ctx.regions[level])), 1, 0x3F)
+    revox = ctx.move_clip_regions(base + [0.61, -0.27, 0.93])
+    assert sum(len(s) for s in revox) >= 3
+    for level, slabs in enumerate(revox):
+        for region in slabs:
+            mn, ext, vs = region
+            ctx.clear_region(V.VOLUME_OPACITY, mn, ext, level)
+            L.orc_clear_region(C.byref(O.vol_struct(ref, RES, LEVELS)), (C.c_int * 3)(*mn), (C.c_int * 3)(*ext), level,
+                               6)
+            desc = ctx.voxelization_desc(level, region)
+            ctx.voxelize_opacity(ds, desc, level)
+            L.orc_voxelize_opacity(C.byref(O.vol_struct(ref, RES, LEVELS)), C.byref(tris),
+                                   C.byref(_desc_to_oracle(desc, region)), 1, 0x3F)
+    np.testing.assert_array_equal(ctx.read_volume(V.VOLUME_OPACITY), ref)

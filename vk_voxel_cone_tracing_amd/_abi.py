@@ -233,6 +233,15 @@ class DenseTraceDesc(C.Structure):
     ]
 
 
+class SpecularFilterDesc(C.Structure):
+    _fields_ = [
+        ("tonemap_gamma", C.c_float),
+        ("tonemap_exposure", C.c_float),
+        ("tonemap_enable", C.c_int32),
+        ("filter_method", C.c_int32),
+    ]
+
+
 class BatchScene(C.Structure):
     _fields_ = [
         ("mesh", Mesh),
@@ -271,6 +280,7 @@ STRUCT_SIZES = {
     "vct_gbuffer": (GBuffer, 48),
     "vct_batch_scene": (BatchScene, 336),
     "vct_batch_tuning": (BatchTuning, 16),
+    "vct_specular_filter_desc": (SpecularFilterDesc, 16),
 }
 
 P = C.c_void_p
@@ -348,6 +358,7 @@ _PROTOS = {
     "vct_batch_level": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
                                   C.POINTER(C.c_uint32)]),
     "vct_batch_read_level": (C.c_int, [P, P, C.c_uint32, C.c_uint32, P, C.c_size_t]),
+    "vct_specular_filter": (C.c_int, [P, P, P, C.c_uint32, C.c_uint32, C.POINTER(SpecularFilterDesc), P]),
     "vct_volume_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_size_t)]),
     "vct_read_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),
     "vct_write_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),
