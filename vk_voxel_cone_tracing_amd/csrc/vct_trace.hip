@@ -73,6 +73,7 @@ struct TraceParams {
     float inv_extent0;     // 1 / (voxel_size * dim): level-0 clip extent
     float inv_voxel_size;  // 1 / voxel_size
     float inv_min_radius;  // 1 / (voxel_size * dim / 2)
+    uint32_t vol_bytes;    // size of the radiance volume (buffer-load range)
 };
 
 template <bool kHalf>
@@ -96,7 +97,15 @@ struct Vol<true> {
     __device__ static __forceinline__ f4 load(const T* __restrict__ v, size_t i) { return unpack_half(v[i]); }
 };
 
+typedef unsigned int vct_u4 __attribute__((ext_vector_type(4)));
+typedef unsigned int vct_u2 __attribute__((ext_vector_type(2)));
+
 // Trilinear sample of the three selected faces of one level, weighted.
+// Texels are fetched with buffer loads: a 32-bit byte offset per lane (the
+// volume is far below 4 GiB) and the y / z / yz neighbour strides as scalar
+// offsets, so the 2x2x2 footprint of a face costs one offset computation and
+// four loads of a texel pair (x, x + 1) each. A fetch past the end of the
+// volume reads zero instead of faulting.
 template <bool kHalf>
 __device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P,
                                            const float pos[3], int level, const int face[3], const float w[3])
@@ -114,9 +123,12 @@ __device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restr
         i = i < 0 ? 0 : (i > P.res ? P.res : i);
         i0[k] = i;
     }
-    const size_t D = (size_t)P.D;
-    const size_t o000 = ((size_t)i0[2] * D + (size_t)i0[1]) * D + (size_t)i0[0];
-    const size_t oy = D, oz = D * D;
+    constexpr uint32_t TB = kHalf ? 8u : 4u;  // bytes per texel
+    const __amdgpu_buffer_rsrc_t R =
+        __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, (int)P.vol_bytes, 0x00020000);
+    const uint32_t D = (uint32_t)P.D;
+    const uint32_t o000 = ((uint32_t)i0[2] * D + (uint32_t)i0[1]) * D + (uint32_t)i0[0];
+    const int sy = (int)(D * TB), sz = (int)(D * D * TB), syz = sy + sz;
     const float ax = fr[0], ay = fr[1], az = fr[2];
     const float w000 = (1.0f - ax) * (1.0f - ay) * (1.0f - az), w100 = ax * (1.0f - ay) * (1.0f - az);
     const float w010 = (1.0f - ax) * ay * (1.0f - az), w110 = ax * ay * (1.0f - az);
@@ -125,12 +137,40 @@ __device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restr
     f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const size_t base = (((size_t)level * kFaces + (size_t)face[a]) * D) * D * D + o000;
-        const typename Vol<kHalf>::T* b = vol + base;
-        const f4 t000 = Vol<kHalf>::load(b, 0), t100 = Vol<kHalf>::load(b, 1);
-        const f4 t010 = Vol<kHalf>::load(b, oy), t110 = Vol<kHalf>::load(b, oy + 1);
-        const f4 t001 = Vol<kHalf>::load(b, oz), t101 = Vol<kHalf>::load(b, oz + 1);
-        const f4 t011 = Vol<kHalf>::load(b, oz + oy), t111 = Vol<kHalf>::load(b, oz + oy + 1);
+        const int voff = (int)(((((uint32_t)level * (uint32_t)kFaces + (uint32_t)face[a]) * D * D * D) + o000) * TB);
+        f4 t000, t100, t010, t110, t001, t101, t011, t111;
+        if (kHalf) {
+            const vct_u4 q0 = __builtin_amdgcn_raw_buffer_load_b128(R, voff, 0, 0);
+            const vct_u4 q1 = __builtin_amdgcn_raw_buffer_load_b128(R, voff, sy, 0);
+            const vct_u4 q2 = __builtin_amdgcn_raw_buffer_load_b128(R, voff, sz, 0);
+            const vct_u4 q3 = __builtin_amdgcn_raw_buffer_load_b128(R, voff, syz, 0);
+            t000 = unpack_half(make_uint2(q0.x, q0.y));
+            t100 = unpack_half(make_uint2(q0.z, q0.w));
+            t010 = unpack_half(make_uint2(q1.x, q1.y));
+            t110 = unpack_half(make_uint2(q1.z, q1.w));
+            t001 = unpack_half(make_uint2(q2.x, q2.y));
+            t101 = unpack_half(make_uint2(q2.z, q2.w));
+            t011 = unpack_half(make_uint2(q3.x, q3.y));
+            t111 = unpack_half(make_uint2(q3.z, q3.w));
+        } else {
+            const float sc = 1.0f / 255.0f;
+            auto u8 = [sc](uint32_t t) -> f4 {
+                return {(float)(t & 0xffu) * sc, (float)((t >> 8) & 0xffu) * sc, (float)((t >> 16) & 0xffu) * sc,
+                        (float)(t >> 24) * sc};
+            };
+            const vct_u2 q0 = __builtin_amdgcn_raw_buffer_load_b64(R, voff, 0, 0);
+            const vct_u2 q1 = __builtin_amdgcn_raw_buffer_load_b64(R, voff, sy, 0);
+            const vct_u2 q2 = __builtin_amdgcn_raw_buffer_load_b64(R, voff, sz, 0);
+            const vct_u2 q3 = __builtin_amdgcn_raw_buffer_load_b64(R, voff, syz, 0);
+            t000 = u8(q0.x);
+            t100 = u8(q0.y);
+            t010 = u8(q1.x);
+            t110 = u8(q1.y);
+            t001 = u8(q2.x);
+            t101 = u8(q2.y);
+            t011 = u8(q3.x);
+            t111 = u8(q3.y);
+        }
         const float wa = w[a];
 #define VCT_TRI(c)                                                                                        \
     acc.c += wa * (w000 * t000.c + w100 * t100.c + w010 * t010.c + w110 * t110.c + w001 * t001.c +       \
@@ -884,6 +924,11 @@ static vct_status make_trace_params(vct_ctx* c, const vct_gbuffer* g, const vct_
     P.inv_extent0 = 1.0f / (d->voxel_size * d->volume_dimension);
     P.inv_voxel_size = 1.0f / d->voxel_size;
     P.inv_min_radius = 1.0f / (d->voxel_size * d->volume_dimension * 0.5f);
+    {
+        const size_t vb = texels_per_volume(c) * texel_bytes(c, VCT_VOLUME_RADIANCE);
+        if (vb > 0xffffffffull) return fail(VCT_ERR_UNSUPPORTED, "radiance volume above 4 GiB");
+        P.vol_bytes = (uint32_t)vb;
+    }
     for (int k = 0; k < 16; ++k) P.vp_inv[k] = cam->view_proj_inv[k];
     for (int k = 0; k < 3; ++k) P.eye[k] = cam->eye_pos[k];
     P.width = (int)g->width;
