@@ -310,6 +310,15 @@ vct_status vct_downsample(vct_ctx* ctx, vct_stream stream, vct_volume volume,
                           const vct_downsample_desc* desc);
 
 /* K6: CopyAlpha::cmdImageCopyAlpha (CopyAlpha.h:26): radiance.a <- opacity.a. */
+/* K3 + K6 fused for an RGBA16F radiance level that is about to be injected
+ * (no reference counterpart as one call): writes (0, 0, 0, opacity alpha)
+ * to every interior texel of the level, and the next vct_inject_radiance of
+ * that level keeps the stored alpha. begin + inject leaves the level exactly
+ * as a full-level vct_clear_region, vct_inject_radiance and vct_copy_alpha do
+ * (RadianceInjectionPass.cpp:64-159) with one pass over the level instead of
+ * two. RGBA8 radiance keeps its count in alpha while injecting, so it is
+ * refused (VCT_ERR_UNSUPPORTED). */
+vct_status vct_begin_radiance_level(vct_ctx* ctx, vct_stream stream, uint32_t level);
 vct_status vct_copy_alpha(vct_ctx* ctx, vct_stream stream, const vct_copy_alpha_desc* desc);
 
 /* K7: BorderWrapper::cmdWrappingOpacityBorder / cmdWrappingRadianceBorder

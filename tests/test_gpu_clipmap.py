@@ -336,3 +336,31 @@ def test_incremental_revoxelization_parity_mode_matches_oracle():
             L.orc_voxelize_opacity(C.byref(O.vol_struct(ref, RES, LEVELS)), C.byref(tris),
                                    C.byref(_desc_to_oracle(desc, region)), 1, 0x3F)
     np.testing.assert_array_equal(ctx.read_volume(V.VOLUME_OPACITY), ref)
+
+
+def test_fused_radiance_begin_equals_clear_inject_copy_alpha():
+    # vct_begin_radiance_level + inject == full clear + inject + copy-alpha, bit for bit
+    import vk_voxel_cone_tracing_amd as V
+
+    scene, ctx, ds = _setup_case(0x3F, fmt=1)
+    pos, nrm, mat = scene.world_triangles()
+    light, depth, depth_d = _light_and_shadow(scene, pos, nrm, mat)
+    ld, ls = V.light_structs(light)
+    sm = V.shadow_struct(depth_d)
+    ctx.clear_volume(V.VOLUME_OPACITY)
+    for level in range(LEVELS):
+        ctx.voxelize_opacity(ds, ctx.voxelization_desc(level), level)
+    results = []
+    for fused in (False, True):
+        ctx.clear_volume(V.VOLUME_RADIANCE)
+        for level in range(LEVELS):
+            if fused:
+                ctx.begin_radiance_level(level)
+            else:
+                ctx.clear_region(V.VOLUME_RADIANCE, (0, 0, 0), (RES, RES, RES), level)
+            ctx.inject_radiance(ds, sm, ld, ls, ctx.voxelization_desc(level), level)
+            if not fused:
+                ctx.copy_alpha(level)
+        results.append(ctx.read_volume(V.VOLUME_RADIANCE))
+    assert (results[0].reshape(results[0].shape[:3] + (-1,)) != 0).any(-1).sum() > 1000
+    np.testing.assert_array_equal(results[0], results[1])

@@ -233,6 +233,9 @@ class Context:
         d = _abi.DownSampleDesc(_abi.i3(*prev_min_corner), level, self.resolution, region_size)
         _abi.call("vct_downsample", self._h, _stream_handle(stream), volume, C.byref(d))
 
+    def begin_radiance_level(self, level, stream=None):
+        _abi.call("vct_begin_radiance_level", self._h, _stream_handle(stream), level)
+
     def copy_alpha(self, level, face_count=6, stream=None):
         d = _abi.CopyAlphaDesc(level, self.resolution, face_count)
         _abi.call("vct_copy_alpha", self._h, _stream_handle(stream), C.byref(d))

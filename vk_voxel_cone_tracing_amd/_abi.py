@@ -305,6 +305,7 @@ _PROTOS = {
     "vct_clear_region": (C.c_int, [P, P, C.c_int, C.POINTER(ImageCleaningDesc)]),
     "vct_downsample": (C.c_int, [P, P, C.c_int, C.POINTER(DownSampleDesc)]),
     "vct_copy_alpha": (C.c_int, [P, P, C.POINTER(CopyAlphaDesc)]),
+    "vct_begin_radiance_level": (C.c_int, [P, P, C.c_uint32]),
     "vct_wrap_border": (C.c_int, [P, P, C.c_int, C.POINTER(BorderWrappingDesc)]),
     "vct_trace": (
         C.c_int,

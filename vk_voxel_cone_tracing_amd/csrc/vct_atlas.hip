@@ -77,6 +77,23 @@ __global__ void k_copy_alpha(typename Texel<kHalf>::T* __restrict__ rad, const u
     }
 }
 
+// K3 + K6 for a half-float radiance level that is about to be injected:
+// every interior texel becomes (0, 0, 0, opacity alpha), which is what the
+// full-level clear followed (after injection) by copy-alpha leaves on the
+// texels no fragment reaches; injected texels keep this alpha.
+__global__ void k_begin_radiance(uint2* __restrict__ rad, const uint32_t* __restrict__ opa, int D, int res,
+                                 int level)
+{
+    uint32_t gx = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t gy = blockIdx.y * blockDim.y + threadIdx.y;
+    uint32_t gz = blockIdx.z * blockDim.z + threadIdx.z;
+    if (gx >= (uint32_t)res || gy >= (uint32_t)res || gz >= (uint32_t)res) return;
+    for (int f = 0; f < kFaces; ++f) {
+        size_t i = vol_index(D, level, f, (int)gx + 1, (int)gy + 1, (int)gz + 1);
+        rad[i] = pack_half(f4{0.0f, 0.0f, 0.0f, u8_to_f(opa[i] >> 24)});
+    }
+}
+
 // ---------------------------------------------------------------- K7
 template <typename T>
 __global__ void k_wrap_border(T* __restrict__ vol, int D, int res, int n, int levels, int faces, int rwb)
@@ -232,6 +249,22 @@ vct_status vct_clear_region(vct_ctx* c, vct_stream stream, vct_volume v, const v
                            (uint32_t*)(v == VCT_VOLUME_OPACITY ? c->opacity.ptr : c->radiance.ptr), c->D, c->res,
                            d->clip_level, minc, ext, (int)d->face_count);
     VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status vct_begin_radiance_level(vct_ctx* c, vct_stream stream, uint32_t level)
+{
+    if (!c) return fail(VCT_ERR_INVALID_ARGUMENT, "null context");
+    if (level >= (uint32_t)c->levels) return fail(VCT_ERR_INVALID_ARGUMENT, "level out of range");
+    if (c->cfg.radiance_format != VCT_FORMAT_RGBA16F)
+        return fail(VCT_ERR_UNSUPPORTED, "the fused clear / copy-alpha is for RGBA16F radiance");
+    hipStream_t s = (hipStream_t)stream;
+    dim3 block(64, 2, 2);
+    dim3 grid((c->res + 63) / 64, (c->res + 1) / 2, (c->res + 1) / 2);
+    hipLaunchKernelGGL(k_begin_radiance, grid, block, 0, s, (uint2*)c->radiance.ptr, (const uint32_t*)c->opacity.ptr,
+                       c->D, c->res, (int)level);
+    VCT_HIP(hipGetLastError());
+    c->keep_alpha[level] = true;
     return VCT_OK;
 }
 

@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_raster_inject(const float* __restrict__
 template <bool kHalf>
 __global__ __launch_bounds__(256) void k_apply(const unsigned long long* __restrict__ keys,
                                                const uint32_t* __restrict__ idx, const Record* __restrict__ vals,
-                                               uint32_t n, void* __restrict__ vol)
+                                               uint32_t n, void* __restrict__ vol, bool keep_alpha)
 {
 #pragma clang fp contract(off)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -222,7 +222,8 @@ __global__ __launch_bounds__(256) void k_apply(const unsigned long long* __restr
             sb = sb + v.b;
             cnt = cnt + 1.0f;
         }
-        ((uint2*)vol)[addr] = pack_half(f4{sr / cnt, sg / cnt, sb / cnt, 1.0f});
+        const float a = keep_alpha ? unpack_half(((const uint2*)vol)[addr]).w : 1.0f;
+        ((uint2*)vol)[addr] = pack_half(f4{sr / cnt, sg / cnt, sb / cnt, a});
     }
 }
 
@@ -322,6 +323,9 @@ vct_status vct_inject_radiance(vct_ctx* c, vct_stream stream, const vct_mesh* me
         return fail(VCT_ERR_INVALID_ARGUMENT, "light, light_shadow and shadow map are required");
     VoxParams P;
     VCT_TRY(make_params(c, d, level, &P));
+    // after vct_begin_radiance_level the texels already carry the opacity alpha
+    const bool keep = c->keep_alpha[level];
+    c->keep_alpha[level] = false;
     hipStream_t s = (hipStream_t)stream;
     TriSoup soup;
     VCT_TRY(run_vertex_stage(c, s, mesh, &soup));
@@ -389,11 +393,11 @@ vct_status vct_inject_radiance(vct_ctx* c, vct_stream stream, const vct_mesh* me
     if (half)
         hipLaunchKernelGGL(k_apply<true>, dim3((nrec + 255) / 256), dim3(256), 0, s,
                            (const unsigned long long*)c->rec_key[1].ptr, (const uint32_t*)c->rec_idx[1].ptr,
-                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr);
+                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr, keep);
     else
         hipLaunchKernelGGL(k_apply<false>, dim3((nrec + 255) / 256), dim3(256), 0, s,
                            (const unsigned long long*)c->rec_key[1].ptr, (const uint32_t*)c->rec_idx[1].ptr,
-                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr);
+                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr, false);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }

@@ -10,7 +10,7 @@ the Vulkan command buffer; every GPU operation is one libvct entry point.
 from __future__ import annotations
 
 from . import _abi
-from . import Context, Frame, VOLUME_OPACITY, VOLUME_RADIANCE
+from . import FORMAT_RGBA16F, Context, Frame, VOLUME_OPACITY, VOLUME_RADIANCE
 
 
 class RenderPassBase:
@@ -125,15 +125,24 @@ class RadianceInjectionPass(RenderPassBase):
         self.frame_index = 0
         # the reference never wraps the radiance border (BorderWrapper.h:33-36 has no caller)
         self.wrap_radiance_border = wrap_radiance_border
+        # RGBA16F: vct_begin_radiance_level instead of clear + copy-alpha (same result)
+        self.fuse_alpha = True
 
     def _due(self, level):
         return self.frame_index % (1 << level) == 0
+
+    def _fused(self):
+        # half-float radiance: clear + copy-alpha become one pass before injection
+        return self.ctx.config.radiance_format == FORMAT_RGBA16F and self.fuse_alpha
 
     def on_begin_render_pass(self, frame):
         res = self.ctx.resolution
         for level in range(self.ctx.levels):
             if self._due(level):
-                self.ctx.clear_region(VOLUME_RADIANCE, (0, 0, 0), (res, res, res), level, 6, frame.stream)
+                if self._fused():
+                    self.ctx.begin_radiance_level(level, frame.stream)
+                else:
+                    self.ctx.clear_region(VOLUME_RADIANCE, (0, 0, 0), (res, res, res), level, 6, frame.stream)
 
     def on_update(self, frame):
         for level in range(self.ctx.levels):
@@ -144,7 +153,7 @@ class RadianceInjectionPass(RenderPassBase):
 
     def on_end_render_pass(self, frame):
         for level in range(self.ctx.levels):
-            if self._due(level):
+            if self._due(level) and not self._fused():
                 self.ctx.copy_alpha(level, 6, frame.stream)
         regions = self.ctx.regions
         for level in range(1, self.ctx.levels):
