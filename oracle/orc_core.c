@@ -177,9 +177,10 @@ void orc_clear_region(orc_volume* vol, const int min_corner[3], const int extent
     for (int gz = 0; gz < extent[2]; ++gz)
         for (int gy = 0; gy < extent[1]; ++gy)
             for (int gx = 0; gx < extent[0]; ++gx) {
-                int px = (gx + min_corner[0]) % res + 1;
-                int py = (gy + min_corner[1]) % res + 1 + clip_level * rwb;
-                int pz = (gz + min_corner[2]) % res + 1;
+                /* toroidal (non-negative) modulo; res is a power of two */
+                int px = ((gx + min_corner[0]) & (res - 1)) + 1;
+                int py = ((gy + min_corner[1]) & (res - 1)) + 1 + clip_level * rwb;
+                int pz = ((gz + min_corner[2]) & (res - 1)) + 1;
                 for (int f = 0; f < face_count; ++f) orc_store(vol, px + rwb * f, py, pz, zero);
             }
 }

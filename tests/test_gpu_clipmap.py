@@ -134,6 +134,9 @@ def test_clear_region_and_copy_alpha_bit_exact():
     # toroidal sub-region clear of level 1 (wraps around the far edge)
     ctx.clear_region(V.VOLUME_RADIANCE, (25, 3, 30), (12, 5, 7), 1)
     L.orc_clear_region(C.byref(O.vol_struct(rad, RES, LEVELS)), (C.c_int * 3)(25, 3, 30), (C.c_int * 3)(12, 5, 7), 1, 6)
+    # a negative corner (a slab below the old region minimum) wraps the same way
+    ctx.clear_region(V.VOLUME_RADIANCE, (-5, 7, -40), (9, 4, 11), 2)
+    L.orc_clear_region(C.byref(O.vol_struct(rad, RES, LEVELS)), (C.c_int * 3)(-5, 7, -40), (C.c_int * 3)(9, 4, 11), 2, 6)
     for level in range(LEVELS):
         ctx.copy_alpha(level)
         L.orc_copy_alpha(C.byref(O.vol_struct(rad, RES, LEVELS)), C.byref(O.vol_struct(opa, RES, LEVELS)), level, 6)

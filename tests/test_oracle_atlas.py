@@ -4,6 +4,7 @@ brute-force index checks and invariants."""
 import ctypes as C
 
 import numpy as np
+import pytest
 
 import oracle_api as O
 
@@ -162,11 +163,14 @@ def test_border_wrap_invariants():
                         assert b1[z, y, x] == b0[src[0], src[1], src[2]]
 
 
-def test_clear_region_toroidal():
+# negative corners come from camera moves toward -x/-y/-z: the slab below the
+# region's old minimum wraps to the top of the toroidal volume
+@pytest.mark.parametrize("mn", [(12, 0, 14), (-3, -16, 5), (-20, 2, -1)])
+def test_clear_region_toroidal(mn):
     rng = np.random.default_rng(4)
     vol = rng.integers(1, 2**32, size=(D, D * LEVELS, D * 6), dtype=np.uint64).astype(np.uint32)
     before = vol.copy()
-    mn, ext = (12, 0, 14), (7, 3, 5)
+    ext = (7, 3, 5)
     O.lib().orc_clear_region(C.byref(O.vol_struct(vol, RES, LEVELS)), (C.c_int * 3)(*mn), (C.c_int * 3)(*ext), 1, 6)
     cleared = np.zeros((D, D, D), bool)
     for z in range(ext[2]):

@@ -40,10 +40,13 @@ __global__ void k_clear_region(typename Texel<kHalf>::T* __restrict__ vol, int D
     uint32_t gy = blockIdx.y * blockDim.y + threadIdx.y;
     uint32_t gz = blockIdx.z * blockDim.z + threadIdx.z;
     if (gx >= ext.x || gy >= ext.y || gz >= ext.z) return;
-    // clipmapCleaning.comp:22-30
-    int x = ((int)gx + minc.x) % res + 1;
-    int y = ((int)gy + minc.y) % res + 1;
-    int z = ((int)gz + minc.z) % res + 1;
+    // clipmapCleaning.comp:22-30 with a non-negative (toroidal) modulo: res is a
+    // power of two, so & (res - 1) wraps negative region corners too (GLSL %
+    // leaves negative operands undefined; the voxelizer's fract() mapping wraps
+    // them this way)
+    int x = (((int)gx + minc.x) & (res - 1)) + 1;
+    int y = (((int)gy + minc.y) & (res - 1)) + 1;
+    int z = (((int)gz + minc.z) & (res - 1)) + 1;
     for (int f = 0; f < faces; ++f) {
         size_t i = vol_index(D, level, f, x, y, z);
         if (kHalf) ((uint2*)vol)[i] = make_uint2(0u, 0u);
@@ -232,7 +235,8 @@ vct_status vct_clear_region(vct_ctx* c, vct_stream stream, vct_volume v, const v
     if (d->clipmap_resolution != c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "clipmap_resolution");
     if (d->face_count > (uint32_t)kFaces) return fail(VCT_ERR_INVALID_ARGUMENT, "face_count");
     for (int k = 0; k < 3; ++k) {
-        if (d->region_min_corner[k] < 0) return fail(VCT_ERR_INVALID_ARGUMENT, "region_min_corner must be >= 0");
+        if (d->region_min_corner[k] < -(1 << 28) || d->region_min_corner[k] > (1 << 28))
+            return fail(VCT_ERR_INVALID_ARGUMENT, "region_min_corner out of range");
         if (d->clip_max_extent[k] > (uint32_t)c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "extent > resolution");
     }
     if (!d->clip_max_extent[0] || !d->clip_max_extent[1] || !d->clip_max_extent[2] || !d->face_count) return VCT_OK;
