@@ -56,7 +56,7 @@ typedef enum vct_format {
 /* Reference behaviours that are defects or driver-defined. Each bit selects
  * the reference's behaviour; a clear bit selects the evident intent. */
 enum {
-    VCT_QUIRK_CLAMP_X_RANGE = 1u << 0,      /* msaaVoxelizer.frag:46 float(vec3) clamp */
+    VCT_QUIRK_CLAMP_X_RANGE = 1u << 0,      /* msaaVoxelizer.frag:43-60 closed region, float(vec3) clamp; off: half-open voxel cells */
     VCT_QUIRK_BORDER_SKIP_FAR = 1u << 1,    /* BorderWrapper.cpp:139-140 writes 0..127 only */
     VCT_QUIRK_SHADOW_RAW_DEPTH = 1u << 2,   /* shadow.glsl:8-36 averages raw depth */
     VCT_QUIRK_SVO_HALF_COORDS = 1u << 3,    /* octreeNodeFlag.comp:28-32 fragPos >> 1 */
@@ -274,8 +274,12 @@ vct_status vct_update_clip_regions(vct_ctx* ctx, const float camera_pos[3]);
  * every level's region in ctx and writes the regions that must be cleared and
  * voxelized again: up to 3 slabs per level (one per axis that moved by at
  * least the level's minimum change) or the whole region when a move reaches
- * the extent. revox[level * 3 + i], i < counts[level]; both arrays hold
- * `levels` entries (revox: 3 per level). */
+ * the extent. For a level >= 1 whose downsampled inner box (the cells under
+ * the level below) moved, one more region: the old inner box within the new
+ * region, whose cells still hold the previous mips (the reference omits it).
+ * revox[level * VCT_REVOX_PER_LEVEL + i], i < counts[level]; revox holds
+ * levels * VCT_REVOX_PER_LEVEL entries, counts `levels`. */
+#define VCT_REVOX_PER_LEVEL 4
 vct_status vct_move_clip_regions(vct_ctx* ctx, const float camera_pos[3], vct_clipmap_region* revox,
                                  uint32_t* counts);
 /* Host half of Voxelizer::cmdVoxelize (Voxelizer.cpp:225-262). */

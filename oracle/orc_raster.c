@@ -110,24 +110,52 @@ void orc_raster_triangle(const float* p, float vs, const int lo[3], const int hi
             }
 }
 
-/* msaaVoxelizer.frag:43-55 / msaaInjectRadiance.frag:162-174 calculateImageCoords */
+static int voxel_cell(const orc_vox_desc* d, float p) { return (int)floorf(p / d->voxel_size); }
+
+/* msaaVoxelizer.frag:43-55 / msaaInjectRadiance.frag:162-174 calculateImageCoords.
+ * Parity (ORC_QUIRK_CLAMP_X_RANGE): clamp(worldPos, float(uRegionMinCorner + c),
+ * float(uRegionMaxCorner - c)) - float(vec3) is the x component, so every axis
+ * uses the x range - then fract(p / extent) * res. Intent: half-open voxel
+ * cells, texel = floor(p / voxel_size) mod res with the cell clamped into the
+ * region's cells; the texel of a point then does not depend on the region. */
 static void image_coords(const orc_vox_desc* d, const float pos[3], uint32_t quirks, int ic[3])
 {
     const int res = d->clipmap_resolution;
-    float c = d->voxel_size * 0.25f;
-    for (int k = 0; k < 3; ++k) {
-        /* clamp(worldPos, float(uRegionMinCorner + c), float(uRegionMaxCorner - c)):
-         * float(vec3) is the x component, so every axis uses the x range. */
-        int kr = (quirks & ORC_QUIRK_CLAMP_X_RANGE) ? 0 : k;
-        float lo = d->region_min[kr] + c;
-        float hi = d->region_max[kr] - c;
-        float w = fminf(fmaxf(pos[k], lo), hi);
-        float q = w / d->clip_max_extent;
-        float fr = q - floorf(q);
-        int iv = ((int)(fr * (float)res)) % res;
-        ic[k] = iv + 1;
+    if (quirks & ORC_QUIRK_CLAMP_X_RANGE) {
+        float c = d->voxel_size * 0.25f;
+        for (int k = 0; k < 3; ++k) {
+            float lo = d->region_min[0] + c;
+            float hi = d->region_max[0] - c;
+            float w = fminf(fmaxf(pos[k], lo), hi);
+            float q = w / d->clip_max_extent;
+            float fr = q - floorf(q);
+            int iv = ((int)(fr * (float)res)) % res;
+            ic[k] = iv + 1;
+        }
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            const int lo = d->region_cell_min[k], hi = d->region_cell_min[k] + d->region_cell_extent[k] - 1;
+            int g = voxel_cell(d, pos[k]);
+            g = g < lo ? lo : (g > hi ? hi : g);
+            ic[k] = ((g % res) + res) % res + 1;
+        }
     }
     ic[1] += (res + 2) * d->clip_level;
+}
+
+/* msaaVoxelizer.frag:59-60: the reference keeps fragments inside the closed
+ * region; intent keeps the region's half-open cells. */
+static int in_region(const orc_vox_desc* d, const float pos[3], uint32_t quirks)
+{
+    for (int k = 0; k < 3; ++k) {
+        if (quirks & ORC_QUIRK_CLAMP_X_RANGE) {
+            if (pos[k] < d->region_min[k] || pos[k] > d->region_max[k]) return 0;
+        } else {
+            const int g = voxel_cell(d, pos[k]);
+            if (g < d->region_cell_min[k] || g >= d->region_cell_min[k] + d->region_cell_extent[k]) return 0;
+        }
+    }
+    return 1;
 }
 
 static void extended_cells(const orc_vox_desc* d, int lo[3], int hi[3])
@@ -152,9 +180,7 @@ static void opacity_frag(void* user, uint32_t tri, const float pos[3], const dou
     (void)lam;
     opa_ctx* c = (opa_ctx*)user;
     const orc_vox_desc* d = c->d;
-    /* msaaVoxelizer.frag:59-60 */
-    for (int k = 0; k < 3; ++k)
-        if (pos[k] < d->region_min[k] || pos[k] > d->region_max[k]) return;
+    if (!in_region(d, pos, c->quirks)) return;
     int ic[3];
     image_coords(d, pos, c->quirks, ic);
     const int rwb = d->clipmap_resolution + 2;

@@ -67,3 +67,12 @@ def test_ctypes_mirror_covers_every_declared_function():
 
     missing = [n for n in _declared_functions() if n not in _abi._PROTOS]
     assert not missing, missing
+
+
+def test_ctypes_mirror_constants_match_header():
+    from vk_voxel_cone_tracing_amd import _abi
+
+    text = (ROOT / "include" / "vct.h").read_text()
+    assert int(re.search(r"#define VCT_REVOX_PER_LEVEL (\d+)", text).group(1)) == _abi.REVOX_PER_LEVEL
+    assert re.search(r"VCT_ADJOINT_RECOMPUTE = 1u << 0", text) and _abi.ADJOINT_RECOMPUTE == 1
+    assert re.search(r"VCT_QUIRK_CLAMP_X_RANGE = 1u << 0", text) and _abi.QUIRK_CLAMP_X_RANGE == 1

@@ -199,10 +199,11 @@ class Context:
         """vct_move_clip_regions: moves the regions with the camera and returns,
         per level, the regions to clear and voxelize again."""
         cp = (C.c_float * 3)(*[float(x) for x in camera_pos])
-        revox = (_abi.ClipmapRegion * (3 * self.levels))()
+        n = _abi.REVOX_PER_LEVEL
+        revox = (_abi.ClipmapRegion * (n * self.levels))()
         counts = (C.c_uint32 * self.levels)()
         _abi.call("vct_move_clip_regions", self._h, cp, revox, counts)
-        return [[(tuple(r.min_corner), tuple(r.extent), r.voxel_size) for r in revox[3 * i: 3 * i + counts[i]]]
+        return [[(tuple(r.min_corner), tuple(r.extent), r.voxel_size) for r in revox[n * i: n * i + counts[i]]]
                 for i in range(self.levels)]
 
     def voxelization_desc(self, level, region=None):

@@ -37,6 +37,7 @@ QUIRK_SVO_MIP_LEVEL_TEST = 1 << 4
 QUIRK_SVO_UNTEXTURED_BLACK = 1 << 5
 QUIRK_ALL = 0x3F
 ADJOINT_RECOMPUTE = 1
+REVOX_PER_LEVEL = 4  # VCT_REVOX_PER_LEVEL: regions per level written by vct_move_clip_regions
 
 MODE_DIFFUSE = 0
 MODE_SPECULAR = 1

@@ -361,6 +361,9 @@ vct_status vct_move_clip_regions(vct_ctx* c, const float cam[3], vct_clipmap_reg
     if (!c || !cam || !revox || !counts) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
     if (!c->regions_set) return fail(VCT_ERR_NOT_READY, "the regions have not been set");
     static const int32_t kMinChange[8] = {2, 2, 2, 2, 2, 1, 1, 1};  // VoxelizationPass.h:57
+    int32_t old_min[kMaxLevels][3];
+    for (int i = 0; i < c->levels; ++i)
+        for (int k = 0; k < 3; ++k) old_min[i][k] = c->regions[i].min_corner[k];
     for (int i = 0; i < c->levels; ++i) {
         vct_clipmap_region& reg = c->regions[i];
         const float vs = reg.voxel_size;
@@ -375,9 +378,10 @@ vct_status vct_move_clip_regions(vct_ctx* c, const float cam[3], vct_clipmap_reg
             reg.min_corner[k] += delta[k];
             if ((uint32_t)std::abs(delta[k]) >= reg.extent[k]) full = true;
         }
+        vct_clipmap_region* out = revox + VCT_REVOX_PER_LEVEL * i;
         uint32_t n = 0;
         if (full) {
-            revox[3 * i + n++] = reg;
+            out[n++] = reg;
         } else {
             for (int k = 0; k < 3; ++k) {
                 const int32_t ad = std::abs(delta[k]);
@@ -385,7 +389,25 @@ vct_status vct_move_clip_regions(vct_ctx* c, const float cam[3], vct_clipmap_reg
                 vct_clipmap_region slab = reg;
                 slab.extent[k] = (uint32_t)ad;
                 slab.min_corner[k] = delta[k] > 0 ? reg.min_corner[k] + (int32_t)reg.extent[k] - ad : reg.min_corner[k];
-                revox[3 * i + n++] = slab;
+                out[n++] = slab;
+            }
+            // The downsample of level i overwrites the mip channels of the cells
+            // under level i-1 ((prev_min >> 1) + [0, res/2), opacityDownSample.comp);
+            // when level i-1 moved, the cells it left still hold the old mips.
+            // Revoxelize the old inner box (intersected with the new region).
+            if (i > 0) {
+                const vct_clipmap_region& prev = c->regions[i - 1];
+                bool moved = false;
+                vct_clipmap_region box = reg;
+                for (int k = 0; k < 3; ++k) {
+                    moved |= (prev.min_corner[k] >> 1) != (old_min[i - 1][k] >> 1);
+                    const int32_t lo = std::max(old_min[i - 1][k] >> 1, reg.min_corner[k]);
+                    const int32_t hi = std::min((old_min[i - 1][k] >> 1) + (int32_t)(prev.extent[k] >> 1),
+                                                reg.min_corner[k] + (int32_t)reg.extent[k]);
+                    box.min_corner[k] = lo;
+                    box.extent[k] = hi > lo ? (uint32_t)(hi - lo) : 0u;
+                }
+                if (moved && box.extent[0] && box.extent[1] && box.extent[2]) out[n++] = box;
             }
         }
         counts[i] = n;

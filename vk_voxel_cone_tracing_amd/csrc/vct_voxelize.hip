@@ -36,20 +36,52 @@ struct VoxParams {
     uint32_t quirks;
 };
 
-// msaaVoxelizer.frag:43-55 / msaaInjectRadiance.frag:162-174
+// Voxel cell of a coordinate on one axis: floor(p / voxel_size).
+__device__ __forceinline__ int voxel_cell(const VoxParams& P, float p)
+{
+    return (int)floorf(p / P.voxel_size);
+}
+
+// msaaVoxelizer.frag:43-55 / msaaInjectRadiance.frag:162-174. With
+// VCT_QUIRK_CLAMP_X_RANGE, the reference's calculateImageCoords: clamp into the
+// closed region (with the x range on every axis), then fract(p / extent).
+// Without it, half-open voxel cells: texel = cell mod res with the cell clamped
+// into the region's cells, so a point's texel does not depend on where the
+// region sits and a sub-region voxelization writes exactly the texels a full
+// one writes inside it.
 __device__ __forceinline__ void image_coords(const VoxParams& P, const float pos[3], int ic[3])
 {
 #pragma clang fp contract(off)
-    const float c = P.voxel_size * 0.25f;
-    for (int k = 0; k < 3; ++k) {
-        const int kr = (P.quirks & VCT_QUIRK_CLAMP_X_RANGE) ? 0 : k;
-        const float lo = P.region_min[kr] + c;
-        const float hi = P.region_max[kr] - c;
-        const float w = fminf(fmaxf(pos[k], lo), hi);
-        const float q = w / P.clip_max_extent;
-        const float fr = q - floorf(q);
-        ic[k] = ((int)(fr * (float)P.res)) % P.res + 1;
+    if (P.quirks & VCT_QUIRK_CLAMP_X_RANGE) {
+        const float c = P.voxel_size * 0.25f;
+        for (int k = 0; k < 3; ++k) {
+            const float lo = P.region_min[0] + c;
+            const float hi = P.region_max[0] - c;
+            const float w = fminf(fmaxf(pos[k], lo), hi);
+            const float q = w / P.clip_max_extent;
+            const float fr = q - floorf(q);
+            ic[k] = ((int)(fr * (float)P.res)) % P.res + 1;
+        }
+        return;
     }
+    for (int k = 0; k < 3; ++k) {
+        const int g = min(max(voxel_cell(P, pos[k]), P.lo[k] + 1), P.hi[k] - 1);
+        ic[k] = (g & (P.res - 1)) + 1;
+    }
+}
+
+// msaaVoxelizer.frag:59-60: the reference keeps fragments in the closed region;
+// without VCT_QUIRK_CLAMP_X_RANGE the region is its half-open cells.
+__device__ __forceinline__ bool in_region(const VoxParams& P, const float fp[3])
+{
+    if (P.quirks & VCT_QUIRK_CLAMP_X_RANGE)
+        return !(fp[0] < P.region_min[0] || fp[1] < P.region_min[1] || fp[2] < P.region_min[2] ||
+                 fp[0] > P.region_max[0] || fp[1] > P.region_max[1] || fp[2] > P.region_max[2]);
+    for (int k = 0; k < 3; ++k) {
+        const int g = voxel_cell(P, fp[k]);
+        if (g <= P.lo[k] || g >= P.hi[k]) return false;
+    }
+    return true;
 }
 
 __global__ void k_count(const float* __restrict__ pos, uint32_t ntris, VoxParams P, unsigned long long* __restrict__ counts)
@@ -100,10 +132,7 @@ __global__ __launch_bounds__(256) void k_raster_opacity(const float* __restrict_
         float fp[3];
         double lam[3];
         if (!tri_sample(p, s, P.voxel_size, P.samples, (uint32_t)(wi - offsets[t]), fp, lam)) continue;
-        // msaaVoxelizer.frag:59-60
-        if (fp[0] < P.region_min[0] || fp[1] < P.region_min[1] || fp[2] < P.region_min[2] ||
-            fp[0] > P.region_max[0] || fp[1] > P.region_max[1] || fp[2] > P.region_max[2])
-            continue;
+        if (!in_region(P, fp)) continue;
         int ic[3];
         image_coords(P, fp, ic);
         // imageStore(vec4(1.0)) into the six faces (msaaVoxelizer.frag:69-73)
