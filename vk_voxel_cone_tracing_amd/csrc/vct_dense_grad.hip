@@ -3,18 +3,8 @@
 // dense grid, through the cone march (vct_dense_trace) and the anisotropic
 // mips (vct_dense_mipmap). No reference counterpart.
 //
-// Trace: per pixel and active cone the march is replayed forward keeping the
-// per-step state (step length, segment, level, accumulated alpha, occlusion;
-// with kStore also the fetched sample), then swept in reverse:
-//   op = 1 - (1 - s.a)^c,  b = 1 - R.a,  q = 1 + (t + vs_l) decay
-//   R.rgb += b c s.rgb,  R.a += b op,  O += (1 - O) op / q
-// gives  ds.rgb = gR.rgb b c,  d op = gR.a b + gO (1 - O) / q,
-//        ds.a = d op c (1 - s.a)^(c-1),
-//        gR.a <- gR.a (1 - op) - c gR.rgb . s.rgb,  gO <- gO (1 - op / q).
-// The sample gradient is spread over the texels of the sample's footprints
-// (trilinear x face weight x level lerp) with hardware float atomics. Without
-// kStore the samples are fetched again in the reverse sweep instead of kept,
-// which halves the per-lane state.
+// Trace: see k_dense_trace_bwd (a reverse sweep per cone, gradients
+// gathered per step in LDS windows before they reach global memory).
 //
 // Mips: each child texel gathers from its parent (no atomics):
 //   parent = mean over 4 pairs of a + (1 - a.a) b
@@ -57,139 +47,526 @@ __global__ void k_level0_grad(const float4* __restrict__ grad0, int res, float* 
     g_opacity[v] = g.w;
 }
 
-// Adds scale * g spread over level m's footprint at p (the transpose of
-// sample()).
-__device__ __forceinline__ void scatter_level(float* __restrict__ grad, const Pyramid& P, int m, const float p[3],
-                                              const int face[3], const float w[3], float scale, const f4& g)
-{
-    const Footprint F = footprint(P, m, p);
-    if (!F.valid || scale == 0.0f) return;
-    const float ax = F.fx, ay = F.fy, az = F.fz;
-    const float cw[8] = {(1.0f - ax) * (1.0f - ay) * (1.0f - az), ax * (1.0f - ay) * (1.0f - az),
-                         (1.0f - ax) * ay * (1.0f - az),          ax * ay * (1.0f - az),
-                         (1.0f - ax) * (1.0f - ay) * az,          ax * (1.0f - ay) * az,
-                         (1.0f - ax) * ay * az,                   ax * ay * az};
-    const unsigned long long off[8] = {0, 1, F.dy, F.dy + 1, F.dz, F.dz + 1, F.dz + F.dy, F.dz + F.dy + 1};
-    const int nf = m == 0 ? 1 : 3;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (a >= nf) break;
-        const float fw = scale * (m == 0 ? (w[0] + w[1] + w[2]) : w[a]);
-        const unsigned long long fb = F.base + (m == 0 ? 0ull : (unsigned long long)face[a] * P.block[m]);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const float k = fw * cw[c];
-            float* t = grad + 4ull * (fb + off[c]);
-            unsafeAtomicAdd(t + 0, k * g.x);
-            unsafeAtomicAdd(t + 1, k * g.y);
-            unsafeAtomicAdd(t + 2, k * g.z);
-            unsafeAtomicAdd(t + 3, k * g.w);
-        }
-    }
-}
-
-struct StepState {
-    float step, seg, lvl, rw, occ;
+// Bordered integer corner and fractions of level m's trilinear footprint at p:
+// the arithmetic of footprint().
+struct Corner {
+    int i[3];
+    float f[3];
+    bool valid;
 };
 
-// One cone: replay forward, then the reverse sweep. Returns false if the
-// march is longer than the state buffer.
-template <bool kStore>
-__device__ bool cone_backward(const void* __restrict__ data, float* __restrict__ grad, const Pyramid& P,
-                              const March& M, const float start_in[3], const float dir[3], const f4& gC)
+__device__ __forceinline__ Corner corner_of(const Pyramid& P, int m, const float p[3])
 {
-    const float vs = P.vs;
-    float start[3];
+    Corner c;
+    c.valid = true;
+    const int n = P.n[m];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) start[k] = start_in[k] + dir[k] * vs * M.start_offset * 0.5f;
-    const int face[3] = {dir[0] > 0.0f ? 0 : 1, dir[1] > 0.0f ? 2 : 3, dir[2] > 0.0f ? 4 : 5};
-    const float w[3] = {dir[0] * dir[0], dir[1] * dir[1], dir[2] * dir[2]};
-    const float max_level = (float)(P.nlevels - 1);
-    const float inv_vs = 1.0f / vs;
-    StepState st[kAdjMaxSteps];
-    f4 smp[kStore ? kAdjMaxSteps : 1];
-    float step = 0.0f, diameter = vs, occlusion = 0.0f, seg = vs;
-    f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
-    int n = 0;
-    while (step < M.max_dist && occlusion < 1.0f) {
-        if (n == kAdjMaxSteps) return false;
-        float pos[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) pos[k] = start[k] + dir[k] * step;
-        const float lvl = fminf(fmaxf(log2f(diameter * inv_vs), 0.0f), max_level);
-        const f4 s = level_sample<true>(data, P, pos, lvl, face, w);
-        st[n] = {step, seg, lvl, result.w, occlusion};
-        if (kStore) smp[n] = s;
-        ++n;
-        accumulate(result, occlusion, s, lvl, step, seg, vs, M.decay);
-        const float prev = step;
-        step += fmaxf(diameter, vs) * M.step_factor;
-        seg = step - prev;
-        diameter = step * M.coef;
+    for (int k = 0; k < 3; ++k) {
+        const float u = p[k] * P.inv_vs[m] - 0.5f;
+        const float f = floorf(u);
+        c.f[k] = u - f;
+        if (!(f >= -1.0f) || f > (float)(n - 1)) c.valid = false;
+        c.i[k] = (int)fminf(fmaxf(f, -1.0f), (float)(n - 1)) + 1;
     }
-    f4 gR = {gC.x, gC.y, gC.z, 0.0f};
-    float gO = -gC.w;
-    for (int k = n - 1; k >= 0; --k) {
-        const StepState S = st[k];
-        float pos[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) pos[i] = start[i] + dir[i] * S.step;
-        const f4 s = kStore ? smp[kStore ? k : 0] : level_sample<true>(data, P, pos, S.lvl, face, w);
-        const float vsl = vs * exp2f(S.lvl);
-        const float c = S.seg / vsl;
-        const float q = 1.0f + (S.step + vsl) * M.decay;
-        const float base = 1.0f - s.w;
-        const float op = 1.0f - powf(base, c);
-        const float b = 1.0f - S.rw;
-        const float g_op = gR.w * b + gO * (1.0f - S.occ) / q;
-        f4 gs;
-        gs.x = gR.x * b * c;
-        gs.y = gR.y * b * c;
-        gs.z = gR.z * b * c;
-        gs.w = base > 0.0f ? g_op * c * powf(base, c - 1.0f) : 0.0f;
-        const float dot = gR.x * s.x + gR.y * s.y + gR.z * s.z;
-        gR.w = gR.w * (1.0f - op) - c * dot;
-        gO = gO * (1.0f - op / q);
-        const int lo = (int)floorf(S.lvl), hi = (int)ceilf(S.lvl);
-        const float fl = S.lvl - floorf(S.lvl);
-        if (hi != lo) {
-            scatter_level(grad, P, lo, pos, face, w, 1.0f - fl, gs);
-            scatter_level(grad, P, hi, pos, face, w, fl, gs);
-        } else {
-            scatter_level(grad, P, lo, pos, face, w, 1.0f, gs);
-        }
-    }
-    return true;
+    return c;
 }
 
-template <bool kStore>
-__global__ __launch_bounds__(256) void k_dense_trace_bwd(const void* __restrict__ data, Pyramid P, March M,
-                                                         const uint32_t* __restrict__ g_diffuse,
-                                                         const uint16_t* __restrict__ g_normal,
-                                                         const float* __restrict__ g_depth, int width, int height,
-                                                         const float4* __restrict__ grad_out,
-                                                         float* __restrict__ grad, uint32_t* __restrict__ status)
+// LDS accumulation is in double: on gfx950 ds_add_f64 runs about as fast as
+// the integer LDS atomics while ds_add_f32 is more than twenty times slower.
+// A window that fits only at float width accumulates in float, still far
+// cheaper than scattered global atomics.
+constexpr int kBwdThreads = 256;  // one 16 x 16 pixel tile, four waves of 8 x 8
+constexpr int kPoolUnits = 1024;  // a wave's LDS pool for its two live level windows, in 16 B units (16 KB)
+constexpr int kMaxCopies = 16;    // lane-striped copies of a small window
+constexpr int kCoarseN = 4;       // levels with at most 4^3 interior cells stay in LDS per tile
+constexpr int kCoarseFloats = 24 * (64 + 8 + 1);
+
+__device__ __forceinline__ float wave_max(float v)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// The interior cells of level m that a set of start points can reach while
+// the level is in use: their box swept along the cone over the level's steps
+// [ta, tb], one cell wider on each side for rounding, clipped to the
+// interior (border texels are constant zero and take no gradient). An empty
+// set gives vol 0.
+struct Box {
+    int lo[3], d[3];
+    int vol, base;
+};
+
+__device__ __forceinline__ Box box_of(const Pyramid& P, int m, const float blo[3], const float bhi[3],
+                                      const float dir[3], float ta, float tb)
+{
+    Box B;
+    B.vol = 1;
+    B.base = 0;
+    const int n = P.n[m];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float lo_w = fminf(blo[k] + dir[k] * ta, blo[k] + dir[k] * tb);
+        const float hi_w = fmaxf(bhi[k] + dir[k] * ta, bhi[k] + dir[k] * tb);
+        const float a = fminf(fmaxf(lo_w * P.inv_vs[m] - 0.5f, -4.0f), (float)n + 4.0f);
+        const float b = fminf(fmaxf(hi_w * P.inv_vs[m] - 0.5f, -4.0f), (float)n + 4.0f);
+        const int lo = max((int)floorf(a), 1);
+        const int hi = min((int)floorf(b) + 3, n);
+        B.lo[k] = lo;
+        B.d[k] = hi >= lo ? hi - lo + 1 : 0;
+        B.vol *= B.d[k];
+    }
+    return B;
+}
+
+// A wave's LDS window for one level: one box, or two when the wave's start
+// points split across the middle of their longest axis (a depth edge) and the
+// two boxes fit where the one does not. Layout in LDS: per part, `copies`
+// blocks of `vol` texels x 4 channels from the part's base (in elements:
+// doubles, or floats when f32); lds is false when nothing fits in `cap`
+// 16-byte units; `used` is the units it takes.
+struct Window {
+    Box part[2];
+    int parts, axis, copies, used;
+    float mid;
+    bool lds, f32;
+};
+
+__device__ __forceinline__ Window window_of(const Pyramid& P, int m, const float start[3], bool lane_in,
+                                            const float blo[3], const float bhi[3], const float dir[3], float ta,
+                                            float tb, int base, int cap)
+{
+    Window W;
+    W.parts = 1;
+    W.axis = 0;
+    W.mid = 0.0f;
+    W.part[0] = box_of(P, m, blo, bhi, dir, ta, tb);
+    W.part[1].vol = 0;
+    if (2 * W.part[0].vol > cap) {
+        int ax = 0;
+        if (bhi[1] - blo[1] > bhi[ax] - blo[ax]) ax = 1;
+        if (bhi[2] - blo[2] > bhi[ax] - blo[ax]) ax = 2;
+        const float mid = 0.5f * (blo[ax] + bhi[ax]);
+        const bool up = lane_in && start[ax] >= mid, down = lane_in && start[ax] < mid;
+        float lo0[3], hi0[3], lo1[3], hi1[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo0[k] = wave_min(down ? start[k] : INFINITY);
+            hi0[k] = wave_max(down ? start[k] : -INFINITY);
+            lo1[k] = wave_min(up ? start[k] : INFINITY);
+            hi1[k] = wave_max(up ? start[k] : -INFINITY);
+        }
+        const Box b0 = box_of(P, m, lo0, hi0, dir, ta, tb), b1 = box_of(P, m, lo1, hi1, dir, ta, tb);
+        if (b0.vol + b1.vol < W.part[0].vol) {
+            W.parts = 2;
+            W.axis = ax;
+            W.mid = mid;
+            W.part[0] = b0;
+            W.part[1] = b1;
+        }
+    }
+    const int total = W.part[0].vol + W.part[1].vol;
+    W.f32 = 2 * total > cap;
+    W.lds = total > 0 && total <= cap;
+    int copies = kMaxCopies;
+    while (copies > 1 && (W.f32 ? 1 : 2) * copies * total > cap) copies >>= 1;
+    W.copies = copies;
+    W.used = W.lds ? (W.f32 ? 1 : 2) * copies * total : 0;
+    W.part[0].base = base;
+    W.part[1].base = base + 4 * copies * W.part[0].vol;
+    return W;
+}
+
+// Per-tile accumulators of the coarsest levels (n <= kCoarseN): 6 faces x 4
+// channel planes of n^3 interior cells per level, from `coarse_base[m]`.
+struct Coarse {
+    int first;     // first level held (nlevels when none)
+    int base[kMaxLevels];
+};
+
+__device__ __forceinline__ Coarse coarse_layout(const Pyramid& P)
+{
+    Coarse C;
+    C.first = P.nlevels;
+    int off = 0;
+    for (int m = 0; m < P.nlevels; ++m) {
+        C.base[m] = off;
+        if (m >= 1 && P.n[m] <= kCoarseN) {
+            if (C.first == P.nlevels) C.first = m;
+            off += 24 * P.n[m] * P.n[m] * P.n[m];
+        }
+    }
+    return C;
+}
+
+// Adds (face weights) x v to channel ch of interior texel (x, y, z) of level
+// m: the tile's LDS accumulator for a coarse level, else global memory; one
+// face at level 0 (weighted by w0 + w1 + w2, as sample() does), three above.
+__device__ __forceinline__ void add_channel(float* __restrict__ grad, double* __restrict__ coarse, const Coarse& C,
+                                            const Pyramid& P, int m, int x, int y, int z, int ch, const int face[3],
+                                            const float w[3], float v)
+{
+    if (m >= C.first) {
+        const int n = P.n[m], n3 = n * n * n;
+        const int t = ((z - 1) * n + (y - 1)) * n + (x - 1);
+        for (int a = 0; a < 3; ++a) atomicAdd(coarse + C.base[m] + (face[a] * 4 + ch) * n3 + t, (double)(w[a] * v));
+        return;
+    }
+    const unsigned long long D = (unsigned long long)P.n[m] + 2;
+    const unsigned long long idx = P.offset[m] + ((unsigned long long)z * D + (unsigned long long)y) * D + x;
+    if (m == 0) {
+        unsafeAtomicAdd(grad + 4ull * idx + ch, (w[0] + w[1] + w[2]) * v);
+        return;
+    }
+    for (int a = 0; a < 3; ++a)
+        unsafeAtomicAdd(grad + 4ull * (idx + (unsigned long long)face[a] * P.block[m]) + ch, w[a] * v);
+}
+
+// Adds v to window element t (4 channels) in the window's precision.
+__device__ __forceinline__ void lds_add4(double* __restrict__ win, bool f32, int t, const float v[4])
+{
+    if (f32) {
+        float* f = (float*)win + t;
+        atomicAdd(f, v[0]);
+        atomicAdd(f + 1, v[1]);
+        atomicAdd(f + 2, v[2]);
+        atomicAdd(f + 3, v[3]);
+    } else {
+        double* d = win + t;
+        atomicAdd(d, (double)v[0]);
+        atomicAdd(d + 1, (double)v[1]);
+        atomicAdd(d + 2, (double)v[2]);
+        atomicAdd(d + 3, (double)v[3]);
+    }
+}
+
+// Gradient of one step's samples on level m: into the wave's LDS window when
+// it has one, else straight to add_channel. A lane visits its 8 corners in
+// the order j ^ r with r set by its pixel position in the 8 x 8 wave, so the
+// lanes of one footprint rarely meet in the same cell in one LDS instruction.
+__device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __restrict__ win,
+                                            double* __restrict__ coarse, const Coarse& C, const Pyramid& P, int m,
+                                            const Window& W, float scale, const float start[3], const float pos[3],
+                                            const f4& gs, const int face[3], const float w[3], int lane)
+{
+    const Corner c = corner_of(P, m, pos);
+    if (!c.valid) return;
+    const Box& B = W.part[W.parts == 2 && start[W.axis] >= W.mid ? 1 : 0];
+    const float g[4] = {gs.x * scale, gs.y * scale, gs.z * scale, gs.w * scale};
+    const int px = lane & 7, py = lane >> 3;
+    const int r = (px & 1) | ((py & 1) << 1) | ((((px >> 1) ^ (py >> 1)) & 1) << 2);
+    const int rx = c.i[0] - B.lo[0], ry = c.i[1] - B.lo[1], rz = c.i[2] - B.lo[2];
+    if (W.lds && rx >= 0 && ry >= 0 && rz >= 0 && rx + 1 < B.d[0] && ry + 1 < B.d[1] && rz + 1 < B.d[2]) {
+        // the whole footprint lies in the window (whose cells are all interior)
+        const int d0 = B.d[0], d01 = B.d[0] * B.d[1];
+        const int cp = 4 * (lane & (W.copies - 1)) * B.vol;
+        const int bx = r & 1, by = (r >> 1) & 1, bz = r >> 2;
+        const int t0 = B.base + cp + 4 * ((rz + bz) * d01 + (ry + by) * d0 + rx + bx);
+        const int sx = bx ? -4 : 4, sy = by ? -4 * d0 : 4 * d0, sz = bz ? -4 * d01 : 4 * d01;
+        const float wx[2] = {bx ? c.f[0] : 1.0f - c.f[0], bx ? 1.0f - c.f[0] : c.f[0]};
+        const float wy[2] = {by ? c.f[1] : 1.0f - c.f[1], by ? 1.0f - c.f[1] : c.f[1]};
+        const float wz[2] = {bz ? c.f[2] : 1.0f - c.f[2], bz ? 1.0f - c.f[2] : c.f[2]};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int jx = j & 1, jy = (j >> 1) & 1, jz = j >> 2;
+            const float k = wx[jx] * wy[jy] * wz[jz];
+            const float v[4] = {k * g[0], k * g[1], k * g[2], k * g[3]};
+            lds_add4(win, W.f32, t0 + (jx ? sx : 0) + (jy ? sy : 0) + (jz ? sz : 0), v);
+        }
+        return;
+    }
+    const int n = P.n[m];
+    const int cp = W.lds ? 4 * (lane & (W.copies - 1)) * B.vol : 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
+        const int x = c.i[0] + qx, y = c.i[1] + qy, z = c.i[2] + qz;
+        if (x < 1 || x > n || y < 1 || y > n || z < 1 || z > n) continue;
+        const float k = (qx ? c.f[0] : 1.0f - c.f[0]) * (qy ? c.f[1] : 1.0f - c.f[1]) * (qz ? c.f[2] : 1.0f - c.f[2]);
+        const float v[4] = {k * g[0], k * g[1], k * g[2], k * g[3]};
+        const int ux = x - B.lo[0], uy = y - B.lo[1], uz = z - B.lo[2];
+        if (W.lds && (unsigned)ux < (unsigned)B.d[0] && (unsigned)uy < (unsigned)B.d[1] &&
+            (unsigned)uz < (unsigned)B.d[2])
+            lds_add4(win, W.f32, B.base + cp + 4 * ((uz * B.d[1] + uy) * B.d[0] + ux), v);
+        else
+            for (int ch = 0; ch < 4; ++ch) add_channel(grad, coarse, C, P, m, x, y, z, ch, face, w, v[ch]);
+    }
+}
+
+// Sums a window's copies into global memory (or the tile's coarse
+// accumulator) and leaves the window zeroed. Lane l takes (texel, channel)
+// entries l, l + 64, ...: consecutive lanes add to consecutive floats of a row
+// of the gradient pyramid. Texel coordinates come from float reciprocals,
+// exact here since a window holds at most 2^10 texels.
+__device__ __forceinline__ void flush_window(float* __restrict__ grad, double* __restrict__ win,
+                                             double* __restrict__ coarse, const Coarse& C, const Pyramid& P, int m,
+                                             const Window& W, const int face[3], const float w[3], int lane)
+{
+    const int n = P.n[m], D = n + 2;
+    const float ws = w[0] + w[1] + w[2];
+    for (int pp = 0; pp < W.parts; ++pp) {
+        const Box& B = W.part[pp];
+        const int d0 = B.d[0], d01 = B.d[0] * B.d[1];
+        const float inv_d0 = 1.0f / (float)d0, inv_d01 = 1.0f / (float)d01;
+        const unsigned long long org =
+            P.offset[m] + ((unsigned long long)B.lo[2] * D + (unsigned long long)B.lo[1]) * D + B.lo[0];
+        float* g0 = grad + 4ull * org;
+        float* gf[3];
+        for (int a = 0; a < 3; ++a) gf[a] = g0 + (m == 0 ? 0ull : 4ull * (unsigned long long)face[a] * P.block[m]);
+        for (int e = lane; e < 4 * B.vol; e += 64) {
+            double v = 0.0;
+            for (int cc = 0; cc < W.copies; ++cc) {
+                const int t = B.base + 4 * cc * B.vol + e;
+                if (W.f32) {
+                    v += ((float*)win)[t];
+                    ((float*)win)[t] = 0.0f;
+                } else {
+                    v += win[t];
+                    win[t] = 0.0;
+                }
+            }
+            if (v == 0.0) continue;
+            const int t = e >> 2, ch = e & 3;
+            const int z = (int)(((float)t + 0.5f) * inv_d01);
+            const int rem = t - z * d01;
+            const int y = (int)(((float)rem + 0.5f) * inv_d0);
+            const int x = rem - y * d0;
+            const float fv = (float)v;
+            if (m >= C.first) {
+                const int n3 = n * n * n;
+                const int tc = ((B.lo[2] + z - 1) * n + (B.lo[1] + y - 1)) * n + (B.lo[0] + x - 1);
+                for (int a = 0; a < 3; ++a) atomicAdd(coarse + C.base[m] + (face[a] * 4 + ch) * n3 + tc, (double)(w[a] * fv));
+                continue;
+            }
+            const int off = 4 * ((z * D + y) * D + x) + ch;
+            if (m == 0) {
+                unsafeAtomicAdd(g0 + off, ws * fv);
+            } else {
+                for (int a = 0; a < 3; ++a) unsafeAtomicAdd(gf[a] + off, w[a] * fv);
+            }
+        }
+    }
+}
+
+// Orders one wave's LDS accesses before and after (the wave's lanes write
+// cells that other lanes of the same wave read next).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One 16 x 16 pixel tile per workgroup; each wave (8 x 8 pixels) runs on its
+// own. Per cone: every lane replays its march forward keeping the per-step
+// state, then the wave sweeps the steps in reverse in lockstep (the step
+// sequence does not depend on the data, so every lane's step k is at the
+// same distance and level):
+//   op = 1 - (1 - s.a)^c,  b = 1 - R.a,  q = 1 + (t + vs_l) decay
+//   R.rgb += b c s.rgb,  R.a += b op,  O += (1 - O) op / q
+// gives  ds.rgb = gR.rgb b c,  d op = gR.a b + gO (1 - O) / q,
+//        ds.a = d op c (1 - s.a)^(c-1),
+//        gR.a <- gR.a (1 - op) - c gR.rgb . s.rgb,  gO <- gO (1 - op / q).
+// Each step's sample gradients are spread over their trilinear footprints
+// into an LDS window per level (lane-striped copies when the window is
+// small, so lanes sharing a few texels do not serialize), then flushed with
+// one add per touched texel, face and channel: into the tile's LDS
+// accumulators for the coarsest levels (written to global memory once, at
+// the end), into global memory with atomics for the others. Without kStore
+// the samples are fetched again in the reverse sweep instead of kept.
+template <bool kStore>
+__global__ __launch_bounds__(kBwdThreads) void k_dense_trace_bwd(const void* __restrict__ data, Pyramid P, March M,
+                                                                 const uint32_t* __restrict__ g_diffuse,
+                                                                 const uint16_t* __restrict__ g_normal,
+                                                                 const float* __restrict__ g_depth, int width,
+                                                                 int height, const float4* __restrict__ grad_out,
+                                                                 float* __restrict__ grad, uint32_t* __restrict__ status)
+{
+    __shared__ double s_win[4][2 * kPoolUnits];
+    __shared__ double s_coarse[kCoarseFloats];
+    __shared__ float s_step[kAdjMaxSteps], s_seg[kAdjMaxSteps], s_lvl[kAdjMaxSteps];
+    __shared__ int s_nsteps;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned tiles_x = ((unsigned)width + 15u) / 16u;
     const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
     const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
-    if (px >= width || py >= height) return;
-    const size_t idx = (size_t)py * (size_t)width + (size_t)px;
-    const float depth = g_depth[idx];
-    if (depth == 1.0f) return;
-    float start[3], normal[3], albedo[3];
-    pixel_setup(P, M, px, py, width, height, depth, g_normal + 4 * idx, g_diffuse[idx], start, normal, albedo);
-    const float4 go = grad_out[idx];
+    const float vs = P.vs;
+    const Coarse C = coarse_layout(P);
+    double* win = s_win[wave];
+    for (int i = lane; i < 2 * kPoolUnits; i += 64) win[i] = 0.0;
+    for (int i = tid; i < kCoarseFloats; i += kBwdThreads) s_coarse[i] = 0.0;
+    if (tid == 0) {
+        // the march's step sequence (march() in vct_dense_march.hpp)
+        const float inv_vs = 1.0f / vs, max_level = (float)(P.nlevels - 1);
+        float step = 0.0f, diameter = vs, seg = vs;
+        int cnt = 0;
+        while (step < M.max_dist && cnt < kAdjMaxSteps) {
+            s_step[cnt] = step;
+            s_seg[cnt] = seg;
+            s_lvl[cnt] = fminf(fmaxf(log2f(diameter * inv_vs), 0.0f), max_level);
+            ++cnt;
+            const float prev = step;
+            step += fmaxf(diameter, vs) * M.step_factor;
+            seg = step - prev;
+            diameter = step * M.coef;
+        }
+        s_nsteps = step < M.max_dist ? -1 : cnt;
+    }
+    bool on = px < width && py < height;
+    size_t idx = 0;
+    float start[3] = {0.0f, 0.0f, 0.0f}, normal[3] = {0.0f, 0.0f, 1.0f}, albedo[3] = {0.0f, 0.0f, 0.0f};
+    float4 go = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (on) {
+        idx = (size_t)py * (size_t)width + (size_t)px;
+        const float depth = g_depth[idx];
+        on = depth != 1.0f;
+        if (on) {
+            pixel_setup(P, M, px, py, width, height, depth, g_normal + 4 * idx, g_diffuse[idx], start, normal, albedo);
+            go = grad_out[idx];
+        }
+    }
+    __syncthreads();
+    const int nsteps = s_nsteps;
+    if (nsteps < 0) {
+        if (tid == 0) atomicOr(status, 1u);
+        return;
+    }
+    const bool wave_on = wave_max(on ? 1.0f : 0.0f) > 0.0f;
     const float inv = 1.0f / (float)M.ncones;
-    for (int i = 0; i < M.ncones; ++i) {
+    const float lead = vs * M.start_offset * 0.5f;
+    for (int i = 0; wave_on && i < M.ncones; ++i) {
         const float* d = cone_dir(M, i);
-        const float ct = normal[0] * d[0] + normal[1] * d[1] + normal[2] * d[2];
-        if (ct < 0.0f) continue;
-        const f4 gC = {go.x * inv * albedo[0] * M.diffuse_intensity * ct, go.y * inv * albedo[1] * M.diffuse_intensity * ct,
-                       go.z * inv * albedo[2] * M.diffuse_intensity * ct, go.w * inv * M.ao_factor * ct};
-        if (!cone_backward<kStore>(data, grad, P, M, start, d, gC)) atomicOr(status, 1u);
+        const float dir[3] = {d[0], d[1], d[2]};
+        const int face[3] = {dir[0] > 0.0f ? 0 : 1, dir[1] > 0.0f ? 2 : 3, dir[2] > 0.0f ? 4 : 5};
+        const float w[3] = {dir[0] * dir[0], dir[1] * dir[1], dir[2] * dir[2]};
+        const float ct = normal[0] * dir[0] + normal[1] * dir[1] + normal[2] * dir[2];
+        const bool lane_on = on && ct >= 0.0f;
+        // box of the start points of the lanes that trace this cone
+        float blo[3], bhi[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            blo[k] = wave_min(lane_on ? start[k] : INFINITY);
+            bhi[k] = wave_max(lane_on ? start[k] : -INFINITY);
+        }
+        float cstart[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) cstart[k] = start[k] + dir[k] * vs * M.start_offset * 0.5f;
+        // forward replay
+        float st_rw[kAdjMaxSteps], st_occ[kAdjMaxSteps];
+        f4 smp[kStore ? kAdjMaxSteps : 1];
+        int n = 0;
+        if (lane_on) {
+            f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
+            float occlusion = 0.0f;
+            for (int k = 0; k < nsteps && occlusion < 1.0f; ++k) {
+                float pos[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) pos[a] = cstart[a] + dir[a] * s_step[k];
+                const f4 s = level_sample<true>(data, P, pos, s_lvl[k], face, w);
+                st_rw[k] = result.w;
+                st_occ[k] = occlusion;
+                if (kStore) smp[kStore ? k : 0] = s;
+                accumulate(result, occlusion, s, s_lvl[k], s_step[k], s_seg[k], vs, M.decay);
+                n = k + 1;
+            }
+        }
+        const int kc = (int)wave_max((float)n);
+        f4 gR = {go.x * inv * albedo[0] * M.diffuse_intensity * ct, go.y * inv * albedo[1] * M.diffuse_intensity * ct,
+                 go.z * inv * albedo[2] * M.diffuse_intensity * ct, 0.0f};
+        float gO = -(go.w * inv * M.ao_factor * ct);
+        // Level m's window lives while the sweep is on m's steps; the two
+        // levels in use at a step are consecutive, so m & 1 names its slot.
+        // Even levels take the wave's pool from the bottom, odd ones from the
+        // top, each up to what the other live window leaves.
+        int slot_m[2] = {-1, -1};
+        Window slot_w[2];
+        for (int k = kc - 1; k >= 0; --k) {
+            const float stp = s_step[k], lvl = s_lvl[k];
+            float pos[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pos[a] = cstart[a] + dir[a] * stp;
+            f4 gs = {0.0f, 0.0f, 0.0f, 0.0f};
+            const bool has = k < n;
+            if (has) {
+                const f4 s = kStore ? smp[kStore ? k : 0] : level_sample<true>(data, P, pos, lvl, face, w);
+                const float vsl = vs * exp2f(lvl);
+                const float c = s_seg[k] / vsl;
+                const float q = 1.0f + (stp + vsl) * M.decay;
+                const float base = 1.0f - s.w;
+                const float op = 1.0f - powf(base, c);
+                const float b = 1.0f - st_rw[k];
+                const float g_op = gR.w * b + gO * (1.0f - st_occ[k]) / q;
+                gs.x = gR.x * b * c;
+                gs.y = gR.y * b * c;
+                gs.z = gR.z * b * c;
+                gs.w = base > 0.0f ? g_op * c * powf(base, c - 1.0f) : 0.0f;
+                const float dot = gR.x * s.x + gR.y * s.y + gR.z * s.z;
+                gR.w = gR.w * (1.0f - op) - c * dot;
+                gO = gO * (1.0f - op / q);
+            }
+            const int lo = (int)floorf(lvl), hi = (int)ceilf(lvl);
+            const float fl = lvl - floorf(lvl);
+            const int nlev = hi != lo ? 2 : 1;
+            const int ml[2] = {lo, hi};
+            const float sc[2] = {hi != lo ? 1.0f - fl : 1.0f, fl};
+            for (int j = 0; j < nlev; ++j) {
+                const int m = ml[j], sl = m & 1;
+                if (slot_m[sl] == m) continue;
+                if (slot_m[sl] >= 0 && slot_w[sl].lds) {
+                    wave_lds_sync();
+                    flush_window(grad, win, s_coarse, C, P, slot_m[sl], slot_w[sl], face, w, lane);
+                    wave_lds_sync();
+                }
+                // steps k1..k that use level m (the level rises with k)
+                int k1 = k;
+                while (k1 > 0 && (int)ceilf(s_lvl[k1 - 1]) >= m) --k1;
+                slot_m[sl] = m;
+                const int other = slot_m[sl ^ 1] >= 0 ? slot_w[sl ^ 1].used : 0;
+                slot_w[sl] = window_of(P, m, start, lane_on, blo, bhi, dir, lead + s_step[k1], lead + stp, 0,
+                                       kPoolUnits - other);
+                if (sl) {
+                    // the top of the pool, in the window's elements (4 floats or 2 doubles per unit)
+                    const int shift = (slot_w[sl].f32 ? 4 : 2) * (kPoolUnits - slot_w[sl].used);
+                    slot_w[sl].part[0].base += shift;
+                    slot_w[sl].part[1].base += shift;
+                }
+            }
+            if (has) {
+                for (int j = 0; j < nlev; ++j)
+                    if (sc[j] != 0.0f)
+                        splat_level(grad, win, s_coarse, C, P, ml[j], slot_w[ml[j] & 1], sc[j], start, pos, gs, face,
+                                    w, lane);
+            }
+        }
+        wave_lds_sync();
+        for (int sl = 0; sl < 2; ++sl)
+            if (slot_m[sl] >= 0 && slot_w[sl].lds)
+                flush_window(grad, win, s_coarse, C, P, slot_m[sl], slot_w[sl], face, w, lane);
+        wave_lds_sync();
+    }
+    // the tile's coarse levels to global memory
+    __syncthreads();
+    for (int m = C.first; m < P.nlevels; ++m) {
+        const int n = P.n[m], n3 = n * n * n;
+        const unsigned long long D = (unsigned long long)n + 2;
+        for (int e = tid; e < 24 * n3; e += kBwdThreads) {
+            const float v = (float)s_coarse[C.base[m] + e];
+            if (v == 0.0f) continue;
+            const int t = e % n3, ch = (e / n3) & 3, f = e / (4 * n3);
+            const int x = t % n + 1, y = (t / n) % n + 1, z = t / (n * n) + 1;
+            const unsigned long long texel =
+                P.offset[m] + (unsigned long long)f * P.block[m] + ((unsigned long long)z * D + y) * D + x;
+            unsafeAtomicAdd(grad + 4ull * texel + ch, v);
+        }
     }
 }
 
@@ -305,14 +682,18 @@ vct_status vct_dense_trace_backward(vct_dense* g, vct_stream stream, const vct_g
     VCT_TRY(dense::make_params(g, cam, d, &P, &M));
     if (march_steps(P, M) + 1 > dense::kAdjMaxSteps)
         return fail(VCT_ERR_UNSUPPORTED, "the march takes more than %d steps", dense::kAdjMaxSteps - 1);
+    int coarse = 0;
+    for (int m = 1; m < P.nlevels; ++m)
+        if (P.n[m] <= dense::kCoarseN) coarse += 24 * P.n[m] * P.n[m] * P.n[m];
+    if (coarse > dense::kCoarseFloats) return fail(VCT_ERR_UNSUPPORTED, "coarse levels exceed the tile accumulator");
     dim3 grid(((gb->width + 15) / 16) * ((gb->height + 15) / 16));
     hipStream_t s = (hipStream_t)stream;
     if (flags & VCT_ADJOINT_RECOMPUTE)
-        hipLaunchKernelGGL(dense::k_dense_trace_bwd<false>, grid, dim3(256), 0, s, g->data.ptr, P, M, gb->diffuse,
+        hipLaunchKernelGGL(dense::k_dense_trace_bwd<false>, grid, dim3(dense::kBwdThreads), 0, s, g->data.ptr, P, M, gb->diffuse,
                            gb->normal, gb->depth, (int)gb->width, (int)gb->height, (const float4*)grad_out,
                            grad_pyramid, status);
     else
-        hipLaunchKernelGGL(dense::k_dense_trace_bwd<true>, grid, dim3(256), 0, s, g->data.ptr, P, M, gb->diffuse,
+        hipLaunchKernelGGL(dense::k_dense_trace_bwd<true>, grid, dim3(dense::kBwdThreads), 0, s, g->data.ptr, P, M, gb->diffuse,
                            gb->normal, gb->depth, (int)gb->width, (int)gb->height, (const float4*)grad_out,
                            grad_pyramid, status);
     VCT_HIP(hipGetLastError());
