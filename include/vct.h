@@ -478,10 +478,13 @@ typedef struct vct_batch_scene {
 typedef struct vct_batch_tuning {
     uint32_t raster_chunk;      /* triangles per raster task: 64, 128 or 256 */
     uint32_t mip_brick;         /* destination brick edge of a mip task: 4 or 8 */
-    uint32_t trace_schedule;    /* 0: one lane per pixel marching its cones in order;
-                                   1: lanes take the wave's (pixel, cone) pairs with
-                                      cos >= 0 in turn, summed per pixel in cone order */
-    uint32_t workgroups_per_cu; /* persistent grid = CUs x this: 1, 2 or 4 */
+    uint32_t trace_schedule;    /* 0: one cone step per loop trip; 1: two-step lookahead
+                                   (the samples of steps k and k+1 fetched together,
+                                   step k+1 accumulated only if the march reaches it) */
+    uint32_t workgroups_per_cu; /* 0: one flat launch per phase (vertex, raster, each mip
+                                   level, trace), kernel boundaries as dependencies;
+                                   1, 2 or 4: one persistent launch of CUs x this
+                                   workgroups pulling the tasks from a queue */
 } vct_batch_tuning;
 
 vct_status vct_batch_create(vct_batch** out, const vct_batch_scene* scenes, uint32_t count,

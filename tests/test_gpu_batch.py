@@ -1,5 +1,6 @@
-"""The batched dense method (one persistent launch, replayed from a hipGraph)
-against the per-scene dense calls it batches.
+"""The batched dense method (one flat launch per phase, or one persistent
+launch; either replayed from a hipGraph) against the per-scene dense calls it
+batches.
 
 Every scene's mip pyramid must be bit-identical to vct_dense_voxelize +
 vct_dense_mipmap on the same scene, the traced images must equal
@@ -143,7 +144,8 @@ def test_batch_matches_per_scene_calls():
     outs = [torch.zeros((H, W, 4), dtype=torch.float32, device="cuda") for _ in items]
     h = _make_batch(items, outs)
     try:
-        schedules = [(256, 8, 0, 2), (256, 8, 0, 2), (64, 4, 1, 1), (128, 8, 1, 4)]
+        # workgroups_per_cu 0: one flat launch per phase; 1..4: the persistent queue
+        schedules = [(256, 8, 0, 0), (256, 8, 0, 0), (64, 4, 1, 0), (256, 8, 0, 2), (64, 4, 1, 1), (128, 8, 1, 4)]
         for rc, mb, sched, occ in schedules:
             A.call("vct_batch_set_tuning", h, C.byref(A.BatchTuning(rc, mb, sched, occ)))
             got_out, got_pyr, total = _run_batch(h, items, outs)

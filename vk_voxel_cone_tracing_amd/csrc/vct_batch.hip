@@ -14,6 +14,11 @@
 // tiles while large scenes are still voxelizing, and a scene's tiny mip
 // levels never leave the GPU idle.
 //
+// The phased schedule (workgroups_per_cu = 0) runs the same task table as one
+// flat launch per phase instead - vertex chunks, raster chunks, each mip
+// level, pixel tiles - each kernel sized to its own task type, with kernel
+// boundaries as the only dependencies; the graph replays them back to back.
+//
 // Each task runs the same device code as the per-scene calls
 // (vct_dense_voxelize / vct_dense_mipmap / vct_dense_trace), so the grids are
 // bit-identical to theirs and the traced images equal theirs.
@@ -43,7 +48,7 @@ enum TaskType : uint32_t { kVertex = 0, kRaster = 1, kMip = 2, kTrace = 3 };
 // Phase p of a scene: 0 vertex, 1 raster, 1 + m mip level m (m >= 1).
 constexpr int kPhases = 1 + dense::kMaxLevels;
 constexpr int kTile = 16;                    // trace tile edge (pixels)
-constexpr uint32_t kSpinLimit = 1u << 22;    // about a second of polling per wait
+constexpr double kWaitSeconds = 2.0;         // a dependency wait gives up (and aborts the launch) after this
 
 struct Task {
     uint32_t type_scene;  // type << 24 | scene
@@ -95,6 +100,7 @@ struct Args {
     unsigned long long* samples;
     uint32_t mip_brick;
     uint32_t schedule;
+    long long wait_ticks;  // kWaitSeconds in wall-clock ticks
 };
 
 __global__ void k_batch_reset(uint32_t* head, uint32_t* done, uint32_t ndone, uint32_t* abort_flag)
@@ -107,14 +113,16 @@ __global__ void k_batch_reset(uint32_t* head, uint32_t* done, uint32_t ndone, ui
     }
 }
 
-// Thread 0 polls the counter; false if the launch is aborting.
-__device__ bool wait_count(const uint32_t* counter, uint32_t need, uint32_t* abort_flag, uint32_t* status)
+// Thread 0 polls the counter; false if the launch is aborting. The wait is
+// bounded by wall-clock time, whatever one poll costs under contention.
+__device__ bool wait_count(const uint32_t* counter, uint32_t need, uint32_t* abort_flag, uint32_t* status,
+                           long long ticks)
 {
-    uint32_t spins = 0;
+    const long long t0 = wall_clock64();
     while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
         if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
         __builtin_amdgcn_s_sleep(2);
-        if (++spins > kSpinLimit) {
+        if (wall_clock64() - t0 > ticks) {
             __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             atomicOr(status, 1u);
             return false;
@@ -285,7 +293,8 @@ __global__ __launch_bounds__(256) void k_batch(Args A)
         else if (type == kTrace) dep = S.pyr.nlevels;  // the last level, phase 1 + (nlevels - 1)
         if (threadIdx.x == 0) {
             bool ok = !__hip_atomic_load(A.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (ok && dep >= 0) ok = wait_count(&A.done[scene * kPhases + dep], S.need[dep], A.abort_flag, A.status);
+            if (ok && dep >= 0)
+                ok = wait_count(&A.done[scene * kPhases + dep], S.need[dep], A.abort_flag, A.status, A.wait_ticks);
             s_go = ok ? 1u : 0u;
         }
         __syncthreads();
@@ -311,6 +320,43 @@ __global__ __launch_bounds__(256) void k_batch(Args A)
     }
 }
 
+// The phased schedule: block b runs task first + b (tiles in XCD order).
+__global__ __launch_bounds__(256) void k_phase_vertex(const Task* __restrict__ tasks, uint32_t first,
+                                                      const SceneDev* __restrict__ scenes)
+{
+    const Task T = tasks[first + blockIdx.x];
+    vertex_task(scenes[T.type_scene & 0xffffffu], T.a, T.b);
+}
+
+__global__ __launch_bounds__(256) void k_phase_raster(const Task* __restrict__ tasks, uint32_t first,
+                                                      const SceneDev* __restrict__ scenes)
+{
+    __shared__ RasterLds lds;
+    const Task T = tasks[first + blockIdx.x];
+    raster_task(scenes[T.type_scene & 0xffffffu], T.a, T.b, lds);
+}
+
+__global__ __launch_bounds__(256) void k_phase_mip(const Task* __restrict__ tasks, uint32_t first,
+                                                   const SceneDev* __restrict__ scenes, uint32_t brick)
+{
+    const Task T = tasks[first + blockIdx.x];
+    mip_task(scenes[T.type_scene & 0xffffffu], T.a, (int)T.level, brick);
+}
+
+template <bool kAhead>
+__global__ __launch_bounds__(256) void k_phase_trace(const Task* __restrict__ tasks, uint32_t first,
+                                                     const SceneDev* __restrict__ scenes, unsigned long long* samples)
+{
+    const Task T = tasks[first + xcd_remap(blockIdx.x, gridDim.x)];
+    uint32_t steps = 0;
+    trace_task(scenes[T.type_scene & 0xffffffu], T.a, kAhead ? 1u : 0u, steps);
+    if (samples) {
+        unsigned long long v = steps;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(samples, v);
+    }
+}
+
 }  // namespace batch
 }  // namespace vct
 
@@ -321,11 +367,16 @@ struct vct_batch {
     std::vector<vct_batch_scene> scenes;
     std::vector<SceneDev> dev;  // host copy of the scene table
     vct_dense_trace_desc trace{};
-    vct_batch_tuning tuning{256, 8, 0, 2};
+    vct_batch_tuning tuning{256, 8, 0, 0};
     int cus = 0;
     DeviceBuffer scene_table, prims, tasks, counters, soup_pos, soup_mat, data, winner;
     uint32_t ntasks = 0;
     uint32_t ndone = 0;
+    struct Phase {
+        uint32_t type, level, first, count;
+    };
+    std::vector<Phase> phases;  // contiguous runs of the task table, in order
+    long long wait_ticks = 0;
     std::vector<size_t> data_off, winner_off;
     hipStream_t capture = nullptr;
     hipGraph_t graph = nullptr;
@@ -383,6 +434,13 @@ static vct_status build_tasks(vct_batch* b)
         for (uint32_t k = 0; k < tiles; ++k) t.push_back({(kTrace << 24) | s, k, 0, 0});
     }
     b->ntasks = (uint32_t)t.size();
+    b->phases.clear();
+    for (uint32_t i = 0; i < b->ntasks; ++i) {
+        const uint32_t type = t[i].type_scene >> 24, level = type == kMip ? t[i].level : 0;
+        if (b->phases.empty() || b->phases.back().type != type || b->phases.back().level != level)
+            b->phases.push_back({type, level, i, 0});
+        ++b->phases.back().count;
+    }
     VCT_TRY(ensure(b->tasks, sizeof(Task) * t.size()));
     VCT_HIP(hipMemcpy(b->tasks.ptr, t.data(), sizeof(Task) * t.size(), hipMemcpyHostToDevice));
     VCT_HIP(hipMemcpy(b->scene_table.ptr, b->dev.data(), sizeof(SceneDev) * ns, hipMemcpyHostToDevice));
@@ -404,11 +462,30 @@ static vct_status capture_graph(vct_batch* b, unsigned long long* samples)
     A.samples = samples;
     A.mip_brick = b->tuning.mip_brick;
     A.schedule = b->tuning.trace_schedule;
+    A.wait_ticks = b->wait_ticks;
     const unsigned grid = (unsigned)(b->cus * (int)b->tuning.workgroups_per_cu);
     VCT_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_batch_reset, dim3((b->ndone + 255) / 256), dim3(256), 0, b->capture, A.head, A.done, b->ndone,
                        A.abort_flag);
-    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), 0, b->capture, A);
+    if (grid) {
+        hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), 0, b->capture, A);
+    } else {
+        for (const vct_batch::Phase& p : b->phases) {
+            const dim3 g(p.count);
+            if (p.type == kVertex)
+                hipLaunchKernelGGL(k_phase_vertex, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes);
+            else if (p.type == kRaster)
+                hipLaunchKernelGGL(k_phase_raster, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes);
+            else if (p.type == kMip)
+                hipLaunchKernelGGL(k_phase_mip, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes, A.mip_brick);
+            else if (A.schedule)
+                hipLaunchKernelGGL(k_phase_trace<true>, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes,
+                                   samples);
+            else
+                hipLaunchKernelGGL(k_phase_trace<false>, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes,
+                                   samples);
+        }
+    }
     hipError_t e = hipStreamEndCapture(b->capture, &b->graph);
     if (e != hipSuccess) {
         b->graph = nullptr;
@@ -451,9 +528,12 @@ static vct_status batch_init(vct_batch* b, const vct_batch_scene* scenes, uint32
 {
     b->scenes.assign(scenes, scenes + count);
     b->trace = *trace;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
         return fail(VCT_ERR_HIP, "device query failed");
+    b->wait_ticks = (long long)(kWaitSeconds * 1e3 * (double)khz);
     // per-scene validation and sizes
     std::vector<Prim> prims;
     size_t tris_total = 0, data_total = 0, winner_total = 0;
@@ -588,8 +668,7 @@ vct_status vct_batch_set_tuning(vct_batch* b, const vct_batch_tuning* t)
         return fail(VCT_ERR_INVALID_ARGUMENT, "raster_chunk must be 64, 128 or 256");
     if (t->mip_brick != 4 && t->mip_brick != 8) return fail(VCT_ERR_INVALID_ARGUMENT, "mip_brick must be 4 or 8");
     if (t->trace_schedule > 1) return fail(VCT_ERR_INVALID_ARGUMENT, "trace_schedule must be 0 or 1");
-    if (t->workgroups_per_cu < 1 || t->workgroups_per_cu > 4)
-        return fail(VCT_ERR_INVALID_ARGUMENT, "workgroups_per_cu must be 1..4");
+    if (t->workgroups_per_cu > 4) return fail(VCT_ERR_INVALID_ARGUMENT, "workgroups_per_cu must be 0..4");
     VCT_HIP(hipDeviceSynchronize());
     const bool retask = t->raster_chunk != b->tuning.raster_chunk || t->mip_brick != b->tuning.mip_brick;
     b->tuning = *t;
@@ -654,14 +733,14 @@ vct_status vct_batch_autotune(vct_batch* b, vct_stream stream, uint32_t reps, vc
         return vct_batch_status(b);
     };
     // coordinate descent over the four knobs, two sweeps
-    static const uint32_t chunks[] = {64, 128, 256}, bricks[] = {4, 8}, scheds[] = {0, 1}, occ[] = {1, 2, 4};
+    static const uint32_t chunks[] = {64, 128, 256}, bricks[] = {4, 8}, scheds[] = {0, 1}, occ[] = {0, 1, 2, 4};
     vct_batch_tuning cur = b->tuning;
     float cur_ms = 0.0f;
     vct_status st = time_it(cur, &cur_ms);
     for (int sweep = 0; st == VCT_OK && sweep < 2; ++sweep) {
         for (int knob = 0; st == VCT_OK && knob < 4; ++knob) {
             const uint32_t* vals = knob == 0 ? chunks : knob == 1 ? bricks : knob == 2 ? scheds : occ;
-            const int nv = knob == 0 ? 3 : knob == 1 ? 2 : knob == 2 ? 2 : 3;
+            const int nv = knob == 0 ? 3 : knob == 1 ? 2 : knob == 2 ? 2 : 4;
             for (int k = 0; st == VCT_OK && k < nv; ++k) {
                 vct_batch_tuning t = cur;
                 uint32_t* field = knob == 0 ? &t.raster_chunk : knob == 1 ? &t.mip_brick
