@@ -14,15 +14,18 @@
 // tiles while large scenes are still voxelizing, and a scene's tiny mip
 // levels never leave the GPU idle.
 //
-// The phased schedule (workgroups_per_cu = 0) runs the same task table as one
-// flat launch per phase instead - vertex chunks, raster chunks, each mip
-// level, pixel tiles - each kernel sized to its own task type, with kernel
-// boundaries as the only dependencies; the graph replays them back to back.
+// The phased schedule (workgroups_per_cu = 0) runs the same work as one flat
+// launch per phase instead - vertex chunks; the raster of every scene's
+// triangles as one count / scan / sample-parallel pass (the per-scene
+// voxelizer's scheme over the whole batch); each mip level; pixel tiles -
+// each kernel sized to its own task type, with kernel boundaries as the only
+// dependencies; the graph replays them back to back.
 //
 // Each task runs the same device code as the per-scene calls
 // (vct_dense_voxelize / vct_dense_mipmap / vct_dense_trace), so the grids are
 // bit-identical to theirs and the traced images equal theirs.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -320,6 +323,84 @@ __global__ __launch_bounds__(256) void k_batch(Args A)
     }
 }
 
+// Scene of global soup triangle g: the last s with tri_start[s] <= g.
+__device__ __forceinline__ uint32_t scene_of(const uint32_t* __restrict__ tri_start, uint32_t nscenes, uint32_t g)
+{
+    uint32_t lo = 0, hi = nscenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tri_start[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Raster samples of every triangle of the batch (k_dense_count per scene).
+__global__ __launch_bounds__(256) void k_phase_count(const SceneDev* __restrict__ scenes,
+                                                     const uint32_t* __restrict__ tri_start, uint32_t nscenes,
+                                                     uint32_t ntris, unsigned long long* __restrict__ counts)
+{
+#pragma clang fp contract(off)
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ntris) return;
+    const uint32_t sc = scene_of(tri_start, nscenes, g);
+    const SceneDev& S = scenes[sc];
+    const uint32_t t = g - tri_start[sc];
+    float p[9];
+    for (int k = 0; k < 9; ++k) p[k] = S.soup_pos[9u * t + k] - S.grid_min[k % 3];
+    const int lo[3] = {0, 0, 0}, hi[3] = {S.res - 1, S.res - 1, S.res - 1};
+    TriSetup st;
+    unsigned long long n = 0;
+    if (tri_setup(p, S.vs, lo, hi, st)) n = (unsigned long long)st.nu * st.nv * (unsigned long long)S.samples;
+    counts[g] = n;
+}
+
+__global__ void k_phase_total(const unsigned long long* __restrict__ counts,
+                              const unsigned long long* __restrict__ offsets, uint32_t n,
+                              unsigned long long* __restrict__ total)
+{
+    total[0] = n ? offsets[n - 1] + counts[n - 1] : 0ull;
+}
+
+// Every covered sample of the batch proposes its (scene-local) triangle for
+// its voxel; the smallest index wins (k_dense_raster per scene).
+__global__ __launch_bounds__(256) void k_phase_raster_flat(const SceneDev* __restrict__ scenes,
+                                                           const uint32_t* __restrict__ tri_start, uint32_t nscenes,
+                                                           uint32_t ntris, const unsigned long long* __restrict__ offsets,
+                                                           const unsigned long long* __restrict__ total)
+{
+#pragma clang fp contract(off)
+    const unsigned long long n = total[0];
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long wi = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; wi < n; wi += stride) {
+        uint32_t lo = 0, hi = ntris;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (offsets[mid] <= wi) lo = mid; else hi = mid;
+        }
+        const uint32_t g = lo;
+        const uint32_t sc = scene_of(tri_start, nscenes, g);
+        const SceneDev& S = scenes[sc];
+        const uint32_t t = g - tri_start[sc];
+        float p[9];
+        for (int k = 0; k < 9; ++k) p[k] = S.soup_pos[9u * t + k] - S.grid_min[k % 3];
+        const int rlo[3] = {0, 0, 0}, rhi[3] = {S.res - 1, S.res - 1, S.res - 1};
+        TriSetup st;
+        if (!tri_setup(p, S.vs, rlo, rhi, st)) continue;
+        float fp[3];
+        double lam[3];
+        if (!tri_sample(p, st, S.vs, S.samples, (uint32_t)(wi - offsets[g]), fp, lam)) continue;
+        int ic[3];
+        bool inside = true;
+        for (int k = 0; k < 3; ++k) {
+            const float f = floorf(fp[k] / S.vs);
+            if (!(f >= 0.0f) || f >= (float)S.res) inside = false;
+            ic[k] = (int)f;
+        }
+        if (!inside) continue;
+        atomicMin(&S.winner[((size_t)ic[2] * S.res + ic[1]) * S.res + ic[0]], t);
+    }
+}
+
 // The phased schedule: block b runs task first + b (tiles in XCD order).
 __global__ __launch_bounds__(256) void k_phase_vertex(const Task* __restrict__ tasks, uint32_t first,
                                                       const SceneDev* __restrict__ scenes)
@@ -370,6 +451,8 @@ struct vct_batch {
     vct_batch_tuning tuning{256, 8, 0, 0};
     int cus = 0;
     DeviceBuffer scene_table, prims, tasks, counters, soup_pos, soup_mat, data, winner;
+    DeviceBuffer tri_start, counts, offsets, total, scan_tmp;  // the phased raster
+    uint32_t ntris = 0;
     uint32_t ntasks = 0;
     uint32_t ndone = 0;
     struct Phase {
@@ -472,10 +555,29 @@ static vct_status capture_graph(vct_batch* b, unsigned long long* samples)
     } else {
         for (const vct_batch::Phase& p : b->phases) {
             const dim3 g(p.count);
-            if (p.type == kVertex)
+            if (p.type == kVertex) {
                 hipLaunchKernelGGL(k_phase_vertex, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes);
-            else if (p.type == kRaster)
-                hipLaunchKernelGGL(k_phase_raster, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes);
+            } else if (p.type == kRaster) {
+                // the raster chunks as one sample-parallel pass over the batch
+                const uint32_t ns = (uint32_t)b->scenes.size(), nt = b->ntris;
+                auto* counts = (unsigned long long*)b->counts.ptr;
+                auto* offsets = (unsigned long long*)b->offsets.ptr;
+                auto* total = (unsigned long long*)b->total.ptr;
+                hipLaunchKernelGGL(k_phase_count, dim3((nt + 255) / 256), dim3(256), 0, b->capture, A.scenes,
+                                   (const uint32_t*)b->tri_start.ptr, ns, nt, counts);
+                size_t tmp = b->scan_tmp.bytes;
+                hipError_t e = hipcub::DeviceScan::ExclusiveSum(b->scan_tmp.ptr, tmp, counts, offsets, (int)nt,
+                                                                b->capture);
+                if (e != hipSuccess) {
+                    hipGraph_t g0 = nullptr;
+                    hipStreamEndCapture(b->capture, &g0);
+                    if (g0) hipGraphDestroy(g0);
+                    return fail(VCT_ERR_HIP, "batched raster scan: %s", hipGetErrorString(e));
+                }
+                hipLaunchKernelGGL(k_phase_total, dim3(1), dim3(1), 0, b->capture, counts, offsets, nt, total);
+                hipLaunchKernelGGL(k_phase_raster_flat, dim3((unsigned)(b->cus * 32)), dim3(256), 0, b->capture,
+                                   A.scenes, (const uint32_t*)b->tri_start.ptr, ns, nt, offsets, total);
+            }
             else if (p.type == kMip)
                 hipLaunchKernelGGL(k_phase_mip, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes, A.mip_brick);
             else if (A.schedule)
@@ -627,15 +729,29 @@ static vct_status batch_init(vct_batch* b, const vct_batch_scene* scenes, uint32
     VCT_TRY(ensure(b->scene_table, sizeof(SceneDev) * count));
     if (!prims.empty())
         VCT_HIP(hipMemcpy(b->prims.ptr, prims.data(), sizeof(Prim) * prims.size(), hipMemcpyHostToDevice));
+    if (tris_total > 0x7fffffffull) return fail(VCT_ERR_CAPACITY, "more than 2^31 triangles in one batch");
+    std::vector<uint32_t> tri_start(count);
     for (uint32_t i = 0; i < count; ++i) {
         SceneDev& d = b->dev[i];
         d.prims = (const Prim*)b->prims.ptr + (uintptr_t)d.prims;
         const size_t tb = (size_t)(uintptr_t)d.soup_pos;
+        tri_start[i] = (uint32_t)tb;
         d.soup_pos = (float*)b->soup_pos.ptr + 9u * tb;
         d.soup_mat = (uint32_t*)b->soup_mat.ptr + tb;
         d.data = (uint32_t*)b->data.ptr + b->data_off[i];
         d.winner = (uint32_t*)b->winner.ptr + b->winner_off[i];
     }
+    // the phased raster's tables
+    b->ntris = (uint32_t)tris_total;
+    VCT_TRY(ensure(b->tri_start, 4u * count));
+    VCT_HIP(hipMemcpy(b->tri_start.ptr, tri_start.data(), 4u * count, hipMemcpyHostToDevice));
+    VCT_TRY(ensure(b->counts, 8u * (tris_total ? tris_total : 1)));
+    VCT_TRY(ensure(b->offsets, 8u * (tris_total ? tris_total : 1)));
+    VCT_TRY(ensure(b->total, 16));
+    size_t scan_bytes = 0;
+    VCT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (unsigned long long*)b->counts.ptr,
+                                             (unsigned long long*)b->offsets.ptr, (int)tris_total, (hipStream_t)0));
+    VCT_TRY(ensure(b->scan_tmp, scan_bytes ? scan_bytes : 16));
     // borders stay zero forever; winners start empty and every launch leaves them empty
     VCT_HIP(hipMemset(b->data.ptr, 0, 4u * data_total));
     VCT_HIP(hipMemset(b->winner.ptr, 0xff, 4u * winner_total));
@@ -654,8 +770,9 @@ vct_status vct_batch_destroy(vct_batch* b)
     hipDeviceSynchronize();
     drop_graph(b);
     if (b->capture) hipStreamDestroy(b->capture);
-    DeviceBuffer* bufs[] = {&b->scene_table, &b->prims, &b->tasks, &b->counters, &b->soup_pos,
-                            &b->soup_mat,    &b->data,  &b->winner};
+    DeviceBuffer* bufs[] = {&b->scene_table, &b->prims,  &b->tasks,  &b->counters, &b->soup_pos, &b->soup_mat,
+                            &b->data,        &b->winner, &b->tri_start, &b->counts, &b->offsets, &b->total,
+                            &b->scan_tmp};
     for (DeviceBuffer* p : bufs) release(*p);
     delete b;
     return VCT_OK;
