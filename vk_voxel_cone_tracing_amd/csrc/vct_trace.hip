@@ -563,6 +563,14 @@ __global__ __launch_bounds__(256) void k_trace(const typename Vol<kHalf>::T* __r
 // last, unused slots hold 0xffffffff. The order inside one roughness bin
 // depends on LDS atomic order, which changes no pixel's value.
 constexpr int kSpecTile = 64;
+
+// Tile-local pixel of index i in Morton (Z) order, so that pixels of one bin
+// fill 8 x 8 blocks in order and a wave's specular cones start close together.
+__device__ __forceinline__ int morton_x(int i)
+{
+    return (i & 1) | ((i >> 1) & 2) | ((i >> 2) & 4) | ((i >> 3) & 8) | ((i >> 4) & 16) | ((i >> 5) & 32);
+}
+__device__ __forceinline__ int morton_y(int i) { return morton_x(i >> 1); }
 constexpr int kSpecTilePixels = kSpecTile * kSpecTile;
 constexpr int kSpecBins = 257;
 
@@ -582,7 +590,7 @@ __global__ __launch_bounds__(256) void k_spec_bin(TraceParams P, const uint32_t*
 #pragma unroll
     for (int j = 0; j < kSpecTilePixels / 256; ++j) {
         const int local = j * 256 + tid;
-        const int px = tx0 + (local % kSpecTile), py = ty0 + (local / kSpecTile);
+        const int px = tx0 + morton_x(local), py = ty0 + morton_y(local);
         uint16_t k = 0xffffu;
         if (px < P.width && py < P.height) {
             const size_t idx = (size_t)py * (size_t)P.width + (size_t)px;
@@ -610,7 +618,7 @@ __global__ __launch_bounds__(256) void k_spec_bin(TraceParams P, const uint32_t*
     for (int j = 0; j < kSpecTilePixels / 256; ++j) {
         const int local = j * 256 + tid;
         if (key[j] == 0xffffu) continue;
-        const int px = tx0 + (local % kSpecTile), py = ty0 + (local / kSpecTile);
+        const int px = tx0 + morton_x(local), py = ty0 + morton_y(local);
         const uint32_t pos = atomicAdd(&cursor[key[j]], 1u);
         out[pos] = (uint32_t)py * (uint32_t)P.width + (uint32_t)px;
     }
