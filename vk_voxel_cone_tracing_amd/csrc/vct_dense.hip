@@ -192,6 +192,10 @@ vct_status vct_dense_destroy(vct_dense* g)
     hipDeviceSynchronize();
     release(g->data);
     release(g->winner);
+    release(g->bwd_rec);
+    release(g->bwd_hdr);
+    release(g->bwd_start);
+    release(g->bwd_ctr);
     if (g->scratch) vct_destroy(g->scratch);
     delete g;
     return VCT_OK;

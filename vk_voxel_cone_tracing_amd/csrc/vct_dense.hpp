@@ -13,5 +13,6 @@ struct vct_dense {
     size_t texels = 0;
     vct::DeviceBuffer data;
     vct::DeviceBuffer winner;
+    vct::DeviceBuffer bwd_rec, bwd_hdr, bwd_start, bwd_ctr;  // the trace adjoint's sweep records
     vct_ctx* scratch = nullptr;  // vertex stage / raster scratch (no clipmap volumes)
 };

@@ -11,6 +11,8 @@
 //   => front child a: 0.25 g.rgb, 0.25 g.a - 0.25 g . b;  back child b: 0.25 (1 - a.a) g.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "vct_dense.hpp"
 #include "vct_dense_march.hpp"
 #include "vct_internal.hpp"
@@ -365,62 +367,73 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One 16 x 16 pixel tile per workgroup; each wave (8 x 8 pixels) runs on its
-// own. Per cone: every lane replays its march forward keeping the per-step
-// state, then the wave sweeps the steps in reverse in lockstep (the step
-// sequence does not depend on the data, so every lane's step k is at the
-// same distance and level):
+// The march's step sequence (march() in vct_dense_march.hpp): it does not
+// depend on the data, only the exit on full occlusion does. Returns the
+// number of steps below max_dist, or -1 when there are more than kAdjMaxSteps.
+__device__ __forceinline__ int step_table(const Pyramid& P, const March& M, float* step_t, float* seg_t, float* lvl_t)
+{
+    const float vs = P.vs, inv_vs = 1.0f / vs, max_level = (float)(P.nlevels - 1);
+    float step = 0.0f, diameter = vs, seg = vs;
+    int cnt = 0;
+    while (step < M.max_dist && cnt < kAdjMaxSteps) {
+        step_t[cnt] = step;
+        seg_t[cnt] = seg;
+        lvl_t[cnt] = fminf(fmaxf(log2f(diameter * inv_vs), 0.0f), max_level);
+        ++cnt;
+        const float prev = step;
+        step += fmaxf(diameter, vs) * M.step_factor;
+        seg = step - prev;
+        diameter = step * M.coef;
+    }
+    return step < M.max_dist ? -1 : cnt;
+}
+
+// One (wave, cone) of the reverse sweep, handed from the sweep kernel to the
+// splat kernel: its sample gradients are records base + k * 64 + lane for
+// steps k < kc; `mask` holds the lanes that trace the cone and blo / bhi
+// bound their start points.
+struct SweepHeader {
+    uint32_t cone, kc, base;
+    int px0, py0;  // the wave's 8 x 8 pixel block
+    uint32_t pad;
+    unsigned long long mask;
+    float blo[3], bhi[3];
+};
+static_assert(sizeof(SweepHeader) == 56, "SweepHeader layout");
+
+// Sweep: one 16 x 16 pixel tile per workgroup, each wave (8 x 8 pixels) on
+// its own, no LDS beyond the step table. Per cone every lane replays its
+// march keeping the per-step state (two steps' samples fetched together, as
+// in the forward), then the wave runs the reverse recurrence in lockstep
+// (every lane's step k is at the same distance and level):
 //   op = 1 - (1 - s.a)^c,  b = 1 - R.a,  q = 1 + (t + vs_l) decay
 //   R.rgb += b c s.rgb,  R.a += b op,  O += (1 - O) op / q
 // gives  ds.rgb = gR.rgb b c,  d op = gR.a b + gO (1 - O) / q,
 //        ds.a = d op c (1 - s.a)^(c-1),
 //        gR.a <- gR.a (1 - op) - c gR.rgb . s.rgb,  gO <- gO (1 - op / q).
-// Each step's sample gradients are spread over their trilinear footprints
-// into an LDS window per level (lane-striped copies when the window is
-// small, so lanes sharing a few texels do not serialize), then flushed with
-// one add per touched texel, face and channel: into the tile's LDS
-// accumulators for the coarsest levels (written to global memory once, at
-// the end), into global memory with atomics for the others. Without kStore
-// the samples are fetched again in the reverse sweep instead of kept.
+// The sample gradients ds (zero where a lane's march has ended) go to one
+// record block per (wave, cone). Without kStore the samples are fetched
+// again in the reverse sweep instead of kept.
 template <bool kStore>
-__global__ __launch_bounds__(kBwdThreads) void k_dense_trace_bwd(const void* __restrict__ data, Pyramid P, March M,
-                                                                 const uint32_t* __restrict__ g_diffuse,
-                                                                 const uint16_t* __restrict__ g_normal,
-                                                                 const float* __restrict__ g_depth, int width,
-                                                                 int height, const float4* __restrict__ grad_out,
-                                                                 float* __restrict__ grad, uint32_t* __restrict__ status)
+__global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict__ data, Pyramid P, March M,
+                                                         const uint32_t* __restrict__ g_diffuse,
+                                                         const uint16_t* __restrict__ g_normal,
+                                                         const float* __restrict__ g_depth, int width, int height,
+                                                         const float4* __restrict__ grad_out, uint32_t tile0,
+                                                         uint32_t rec_cap, float4* __restrict__ starts,
+                                                         float4* __restrict__ rec,
+                                                         SweepHeader* __restrict__ hdr, uint32_t* __restrict__ ctr,
+                                                         uint32_t* __restrict__ status)
 {
-    __shared__ double s_win[4][2 * kPoolUnits];
-    __shared__ double s_coarse[kCoarseFloats];
     __shared__ float s_step[kAdjMaxSteps], s_seg[kAdjMaxSteps], s_lvl[kAdjMaxSteps];
     __shared__ int s_nsteps;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned tiles_x = ((unsigned)width + 15u) / 16u;
-    const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int px = (int)(tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const unsigned tile = tile0 + xcd_remap(blockIdx.x, gridDim.x);
+    const int px0 = (int)(tile % tiles_x) * 16 + (wave & 1) * 8, py0 = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8;
+    const int px = px0 + (lane & 7), py = py0 + (lane >> 3);
     const float vs = P.vs;
-    const Coarse C = coarse_layout(P);
-    double* win = s_win[wave];
-    for (int i = lane; i < 2 * kPoolUnits; i += 64) win[i] = 0.0;
-    for (int i = tid; i < kCoarseFloats; i += kBwdThreads) s_coarse[i] = 0.0;
-    if (tid == 0) {
-        // the march's step sequence (march() in vct_dense_march.hpp)
-        const float inv_vs = 1.0f / vs, max_level = (float)(P.nlevels - 1);
-        float step = 0.0f, diameter = vs, seg = vs;
-        int cnt = 0;
-        while (step < M.max_dist && cnt < kAdjMaxSteps) {
-            s_step[cnt] = step;
-            s_seg[cnt] = seg;
-            s_lvl[cnt] = fminf(fmaxf(log2f(diameter * inv_vs), 0.0f), max_level);
-            ++cnt;
-            const float prev = step;
-            step += fmaxf(diameter, vs) * M.step_factor;
-            seg = step - prev;
-            diameter = step * M.coef;
-        }
-        s_nsteps = step < M.max_dist ? -1 : cnt;
-    }
+    if (tid == 0) s_nsteps = step_table(P, M, s_step, s_seg, s_lvl);
     bool on = px < width && py < height;
     size_t idx = 0;
     float start[3] = {0.0f, 0.0f, 0.0f}, normal[3] = {0.0f, 0.0f, 1.0f}, albedo[3] = {0.0f, 0.0f, 0.0f};
@@ -433,6 +446,7 @@ __global__ __launch_bounds__(kBwdThreads) void k_dense_trace_bwd(const void* __r
             pixel_setup(P, M, px, py, width, height, depth, g_normal + 4 * idx, g_diffuse[idx], start, normal, albedo);
             go = grad_out[idx];
         }
+        starts[idx] = make_float4(start[0], start[1], start[2], 0.0f);
     }
     __syncthreads();
     const int nsteps = s_nsteps;
@@ -440,79 +454,183 @@ __global__ __launch_bounds__(kBwdThreads) void k_dense_trace_bwd(const void* __r
         if (tid == 0) atomicOr(status, 1u);
         return;
     }
-    const bool wave_on = wave_max(on ? 1.0f : 0.0f) > 0.0f;
+    if (!(wave_max(on ? 1.0f : 0.0f) > 0.0f)) return;  // no surface pixel in this wave
     const float inv = 1.0f / (float)M.ncones;
-    const float lead = vs * M.start_offset * 0.5f;
-    for (int i = 0; wave_on && i < M.ncones; ++i) {
+    for (int i = 0; i < M.ncones; ++i) {
         const float* d = cone_dir(M, i);
         const float dir[3] = {d[0], d[1], d[2]};
         const int face[3] = {dir[0] > 0.0f ? 0 : 1, dir[1] > 0.0f ? 2 : 3, dir[2] > 0.0f ? 4 : 5};
         const float w[3] = {dir[0] * dir[0], dir[1] * dir[1], dir[2] * dir[2]};
         const float ct = normal[0] * dir[0] + normal[1] * dir[1] + normal[2] * dir[2];
         const bool lane_on = on && ct >= 0.0f;
-        // box of the start points of the lanes that trace this cone
-        float blo[3], bhi[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            blo[k] = wave_min(lane_on ? start[k] : INFINITY);
-            bhi[k] = wave_max(lane_on ? start[k] : -INFINITY);
-        }
         float cstart[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) cstart[k] = start[k] + dir[k] * vs * M.start_offset * 0.5f;
-        // forward replay
+        // forward replay, two steps' samples in flight
         float st_rw[kAdjMaxSteps], st_occ[kAdjMaxSteps];
         f4 smp[kStore ? kAdjMaxSteps : 1];
         int n = 0;
         if (lane_on) {
             f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
             float occlusion = 0.0f;
-            for (int k = 0; k < nsteps && occlusion < 1.0f; ++k) {
-                float pos[3];
+            for (int k = 0; k < nsteps && occlusion < 1.0f; k += 2) {
+                float pa[3], pb[3];
+                const bool two = k + 1 < nsteps;
 #pragma unroll
-                for (int a = 0; a < 3; ++a) pos[a] = cstart[a] + dir[a] * s_step[k];
-                const f4 s = level_sample<true>(data, P, pos, s_lvl[k], face, w);
+                for (int a = 0; a < 3; ++a) {
+                    pa[a] = cstart[a] + dir[a] * s_step[k];
+                    pb[a] = cstart[a] + dir[a] * s_step[two ? k + 1 : k];
+                }
+                const f4 sa = level_sample<true>(data, P, pa, s_lvl[k], face, w);
+                const f4 sb = level_sample<true>(data, P, pb, s_lvl[two ? k + 1 : k], face, w);
                 st_rw[k] = result.w;
                 st_occ[k] = occlusion;
-                if (kStore) smp[kStore ? k : 0] = s;
-                accumulate(result, occlusion, s, s_lvl[k], s_step[k], s_seg[k], vs, M.decay);
+                if (kStore) smp[kStore ? k : 0] = sa;
+                accumulate(result, occlusion, sa, s_lvl[k], s_step[k], s_seg[k], vs, M.decay);
                 n = k + 1;
+                if (!two || !(occlusion < 1.0f)) break;
+                st_rw[k + 1] = result.w;
+                st_occ[k + 1] = occlusion;
+                if (kStore) smp[kStore ? k + 1 : 0] = sb;
+                accumulate(result, occlusion, sb, s_lvl[k + 1], s_step[k + 1], s_seg[k + 1], vs, M.decay);
+                n = k + 2;
             }
         }
+        float blo[3], bhi[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            blo[k] = wave_min(lane_on ? start[k] : INFINITY);
+            bhi[k] = wave_max(lane_on ? start[k] : -INFINITY);
+        }
         const int kc = (int)wave_max((float)n);
+        if (kc == 0) continue;
+        const unsigned long long mask = __ballot(lane_on);
+        uint32_t base = 0, h = 0;
+        if (lane == 0) base = atomicAdd(&ctr[0], (uint32_t)kc * 64u);
+        base = __shfl(base, 0, 64);
+        if (base + (uint32_t)kc * 64u > rec_cap) {  // never with the host's sizing; checked all the same
+            if (lane == 0) atomicOr(status, 2u);
+            continue;
+        }
+        if (lane == 0) {
+            h = atomicAdd(&ctr[1], 1u);
+            SweepHeader H;
+            H.cone = (uint32_t)i;
+            H.kc = (uint32_t)kc;
+            H.base = base;
+            H.px0 = px0;
+            H.py0 = py0;
+            for (int k = 0; k < 3; ++k) {
+                H.blo[k] = blo[k];
+                H.bhi[k] = bhi[k];
+            }
+            H.pad = 0u;
+            H.mask = mask;
+            hdr[h] = H;
+        }
         f4 gR = {go.x * inv * albedo[0] * M.diffuse_intensity * ct, go.y * inv * albedo[1] * M.diffuse_intensity * ct,
                  go.z * inv * albedo[2] * M.diffuse_intensity * ct, 0.0f};
         float gO = -(go.w * inv * M.ao_factor * ct);
+        for (int k = kc - 1; k >= 0; --k) {
+            float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (k < n) {
+                const float stp = s_step[k], lvl = s_lvl[k];
+                f4 s;
+                if (kStore) {
+                    s = smp[kStore ? k : 0];
+                } else {
+                    float pos[3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) pos[a] = cstart[a] + dir[a] * stp;
+                    s = level_sample<true>(data, P, pos, lvl, face, w);
+                }
+                const float vsl = vs * exp2f(lvl);
+                const float c = s_seg[k] / vsl;
+                const float q = 1.0f + (stp + vsl) * M.decay;
+                const float base_a = 1.0f - s.w;
+                const float pc = powf(base_a, c);
+                const float op = 1.0f - pc;
+                const float b = 1.0f - st_rw[k];
+                const float g_op = gR.w * b + gO * (1.0f - st_occ[k]) / q;
+                out.x = gR.x * b * c;
+                out.y = gR.y * b * c;
+                out.z = gR.z * b * c;
+                out.w = base_a > 0.0f ? g_op * c * powf(base_a, c - 1.0f) : 0.0f;
+                const float dot = gR.x * s.x + gR.y * s.y + gR.z * s.z;
+                gR.w = gR.w * (1.0f - op) - c * dot;
+                gO = gO * (1.0f - op / q);
+            }
+            rec[base + (uint32_t)k * 64u + (uint32_t)lane] = out;
+        }
+    }
+}
+
+// Splat: resident workgroups whose waves pull sweep headers from a counter
+// until none is left (the LDS set-up and the coarse flush are paid once per
+// workgroup, not per header). Each step's sample
+// gradients are spread over their trilinear footprints into an LDS window
+// per level (lane-striped copies when the window is small, so lanes sharing
+// a few texels do not serialize), then flushed with one add per touched
+// texel, face and channel: into the workgroup's LDS accumulators for the
+// coarsest levels (written to global memory once, at the end), into global
+// memory with atomics for the others.
+__global__ __launch_bounds__(256) void k_dense_bwd_splat(Pyramid P, March M, const SweepHeader* __restrict__ hdr,
+                                                         const uint32_t* __restrict__ ctr,
+                                                         const float4* __restrict__ rec,
+                                                         const float4* __restrict__ starts, int width, int height,
+                                                         float* __restrict__ grad, uint32_t* __restrict__ next)
+{
+    __shared__ double s_win[4][2 * kPoolUnits];
+    __shared__ double s_coarse[kCoarseFloats];
+    __shared__ float s_step[kAdjMaxSteps], s_seg[kAdjMaxSteps], s_lvl[kAdjMaxSteps];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const Coarse C = coarse_layout(P);
+    double* win = s_win[wave];
+    for (int i = lane; i < 2 * kPoolUnits; i += 64) win[i] = 0.0;
+    for (int i = tid; i < kCoarseFloats; i += 256) s_coarse[i] = 0.0;
+    if (tid == 0) step_table(P, M, s_step, s_seg, s_lvl);
+    __syncthreads();
+    const uint32_t nh = ctr[1];
+    for (;;) {
+        uint32_t h = 0;
+        if (lane == 0) h = atomicAdd(next, 1u);
+        h = __shfl(h, 0, 64);
+        if (h >= nh) break;
+        const SweepHeader H = hdr[h];
+        const int px = H.px0 + (lane & 7), py = H.py0 + (lane >> 3);
+        float start[3] = {0.0f, 0.0f, 0.0f};
+        if (px < width && py < height) {
+            const float4 st = starts[(size_t)py * (size_t)width + (size_t)px];
+            start[0] = st.x;
+            start[1] = st.y;
+            start[2] = st.z;
+        }
+        const float* d = cone_dir(M, (int)H.cone);
+        const float dir[3] = {d[0], d[1], d[2]};
+        const int face[3] = {dir[0] > 0.0f ? 0 : 1, dir[1] > 0.0f ? 2 : 3, dir[2] > 0.0f ? 4 : 5};
+        const float w[3] = {dir[0] * dir[0], dir[1] * dir[1], dir[2] * dir[2]};
+        const float lead = P.vs * M.start_offset * 0.5f;
+        float cstart[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) cstart[k] = start[k] + dir[k] * P.vs * M.start_offset * 0.5f;
+        const float blo[3] = {H.blo[0], H.blo[1], H.blo[2]}, bhi[3] = {H.bhi[0], H.bhi[1], H.bhi[2]};
         // Level m's window lives while the sweep is on m's steps; the two
         // levels in use at a step are consecutive, so m & 1 names its slot.
         // Even levels take the wave's pool from the bottom, odd ones from the
         // top, each up to what the other live window leaves.
         int slot_m[2] = {-1, -1};
         Window slot_w[2];
+        const int kc = (int)H.kc;
+        float4 gnext = rec[H.base + (uint32_t)(kc - 1) * 64u + (uint32_t)lane];
         for (int k = kc - 1; k >= 0; --k) {
+            const float4 g4 = gnext;
+            if (k > 0) gnext = rec[H.base + (uint32_t)(k - 1) * 64u + (uint32_t)lane];
+            const f4 gs = {g4.x, g4.y, g4.z, g4.w};
+            const bool has = g4.x != 0.0f || g4.y != 0.0f || g4.z != 0.0f || g4.w != 0.0f;
             const float stp = s_step[k], lvl = s_lvl[k];
             float pos[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) pos[a] = cstart[a] + dir[a] * stp;
-            f4 gs = {0.0f, 0.0f, 0.0f, 0.0f};
-            const bool has = k < n;
-            if (has) {
-                const f4 s = kStore ? smp[kStore ? k : 0] : level_sample<true>(data, P, pos, lvl, face, w);
-                const float vsl = vs * exp2f(lvl);
-                const float c = s_seg[k] / vsl;
-                const float q = 1.0f + (stp + vsl) * M.decay;
-                const float base = 1.0f - s.w;
-                const float op = 1.0f - powf(base, c);
-                const float b = 1.0f - st_rw[k];
-                const float g_op = gR.w * b + gO * (1.0f - st_occ[k]) / q;
-                gs.x = gR.x * b * c;
-                gs.y = gR.y * b * c;
-                gs.z = gR.z * b * c;
-                gs.w = base > 0.0f ? g_op * c * powf(base, c - 1.0f) : 0.0f;
-                const float dot = gR.x * s.x + gR.y * s.y + gR.z * s.z;
-                gR.w = gR.w * (1.0f - op) - c * dot;
-                gO = gO * (1.0f - op / q);
-            }
             const int lo = (int)floorf(lvl), hi = (int)ceilf(lvl);
             const float fl = lvl - floorf(lvl);
             const int nlev = hi != lo ? 2 : 1;
@@ -531,8 +649,8 @@ __global__ __launch_bounds__(kBwdThreads) void k_dense_trace_bwd(const void* __r
                 while (k1 > 0 && (int)ceilf(s_lvl[k1 - 1]) >= m) --k1;
                 slot_m[sl] = m;
                 const int other = slot_m[sl ^ 1] >= 0 ? slot_w[sl ^ 1].used : 0;
-                slot_w[sl] = window_of(P, m, start, lane_on, blo, bhi, dir, lead + s_step[k1], lead + stp, 0,
-                                       kPoolUnits - other);
+                slot_w[sl] = window_of(P, m, start, ((H.mask >> lane) & 1ull) != 0ull, blo, bhi, dir,
+                                       lead + s_step[k1], lead + stp, 0, kPoolUnits - other);
                 if (sl) {
                     // the top of the pool, in the window's elements (4 floats or 2 doubles per unit)
                     const int shift = (slot_w[sl].f32 ? 4 : 2) * (kPoolUnits - slot_w[sl].used);
@@ -553,12 +671,12 @@ __global__ __launch_bounds__(kBwdThreads) void k_dense_trace_bwd(const void* __r
                 flush_window(grad, win, s_coarse, C, P, slot_m[sl], slot_w[sl], face, w, lane);
         wave_lds_sync();
     }
-    // the tile's coarse levels to global memory
+    // the workgroup's coarse levels to global memory
     __syncthreads();
     for (int m = C.first; m < P.nlevels; ++m) {
         const int n = P.n[m], n3 = n * n * n;
         const unsigned long long D = (unsigned long long)n + 2;
-        for (int e = tid; e < 24 * n3; e += kBwdThreads) {
+        for (int e = tid; e < 24 * n3; e += 256) {
             const float v = (float)s_coarse[C.base[m] + e];
             if (v == 0.0f) continue;
             const int t = e % n3, ch = (e / n3) & 3, f = e / (4 * n3);
@@ -568,6 +686,11 @@ __global__ __launch_bounds__(kBwdThreads) void k_dense_trace_bwd(const void* __r
             unsafeAtomicAdd(grad + 4ull * texel + ch, v);
         }
     }
+}
+
+__global__ void k_bwd_reset(uint32_t* __restrict__ ctr)
+{
+    if (threadIdx.x < 4) ctr[threadIdx.x] = 0u;
 }
 
 // Gradient of child level c = m - 1 from parent level m (see the header).
@@ -680,22 +803,49 @@ vct_status vct_dense_trace_backward(vct_dense* g, vct_stream stream, const vct_g
     dense::Pyramid P;
     dense::March M;
     VCT_TRY(dense::make_params(g, cam, d, &P, &M));
-    if (march_steps(P, M) + 1 > dense::kAdjMaxSteps)
+    const int steps = march_steps(P, M);
+    if (steps + 1 > dense::kAdjMaxSteps)
         return fail(VCT_ERR_UNSUPPORTED, "the march takes more than %d steps", dense::kAdjMaxSteps - 1);
     int coarse = 0;
     for (int m = 1; m < P.nlevels; ++m)
         if (P.n[m] <= dense::kCoarseN) coarse += 24 * P.n[m] * P.n[m] * P.n[m];
     if (coarse > dense::kCoarseFloats) return fail(VCT_ERR_UNSUPPORTED, "coarse levels exceed the tile accumulator");
-    dim3 grid(((gb->width + 15) / 16) * ((gb->height + 15) / 16));
+    // Records of one chunk of tiles: every (wave, cone) may take the whole march.
+    const uint32_t tiles = ((gb->width + 15) / 16) * ((gb->height + 15) / 16);
+    // (two steps of margin over the host's count of the march)
+    const size_t per_tile = (size_t)4 * (size_t)M.ncones * (size_t)(steps + 2) * 64 * sizeof(float4);
+    const size_t cap = (size_t)4 << 30;
+    const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(tiles, cap / per_tile));
+    VCT_TRY(ensure(g->bwd_rec, per_tile * chunk));
+    VCT_TRY(ensure(g->bwd_hdr, sizeof(dense::SweepHeader) * 4 * (size_t)M.ncones * chunk));
+    VCT_TRY(ensure(g->bwd_start, sizeof(float4) * (size_t)gb->width * gb->height));
+    VCT_TRY(ensure(g->bwd_ctr, 16));
     hipStream_t s = (hipStream_t)stream;
-    if (flags & VCT_ADJOINT_RECOMPUTE)
-        hipLaunchKernelGGL(dense::k_dense_trace_bwd<false>, grid, dim3(dense::kBwdThreads), 0, s, g->data.ptr, P, M, gb->diffuse,
-                           gb->normal, gb->depth, (int)gb->width, (int)gb->height, (const float4*)grad_out,
-                           grad_pyramid, status);
-    else
-        hipLaunchKernelGGL(dense::k_dense_trace_bwd<true>, grid, dim3(dense::kBwdThreads), 0, s, g->data.ptr, P, M, gb->diffuse,
-                           gb->normal, gb->depth, (int)gb->width, (int)gb->height, (const float4*)grad_out,
-                           grad_pyramid, status);
+    auto* ctr = (uint32_t*)g->bwd_ctr.ptr;
+    const uint32_t rec_cap = (uint32_t)(per_tile * chunk / sizeof(float4));
+    // the splat's resident grid: two workgroups per CU (its LDS allows two)
+    int dev = 0, cus = 0;
+    VCT_HIP(hipGetDevice(&dev));
+    VCT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t splat_grid = (uint32_t)std::max(1, 2 * cus);
+    for (uint32_t t0 = 0; t0 < tiles; t0 += chunk) {
+        const uint32_t nt = std::min(chunk, tiles - t0);
+        hipLaunchKernelGGL(dense::k_bwd_reset, dim3(1), dim3(64), 0, s, ctr);
+        if (flags & VCT_ADJOINT_RECOMPUTE)
+            hipLaunchKernelGGL(dense::k_dense_bwd_sweep<false>, dim3(nt), dim3(256), 0, s, g->data.ptr, P, M,
+                               gb->diffuse, gb->normal, gb->depth, (int)gb->width, (int)gb->height,
+                               (const float4*)grad_out, t0, rec_cap, (float4*)g->bwd_start.ptr, (float4*)g->bwd_rec.ptr,
+                               (dense::SweepHeader*)g->bwd_hdr.ptr, ctr, status);
+        else
+            hipLaunchKernelGGL(dense::k_dense_bwd_sweep<true>, dim3(nt), dim3(256), 0, s, g->data.ptr, P, M,
+                               gb->diffuse, gb->normal, gb->depth, (int)gb->width, (int)gb->height,
+                               (const float4*)grad_out, t0, rec_cap, (float4*)g->bwd_start.ptr, (float4*)g->bwd_rec.ptr,
+                               (dense::SweepHeader*)g->bwd_hdr.ptr, ctr, status);
+        hipLaunchKernelGGL(dense::k_dense_bwd_splat, dim3(splat_grid), dim3(256), 0, s, P, M,
+                           (const dense::SweepHeader*)g->bwd_hdr.ptr, (const uint32_t*)ctr,
+                           (const float4*)g->bwd_rec.ptr, (const float4*)g->bwd_start.ptr, (int)gb->width,
+                           (int)gb->height, grad_pyramid, ctr + 2);
+    }
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
