@@ -187,26 +187,33 @@ __device__ __forceinline__ Window window_of(const Pyramid& P, int m, const float
     return W;
 }
 
-// Per-tile accumulators of the coarsest levels (n <= kCoarseN): 6 faces x 4
-// channel planes of n^3 interior cells per level, from `coarse_base[m]`.
+// Per-workgroup accumulators of the coarsest levels (n <= kCoarseN, at most
+// three: n = 4, 2, 1): 6 faces x 4 channel planes of n^3 interior cells per
+// level, level first + r from offset b[r].
 struct Coarse {
-    int first;     // first level held (nlevels when none)
-    int base[kMaxLevels];
+    int first;  // first level held (nlevels when none)
+    int b0, b1, b2;
 };
 
 __device__ __forceinline__ Coarse coarse_layout(const Pyramid& P)
 {
     Coarse C;
     C.first = P.nlevels;
-    int off = 0;
-    for (int m = 0; m < P.nlevels; ++m) {
-        C.base[m] = off;
-        if (m >= 1 && P.n[m] <= kCoarseN) {
-            if (C.first == P.nlevels) C.first = m;
-            off += 24 * P.n[m] * P.n[m] * P.n[m];
+    for (int m = 1; m < P.nlevels; ++m)
+        if (P.n[m] <= kCoarseN) {
+            C.first = m;
+            break;
         }
-    }
+    C.b0 = 0;
+    C.b1 = C.first < P.nlevels ? 24 * P.n[C.first] * P.n[C.first] * P.n[C.first] : 0;
+    C.b2 = C.first + 1 < P.nlevels ? C.b1 + 24 * P.n[C.first + 1] * P.n[C.first + 1] * P.n[C.first + 1] : C.b1;
     return C;
+}
+
+__device__ __forceinline__ int coarse_base(const Coarse& C, int m)
+{
+    const int r = m - C.first;
+    return r == 0 ? C.b0 : (r == 1 ? C.b1 : C.b2);
 }
 
 // Adds (face weights) x v to channel ch of interior texel (x, y, z) of level
@@ -219,7 +226,7 @@ __device__ __forceinline__ void add_channel(float* __restrict__ grad, double* __
     if (m >= C.first) {
         const int n = P.n[m], n3 = n * n * n;
         const int t = ((z - 1) * n + (y - 1)) * n + (x - 1);
-        for (int a = 0; a < 3; ++a) atomicAdd(coarse + C.base[m] + (face[a] * 4 + ch) * n3 + t, (double)(w[a] * v));
+        for (int a = 0; a < 3; ++a) atomicAdd(coarse + coarse_base(C, m) + (face[a] * 4 + ch) * n3 + t, (double)(w[a] * v));
         return;
     }
     const unsigned long long D = (unsigned long long)P.n[m] + 2;
@@ -261,7 +268,15 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __
 {
     const Corner c = corner_of(P, m, pos);
     if (!c.valid) return;
-    const Box& B = W.part[W.parts == 2 && start[W.axis] >= W.mid ? 1 : 0];
+    const bool second = W.parts == 2 && start[W.axis] >= W.mid;
+    Box B;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        B.lo[k] = second ? W.part[1].lo[k] : W.part[0].lo[k];
+        B.d[k] = second ? W.part[1].d[k] : W.part[0].d[k];
+    }
+    B.vol = second ? W.part[1].vol : W.part[0].vol;
+    B.base = second ? W.part[1].base : W.part[0].base;
     const float g[4] = {gs.x * scale, gs.y * scale, gs.z * scale, gs.w * scale};
     const int px = lane & 7, py = lane >> 3;
     const int r = (px & 1) | ((py & 1) << 1) | ((((px >> 1) ^ (py >> 1)) & 1) << 2);
@@ -314,8 +329,10 @@ __device__ __forceinline__ void flush_window(float* __restrict__ grad, double* _
 {
     const int n = P.n[m], D = n + 2;
     const float ws = w[0] + w[1] + w[2];
-    for (int pp = 0; pp < W.parts; ++pp) {
-        const Box& B = W.part[pp];
+#pragma unroll
+    for (int pp = 0; pp < 2; ++pp) {
+        if (pp >= W.parts) break;
+        const Box& B = pp ? W.part[1] : W.part[0];
         const int d0 = B.d[0], d01 = B.d[0] * B.d[1];
         const float inv_d0 = 1.0f / (float)d0, inv_d01 = 1.0f / (float)d01;
         const unsigned long long org =
@@ -345,7 +362,7 @@ __device__ __forceinline__ void flush_window(float* __restrict__ grad, double* _
             if (m >= C.first) {
                 const int n3 = n * n * n;
                 const int tc = ((B.lo[2] + z - 1) * n + (B.lo[1] + y - 1)) * n + (B.lo[0] + x - 1);
-                for (int a = 0; a < 3; ++a) atomicAdd(coarse + C.base[m] + (face[a] * 4 + ch) * n3 + tc, (double)(w[a] * fv));
+                for (int a = 0; a < 3; ++a) atomicAdd(coarse + coarse_base(C, m) + (face[a] * 4 + ch) * n3 + tc, (double)(w[a] * fv));
                 continue;
             }
             const int off = 4 * ((z * D + y) * D + x) + ch;
@@ -618,8 +635,9 @@ __global__ __launch_bounds__(256) void k_dense_bwd_splat(Pyramid P, March M, con
         // levels in use at a step are consecutive, so m & 1 names its slot.
         // Even levels take the wave's pool from the bottom, odd ones from the
         // top, each up to what the other live window leaves.
-        int slot_m[2] = {-1, -1};
-        Window slot_w[2];
+        int m0 = -1, m1 = -1;  // the levels held by the even and the odd slot
+        Window w0, w1;
+        const bool lane_in = ((H.mask >> lane) & 1ull) != 0ull;
         const int kc = (int)H.kc;
         float4 gnext = rec[H.base + (uint32_t)(kc - 1) * 64u + (uint32_t)lane];
         for (int k = kc - 1; k >= 0; --k) {
@@ -634,41 +652,55 @@ __global__ __launch_bounds__(256) void k_dense_bwd_splat(Pyramid P, March M, con
             const int lo = (int)floorf(lvl), hi = (int)ceilf(lvl);
             const float fl = lvl - floorf(lvl);
             const int nlev = hi != lo ? 2 : 1;
-            const int ml[2] = {lo, hi};
-            const float sc[2] = {hi != lo ? 1.0f - fl : 1.0f, fl};
-            for (int j = 0; j < nlev; ++j) {
-                const int m = ml[j], sl = m & 1;
-                if (slot_m[sl] == m) continue;
-                if (slot_m[sl] >= 0 && slot_w[sl].lds) {
-                    wave_lds_sync();
-                    flush_window(grad, win, s_coarse, C, P, slot_m[sl], slot_w[sl], face, w, lane);
-                    wave_lds_sync();
-                }
+            const float sc_lo = hi != lo ? 1.0f - fl : 1.0f, sc_hi = fl;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (j >= nlev) break;
+                const int m = j ? hi : lo;
                 // steps k1..k that use level m (the level rises with k)
                 int k1 = k;
                 while (k1 > 0 && (int)ceilf(s_lvl[k1 - 1]) >= m) --k1;
-                slot_m[sl] = m;
-                const int other = slot_m[sl ^ 1] >= 0 ? slot_w[sl ^ 1].used : 0;
-                slot_w[sl] = window_of(P, m, start, ((H.mask >> lane) & 1ull) != 0ull, blo, bhi, dir,
-                                       lead + s_step[k1], lead + stp, 0, kPoolUnits - other);
-                if (sl) {
+                if (m & 1) {
+                    if (m1 == m) continue;
+                    if (m1 >= 0 && w1.lds) {
+                        wave_lds_sync();
+                        flush_window(grad, win, s_coarse, C, P, m1, w1, face, w, lane);
+                        wave_lds_sync();
+                    }
+                    w1 = window_of(P, m, start, lane_in, blo, bhi, dir, lead + s_step[k1], lead + stp, 0,
+                                   kPoolUnits - (m0 >= 0 ? w0.used : 0));
                     // the top of the pool, in the window's elements (4 floats or 2 doubles per unit)
-                    const int shift = (slot_w[sl].f32 ? 4 : 2) * (kPoolUnits - slot_w[sl].used);
-                    slot_w[sl].part[0].base += shift;
-                    slot_w[sl].part[1].base += shift;
+                    const int shift = (w1.f32 ? 4 : 2) * (kPoolUnits - w1.used);
+                    w1.part[0].base += shift;
+                    w1.part[1].base += shift;
+                    m1 = m;
+                } else {
+                    if (m0 == m) continue;
+                    if (m0 >= 0 && w0.lds) {
+                        wave_lds_sync();
+                        flush_window(grad, win, s_coarse, C, P, m0, w0, face, w, lane);
+                        wave_lds_sync();
+                    }
+                    w0 = window_of(P, m, start, lane_in, blo, bhi, dir, lead + s_step[k1], lead + stp, 0,
+                                   kPoolUnits - (m1 >= 0 ? w1.used : 0));
+                    m0 = m;
                 }
             }
             if (has) {
-                for (int j = 0; j < nlev; ++j)
-                    if (sc[j] != 0.0f)
-                        splat_level(grad, win, s_coarse, C, P, ml[j], slot_w[ml[j] & 1], sc[j], start, pos, gs, face,
-                                    w, lane);
+                if (lo & 1) {
+                    if (sc_lo != 0.0f) splat_level(grad, win, s_coarse, C, P, lo, w1, sc_lo, start, pos, gs, face, w, lane);
+                    if (nlev == 2 && sc_hi != 0.0f)
+                        splat_level(grad, win, s_coarse, C, P, hi, w0, sc_hi, start, pos, gs, face, w, lane);
+                } else {
+                    if (sc_lo != 0.0f) splat_level(grad, win, s_coarse, C, P, lo, w0, sc_lo, start, pos, gs, face, w, lane);
+                    if (nlev == 2 && sc_hi != 0.0f)
+                        splat_level(grad, win, s_coarse, C, P, hi, w1, sc_hi, start, pos, gs, face, w, lane);
+                }
             }
         }
         wave_lds_sync();
-        for (int sl = 0; sl < 2; ++sl)
-            if (slot_m[sl] >= 0 && slot_w[sl].lds)
-                flush_window(grad, win, s_coarse, C, P, slot_m[sl], slot_w[sl], face, w, lane);
+        if (m0 >= 0 && w0.lds) flush_window(grad, win, s_coarse, C, P, m0, w0, face, w, lane);
+        if (m1 >= 0 && w1.lds) flush_window(grad, win, s_coarse, C, P, m1, w1, face, w, lane);
         wave_lds_sync();
     }
     // the workgroup's coarse levels to global memory
@@ -677,7 +709,7 @@ __global__ __launch_bounds__(256) void k_dense_bwd_splat(Pyramid P, March M, con
         const int n = P.n[m], n3 = n * n * n;
         const unsigned long long D = (unsigned long long)n + 2;
         for (int e = tid; e < 24 * n3; e += 256) {
-            const float v = (float)s_coarse[C.base[m] + e];
+            const float v = (float)s_coarse[coarse_base(C, m) + e];
             if (v == 0.0f) continue;
             const int t = e % n3, ch = (e / n3) & 3, f = e / (4 * n3);
             const int x = t % n + 1, y = (t / n) % n + 1, z = t / (n * n) + 1;
