@@ -132,9 +132,9 @@ __device__ __forceinline__ Box box_of(const Pyramid& P, int m, const float blo[3
 // A wave's LDS window for one level: one box, or two when the wave's start
 // points split across the middle of their longest axis (a depth edge) and the
 // two boxes fit where the one does not. Layout in LDS: per part, `copies`
-// blocks of `vol` texels x 4 channels from the part's base (in elements:
-// doubles, or floats when f32); lds is false when nothing fits in `cap`
-// 16-byte units; `used` is the units it takes.
+// blocks of 4 channel planes of `vol` texels from the part's base (in
+// elements: doubles, or floats when f32); lds is false when nothing fits in
+// `cap` 16-byte units; `used` is the units it takes.
 struct Window {
     Box part[2];
     int parts, axis, copies, used;
@@ -239,21 +239,23 @@ __device__ __forceinline__ void add_channel(float* __restrict__ grad, double* __
         unsafeAtomicAdd(grad + 4ull * (idx + (unsigned long long)face[a] * P.block[m]) + ch, w[a] * v);
 }
 
-// Adds v to window element t (4 channels) in the window's precision.
-__device__ __forceinline__ void lds_add4(double* __restrict__ win, bool f32, int t, const float v[4])
+// Adds v to texel element t of a window whose channel planes are `plane`
+// elements apart, in the window's precision. Channel planes (not 4 channels
+// per texel) let one LDS instruction's 64 lanes spread over all banks.
+__device__ __forceinline__ void lds_add4(double* __restrict__ win, bool f32, int t, int plane, const float v[4])
 {
     if (f32) {
         float* f = (float*)win + t;
         atomicAdd(f, v[0]);
-        atomicAdd(f + 1, v[1]);
-        atomicAdd(f + 2, v[2]);
-        atomicAdd(f + 3, v[3]);
+        atomicAdd(f + plane, v[1]);
+        atomicAdd(f + 2 * plane, v[2]);
+        atomicAdd(f + 3 * plane, v[3]);
     } else {
         double* d = win + t;
         atomicAdd(d, (double)v[0]);
-        atomicAdd(d + 1, (double)v[1]);
-        atomicAdd(d + 2, (double)v[2]);
-        atomicAdd(d + 3, (double)v[3]);
+        atomicAdd(d + plane, (double)v[1]);
+        atomicAdd(d + 2 * plane, (double)v[2]);
+        atomicAdd(d + 3 * plane, (double)v[3]);
     }
 }
 
@@ -286,8 +288,8 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __
         const int d0 = B.d[0], d01 = B.d[0] * B.d[1];
         const int cp = 4 * (lane & (W.copies - 1)) * B.vol;
         const int bx = r & 1, by = (r >> 1) & 1, bz = r >> 2;
-        const int t0 = B.base + cp + 4 * ((rz + bz) * d01 + (ry + by) * d0 + rx + bx);
-        const int sx = bx ? -4 : 4, sy = by ? -4 * d0 : 4 * d0, sz = bz ? -4 * d01 : 4 * d01;
+        const int t0 = B.base + cp + (rz + bz) * d01 + (ry + by) * d0 + rx + bx;
+        const int sx = bx ? -1 : 1, sy = by ? -d0 : d0, sz = bz ? -d01 : d01;
         const float wx[2] = {bx ? c.f[0] : 1.0f - c.f[0], bx ? 1.0f - c.f[0] : c.f[0]};
         const float wy[2] = {by ? c.f[1] : 1.0f - c.f[1], by ? 1.0f - c.f[1] : c.f[1]};
         const float wz[2] = {bz ? c.f[2] : 1.0f - c.f[2], bz ? 1.0f - c.f[2] : c.f[2]};
@@ -296,7 +298,7 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __
             const int jx = j & 1, jy = (j >> 1) & 1, jz = j >> 2;
             const float k = wx[jx] * wy[jy] * wz[jz];
             const float v[4] = {k * g[0], k * g[1], k * g[2], k * g[3]};
-            lds_add4(win, W.f32, t0 + (jx ? sx : 0) + (jy ? sy : 0) + (jz ? sz : 0), v);
+            lds_add4(win, W.f32, t0 + (jx ? sx : 0) + (jy ? sy : 0) + (jz ? sz : 0), B.vol, v);
         }
         return;
     }
@@ -312,7 +314,7 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __
         const int ux = x - B.lo[0], uy = y - B.lo[1], uz = z - B.lo[2];
         if (W.lds && (unsigned)ux < (unsigned)B.d[0] && (unsigned)uy < (unsigned)B.d[1] &&
             (unsigned)uz < (unsigned)B.d[2])
-            lds_add4(win, W.f32, B.base + cp + 4 * ((uz * B.d[1] + uy) * B.d[0] + ux), v);
+            lds_add4(win, W.f32, B.base + cp + (uz * B.d[1] + uy) * B.d[0] + ux, B.vol, v);
         else
             for (int ch = 0; ch < 4; ++ch) add_channel(grad, coarse, C, P, m, x, y, z, ch, face, w, v[ch]);
     }
@@ -342,18 +344,18 @@ __device__ __forceinline__ void flush_window(float* __restrict__ grad, double* _
         for (int a = 0; a < 3; ++a) gf[a] = g0 + (m == 0 ? 0ull : 4ull * (unsigned long long)face[a] * P.block[m]);
         for (int e = lane; e < 4 * B.vol; e += 64) {
             double v = 0.0;
+            const int t = e >> 2, ch = e & 3;
             for (int cc = 0; cc < W.copies; ++cc) {
-                const int t = B.base + 4 * cc * B.vol + e;
+                const int li = B.base + (4 * cc + ch) * B.vol + t;
                 if (W.f32) {
-                    v += ((float*)win)[t];
-                    ((float*)win)[t] = 0.0f;
+                    v += ((float*)win)[li];
+                    ((float*)win)[li] = 0.0f;
                 } else {
-                    v += win[t];
-                    win[t] = 0.0;
+                    v += win[li];
+                    win[li] = 0.0;
                 }
             }
             if (v == 0.0) continue;
-            const int t = e >> 2, ch = e & 3;
             const int z = (int)(((float)t + 0.5f) * inv_d01);
             const int rem = t - z * d01;
             const int y = (int)(((float)rem + 0.5f) * inv_d0);
