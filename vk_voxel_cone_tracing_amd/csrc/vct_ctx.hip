@@ -67,8 +67,16 @@ void release(DeviceBuffer& b)
 
 vct_status upload_small(vct_ctx* ctx, hipStream_t s, DeviceBuffer& dst, const void* src, size_t bytes)
 {
-    VCT_TRY(ensure(dst, bytes));
     if (bytes == 0) return VCT_OK;
+    // The pinned block keeps the bytes of the last finished upload: when dst
+    // already holds exactly these bytes (an unchanged draw list, the common
+    // case from one call to the next) nothing is copied and nothing waits.
+    if (ctx->pinned_dst == &dst && ctx->pinned_valid == bytes && dst.ptr && dst.bytes >= bytes &&
+        std::memcmp(ctx->pinned, src, bytes) == 0)
+        return VCT_OK;
+    ctx->pinned_dst = nullptr;
+    ctx->pinned_valid = 0;
+    VCT_TRY(ensure(dst, bytes));
     if (bytes > ctx->pinned_bytes) {
         if (ctx->pinned) hipHostFree(ctx->pinned);
         ctx->pinned = nullptr;
@@ -76,12 +84,13 @@ vct_status upload_small(vct_ctx* ctx, hipStream_t s, DeviceBuffer& dst, const vo
         VCT_HIP(hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
         ctx->pinned_bytes = bytes;
     }
-    // The pinned block is reused by the next upload, so the copy must finish
-    // before this call returns.
-    VCT_HIP(hipStreamSynchronize(s));
+    // Every upload waits for its copy before returning, so no copy out of the
+    // pinned block is still in flight here.
     std::memcpy(ctx->pinned, src, bytes);
     VCT_HIP(hipMemcpyAsync(dst.ptr, ctx->pinned, bytes, hipMemcpyHostToDevice, s));
     VCT_HIP(hipStreamSynchronize(s));
+    ctx->pinned_dst = &dst;
+    ctx->pinned_valid = bytes;
     return VCT_OK;
 }
 

@@ -89,10 +89,13 @@ struct vct_ctx {
     vct::DeviceBuffer rec_key[2], rec_idx[2], rec_val, sort_tmp;
     vct::DeviceBuffer staging;
     vct::DeviceBuffer materials_flat;
-    vct::DeviceBuffer spec_perm;
-    bool keep_alpha[vct::kMaxLevels]{};  // set by vct_begin_radiance_level, used by the next injection  // roughness-sorted pixel order of the specular cones
+    vct::DeviceBuffer spec_perm;         // roughness-sorted pixel order of the specular cones
+    bool keep_alpha[vct::kMaxLevels]{};  // set by vct_begin_radiance_level, used by the next injection
+    uint32_t inject_cap = 0;             // injection record capacity (grow-only)
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    const vct::DeviceBuffer* pinned_dst = nullptr;  // buffer that holds the pinned block's first pinned_valid bytes
+    size_t pinned_valid = 0;
 
     vct::SvoState svo;
 };

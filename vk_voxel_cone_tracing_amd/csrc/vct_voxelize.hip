@@ -104,6 +104,8 @@ __global__ void k_total(const unsigned long long* __restrict__ counts, const uns
     total[1] = 0ull;  // record counter for K2
 }
 
+__global__ void k_reset_records(unsigned long long* __restrict__ total) { total[1] = 0ull; }
+
 __device__ __forceinline__ uint32_t find_tri(const unsigned long long* __restrict__ offsets, uint32_t n,
                                              unsigned long long wi)
 {
@@ -157,6 +159,7 @@ __global__ __launch_bounds__(256) void k_raster_inject(const float* __restrict__
 {
 #pragma clang fp contract(off)
     const unsigned long long n = total[0];
+    if (n > 0xffffffffull) return;  // record keys hold 32 bits of work index; the host reports it
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     const float box_lo[3] = {box_lo0, box_lo1, box_lo2};
     const float box_hi[3] = {box_hi0, box_hi1, box_hi2};
@@ -309,14 +312,6 @@ static vct_status expand(vct_ctx* c, hipStream_t s, const TriSoup& soup, const V
     return VCT_OK;
 }
 
-static int raster_grid(unsigned long long work)
-{
-    unsigned long long blocks = (work + 255) / 256;
-    if (blocks > 8192) blocks = 8192;  // 256 CUs x 32 waves of grid-stride work
-    if (blocks < 1) blocks = 1;
-    return (int)blocks;
-}
-
 }  // namespace vct
 
 using namespace vct;
@@ -384,28 +379,38 @@ vct_status vct_inject_radiance(vct_ctx* c, vct_stream stream, const vct_mesh* me
         box_lo[k] = (float)P.lo[k] * P.voxel_size;
         box_hi[k] = (float)(P.hi[k] + 1) * P.voxel_size;
     }
-    // The record buffer is sized from the work total: one host read-back.
+    // The raster writes into a grow-only record buffer and counts every record
+    // (those past the capacity are counted, not stored). One host read-back
+    // of the count sizes the sort; a count above the capacity grows the
+    // buffer and runs the raster again, which happens while the capacity
+    // warms up. Record keys carry the work index, so the sorted order does
+    // not depend on which run stored them.
     unsigned long long tot[2] = {0, 0};
-    VCT_HIP(hipMemcpyAsync(tot, c->totals.ptr, 16, hipMemcpyDeviceToHost, s));
-    VCT_HIP(hipStreamSynchronize(s));
-    if (tot[0] == 0) return VCT_OK;
-    if (tot[0] > 0xffffffffull) return fail(VCT_ERR_CAPACITY, "raster work %llu exceeds 2^32", tot[0]);
-    const unsigned long long cap64 = tot[0] * 6ull;
-    const uint32_t cap = cap64 > 0xfffffff0ull ? 0xfffffff0u : (uint32_t)cap64;
-    for (int b = 0; b < 2; ++b) {
-        VCT_TRY(ensure(c->rec_key[b], 8u * (size_t)cap));
-        VCT_TRY(ensure(c->rec_idx[b], 4u * (size_t)cap));
+    for (int attempt = 0;; ++attempt) {
+        const uint32_t cap = c->inject_cap;
+        hipLaunchKernelGGL(k_raster_inject, dim3(8192), dim3(256), 0, s, soup.pos, soup.nrm, soup.mat,
+                           mesh->materials, (const unsigned long long*)c->offsets.ptr, soup.count,
+                           (unsigned long long*)c->totals.ptr, P, L, box_lo[0], box_lo[1], box_lo[2], box_hi[0],
+                           box_hi[1], box_hi[2], (unsigned long long*)c->rec_key[0].ptr, (uint32_t*)c->rec_idx[0].ptr,
+                           (Record*)c->rec_val.ptr, cap);
+        VCT_HIP(hipGetLastError());
+        VCT_HIP(hipMemcpyAsync(tot, c->totals.ptr, 16, hipMemcpyDeviceToHost, s));
+        VCT_HIP(hipStreamSynchronize(s));
+        if (tot[0] > 0xffffffffull) return fail(VCT_ERR_CAPACITY, "raster work %llu exceeds 2^32", tot[0]);
+        if (tot[1] <= cap) break;
+        if (tot[1] > 0xfffffff0ull || attempt > 0)
+            return fail(VCT_ERR_CAPACITY, "injection records overflow (%llu > %u)", tot[1], cap);
+        const unsigned long long want = tot[1] + tot[1] / 4 + 1024;
+        const uint32_t grown = want > 0xfffffff0ull ? 0xfffffff0u : (uint32_t)want;
+        for (int b = 0; b < 2; ++b) {
+            VCT_TRY(ensure(c->rec_key[b], 8u * (size_t)grown));
+            VCT_TRY(ensure(c->rec_idx[b], 4u * (size_t)grown));
+        }
+        VCT_TRY(ensure(c->rec_val, sizeof(Record) * (size_t)grown));
+        c->inject_cap = grown;
+        hipLaunchKernelGGL(k_reset_records, dim3(1), dim3(1), 0, s, (unsigned long long*)c->totals.ptr);
+        VCT_HIP(hipGetLastError());
     }
-    VCT_TRY(ensure(c->rec_val, sizeof(Record) * (size_t)cap));
-    hipLaunchKernelGGL(k_raster_inject, dim3(raster_grid(tot[0])), dim3(256), 0, s, soup.pos, soup.nrm, soup.mat,
-                       mesh->materials, (const unsigned long long*)c->offsets.ptr, soup.count,
-                       (unsigned long long*)c->totals.ptr, P, L, box_lo[0], box_lo[1], box_lo[2], box_hi[0],
-                       box_hi[1], box_hi[2], (unsigned long long*)c->rec_key[0].ptr, (uint32_t*)c->rec_idx[0].ptr,
-                       (Record*)c->rec_val.ptr, cap);
-    VCT_HIP(hipGetLastError());
-    VCT_HIP(hipMemcpyAsync(tot, c->totals.ptr, 16, hipMemcpyDeviceToHost, s));
-    VCT_HIP(hipStreamSynchronize(s));
-    if (tot[1] > cap) return fail(VCT_ERR_CAPACITY, "injection records overflow (%llu > %u)", tot[1], cap);
     const uint32_t nrec = (uint32_t)tot[1];
     if (!nrec) return VCT_OK;
     int end_bit = 32;
