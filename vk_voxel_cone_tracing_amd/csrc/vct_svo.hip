@@ -193,6 +193,14 @@ __global__ __launch_bounds__(256) void k_svo_flag(uint2* __restrict__ pool, cons
     uint32_t p[3], r;
     frag_pos(frags[f], half, p);
     const uint32_t idx = walk(pool, p, R, 1u, r);
+    // Setting a set bit changes nothing, so the pool ends the same when the
+    // lanes of a wave that reach the node of the wave's first active lane
+    // leave it to that lane, and when a flag already visible is not set
+    // again. Near the root nearly every fragment reaches the same node, and
+    // atomics on one address serialize.
+    const uint32_t lane = __lane_id();
+    if (idx == __builtin_amdgcn_readfirstlane(idx) && lane != __builtin_amdgcn_readfirstlane(lane)) return;
+    if (pool[idx].x & 0x80000000u) return;
     atomicOr(&pool[idx].x, 0x80000000u);
 }
 
