@@ -58,7 +58,7 @@ __device__ __forceinline__ float shadow_tap(const LightParams& L, float s, float
 }
 
 // shadow.glsl:28-36 calcVisibility + PCF16 (:14-26)
-__device__ __noinline__ float visibility(const LightParams& L, const float pos[3], uint32_t quirks)
+__device__ __forceinline__ float visibility(const LightParams& L, const float pos[3], uint32_t quirks)
 {
 #pragma clang fp contract(off)
     const float* V = L.view;
@@ -78,9 +78,19 @@ __device__ __noinline__ float visibility(const LightParams& L, const float pos[3
     xy[1] = xy[1] * 0.5f + 0.5f;
     const float sx = 1.0f / (float)L.width;
     const float sy = 1.0f / (float)L.height;
+    // offsets -1.5, -0.5, 0.5, 1.5 (exact in float), y outer as in the
+    // reference; integer counters let the 16 taps unroll so their fetches
+    // are in flight together
     float sum = 0.0f;
-    for (float y = -1.5f; y <= 1.5f; y += 1.0f)
-        for (float x = -1.5f; x <= 1.5f; x += 1.0f) sum = sum + shadow_tap(L, xy[0] + x * sx, xy[1] + y * sy, compare, ref);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float y = -1.5f + (float)j;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float x = -1.5f + (float)i;
+            sum = sum + shadow_tap(L, xy[0] + x * sx, xy[1] + y * sy, compare, ref);
+        }
+    }
     return sum * 0.0625f;
 }
 
