@@ -371,13 +371,11 @@ __global__ __launch_bounds__(256) void k_phase_raster_flat(const SceneDev* __res
 #pragma clang fp contract(off)
     const unsigned long long n = total[0];
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-    for (unsigned long long wi = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; wi < n; wi += stride) {
-        uint32_t lo = 0, hi = ntris;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (offsets[mid] <= wi) lo = mid; else hi = mid;
-        }
-        const uint32_t g = lo;
+    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < n; base += stride) {
+        const TriRange tr = block_triangles(offsets, ntris, base, min(base + blockDim.x - 1, n - 1));
+        const unsigned long long wi = base + threadIdx.x;
+        if (wi >= n) continue;
+        const uint32_t g = work_triangle(offsets, tr.lo, tr.hi, wi);
         const uint32_t sc = scene_of(tri_start, nscenes, g);
         const SceneDev& S = scenes[sc];
         const uint32_t t = g - tri_start[sc];

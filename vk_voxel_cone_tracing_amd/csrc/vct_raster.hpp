@@ -131,4 +131,31 @@ __device__ __forceinline__ bool tri_sample(const float* p, const TriSetup& s, fl
     return true;
 }
 
+// The triangle of work item wi: the last t in [lo, hi] with offsets[t] <= wi,
+// where offsets is the exclusive scan of the triangles' work counts
+// (nondecreasing) and offsets[lo] <= wi.
+__device__ __forceinline__ uint32_t work_triangle(const unsigned long long* __restrict__ offsets, uint32_t lo,
+                                                  uint32_t hi, unsigned long long wi)
+{
+    ++hi;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (offsets[mid] <= wi) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Bounds of the triangles of consecutive work items [first, last] of a block
+// (n >= 1 triangles): the triangles of the two ends, searched with
+// block-uniform values, so each lane then searches only between them.
+struct TriRange {
+    uint32_t lo, hi;
+};
+
+__device__ __forceinline__ TriRange block_triangles(const unsigned long long* __restrict__ offsets, uint32_t n,
+                                                    unsigned long long first, unsigned long long last)
+{
+    return {work_triangle(offsets, 0u, n - 1u, first), work_triangle(offsets, 0u, n - 1u, last)};
+}
+
 }  // namespace vct

@@ -58,26 +58,17 @@ __global__ void k_svo_count(const float* __restrict__ npos, uint32_t ntris, SvoR
     counts[t] = n;
 }
 
-__device__ __forceinline__ uint32_t find_tri64(const unsigned long long* __restrict__ offsets, uint32_t n,
-                                               unsigned long long wi)
-{
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (offsets[mid] <= wi) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
 // covered[wi] = 1 for every covered sample: each is one fragment invocation
 // and consumes one fragment-list slot (voxelizer.frag:55).
 __global__ __launch_bounds__(256) void k_svo_cover(const float* __restrict__ npos,
                                                    const unsigned long long* __restrict__ offsets, uint32_t ntris,
                                                    uint32_t nwork, SvoRaster R, uint32_t* __restrict__ covered)
 {
-    const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t base = blockIdx.x * blockDim.x;
+    const TriRange tr = block_triangles(offsets, ntris, base, min(base + blockDim.x, nwork) - 1u);
+    const uint32_t wi = base + threadIdx.x;
     if (wi >= nwork) return;
-    const uint32_t t = find_tri64(offsets, ntris, wi);
+    const uint32_t t = work_triangle(offsets, tr.lo, tr.hi, wi);
     float p[9];
     for (int k = 0; k < 9; ++k) p[k] = npos[9u * t + k];
     TriSetup s;
@@ -90,19 +81,30 @@ __global__ __launch_bounds__(256) void k_svo_cover(const float* __restrict__ npo
     covered[wi] = c;
 }
 
-// voxelizer.frag:48-100 for every covered sample, written at its scan slot.
+// The work index of every fragment: covered sample wi is fragment slot[wi].
+__global__ void k_svo_compact(const uint32_t* __restrict__ covered, const uint32_t* __restrict__ slot, uint32_t nwork,
+                              uint32_t* __restrict__ work_of)
+{
+    const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wi < nwork && covered[wi]) work_of[slot[wi]] = wi;
+}
+
+// voxelizer.frag:48-100, one thread per fragment (so every lane of a wave
+// has a fragment to light), in fragment order.
 __global__ __launch_bounds__(256) void k_svo_write(const float* __restrict__ npos, const float* __restrict__ nrm,
                                                    const uint32_t* __restrict__ mat,
                                                    const vct_gltf_material* __restrict__ mats,
                                                    const unsigned long long* __restrict__ offsets, uint32_t ntris,
-                                                   uint32_t nwork, SvoRaster R, LightParams L,
-                                                   const uint32_t* __restrict__ covered,
-                                                   const uint32_t* __restrict__ slot, uint2* __restrict__ frags)
+                                                   uint32_t nfrag, SvoRaster R, LightParams L,
+                                                   const uint32_t* __restrict__ work_of, uint2* __restrict__ frags)
 {
 #pragma clang fp contract(off)
-    const uint32_t wi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (wi >= nwork || !covered[wi]) return;
-    const uint32_t t = find_tri64(offsets, ntris, wi);
+    const uint32_t f0 = blockIdx.x * blockDim.x;
+    const TriRange tr = block_triangles(offsets, ntris, work_of[f0], work_of[min(f0 + blockDim.x, nfrag) - 1u]);
+    const uint32_t cur = f0 + threadIdx.x;
+    if (cur >= nfrag) return;
+    const uint32_t wi = work_of[cur];
+    const uint32_t t = work_triangle(offsets, tr.lo, tr.hi, wi);
     float p[9];
     for (int k = 0; k < 9; ++k) p[k] = npos[9u * t + k];
     TriSetup s;
@@ -110,7 +112,6 @@ __global__ __launch_bounds__(256) void k_svo_write(const float* __restrict__ npo
     float fp[3];
     double lam[3];
     if (!tri_sample(p, s, R.vs, R.samples, (uint32_t)(wi - offsets[t]), fp, lam)) return;
-    const uint32_t cur = slot[wi];
     float q[3];
     for (int k = 0; k < 3; ++k) q[k] = (fp[k] + 1.0f) * 0.5f;  // biasAndScale (voxelizer.geom:27-30)
     const vct_gltf_material& m = mats[mat[t]];
@@ -397,9 +398,12 @@ vct_status vct_svo_voxelize(vct_ctx* c, vct_stream stream, const vct_mesh* mesh,
         hipLaunchKernelGGL(k_fill_u2, dim3(blocks_for(nfrag)), dim3(256), 0, s, (uint2*)c->svo.frags.ptr, nfrag,
                            make_uint2(0u, 0u));
         const LightParams L = make_light(shadow, light, light_shadow);
-        hipLaunchKernelGGL(k_svo_write, dim3(blocks_for(nwork)), dim3(256), 0, s, npos, soup.nrm, soup.mat,
-                           mesh->materials, (const unsigned long long*)c->offsets.ptr, soup.count, nwork, Rr, L,
-                           covered, slot, (uint2*)c->svo.frags.ptr);
+        VCT_TRY(ensure(c->rec_key[0], 4u * (size_t)nfrag));
+        uint32_t* work_of = (uint32_t*)c->rec_key[0].ptr;
+        hipLaunchKernelGGL(k_svo_compact, dim3(blocks_for(nwork)), dim3(256), 0, s, covered, slot, nwork, work_of);
+        hipLaunchKernelGGL(k_svo_write, dim3(blocks_for(nfrag)), dim3(256), 0, s, npos, soup.nrm, soup.mat,
+                           mesh->materials, (const unsigned long long*)c->offsets.ptr, soup.count, nfrag, Rr, L,
+                           work_of, (uint2*)c->svo.frags.ptr);
         VCT_HIP(hipGetLastError());
     }
     c->svo.nfrag = nfrag;

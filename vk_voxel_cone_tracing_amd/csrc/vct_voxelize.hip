@@ -106,17 +106,6 @@ __global__ void k_total(const unsigned long long* __restrict__ counts, const uns
 
 __global__ void k_reset_records(unsigned long long* __restrict__ total) { total[1] = 0ull; }
 
-__device__ __forceinline__ uint32_t find_tri(const unsigned long long* __restrict__ offsets, uint32_t n,
-                                             unsigned long long wi)
-{
-    uint32_t lo = 0, hi = n;  // last t with offsets[t] <= wi
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (offsets[mid] <= wi) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
 __global__ __launch_bounds__(256) void k_raster_opacity(const float* __restrict__ pos,
                                                         const unsigned long long* __restrict__ offsets, uint32_t ntris,
                                                         const unsigned long long* __restrict__ total, VoxParams P,
@@ -125,7 +114,9 @@ __global__ __launch_bounds__(256) void k_raster_opacity(const float* __restrict_
     const unsigned long long n = total[0];
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long wi = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; wi < n; wi += stride) {
-        const uint32_t t = find_tri(offsets, ntris, wi);
+        // a full search per lane: with tens of thousands of triangles it is
+        // cheaper than narrowing it with block-uniform searches first
+        const uint32_t t = work_triangle(offsets, 0u, ntris - 1u, wi);
         float p[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) p[k] = pos[9u * t + k];
@@ -164,7 +155,9 @@ __global__ __launch_bounds__(256) void k_raster_inject(const float* __restrict__
     const float box_lo[3] = {box_lo0, box_lo1, box_lo2};
     const float box_hi[3] = {box_hi0, box_hi1, box_hi2};
     for (unsigned long long wi = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; wi < n; wi += stride) {
-        const uint32_t t = find_tri(offsets, ntris, wi);
+        // a full search per lane: with tens of thousands of triangles it is
+        // cheaper than narrowing it with block-uniform searches first
+        const uint32_t t = work_triangle(offsets, 0u, ntris - 1u, wi);
         float p[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) p[k] = pos[9u * t + k];
@@ -216,9 +209,9 @@ __global__ __launch_bounds__(256) void k_raster_inject(const float* __restrict__
             }
             nout = 3;
         }
-        const unsigned long long base = atomicAdd(&total[1], (unsigned long long)nout);
+        const unsigned long long first = atomicAdd(&total[1], (unsigned long long)nout);
         for (int j = 0; j < nout; ++j) {
-            const unsigned long long r = base + (unsigned long long)j;
+            const unsigned long long r = first + (unsigned long long)j;
             if (r >= cap) break;  // capacity is checked on the host from total[1]
             const size_t addr = vol_index(P.D, P.clip_level, faces[j], ic[0], ic[1], ic[2]);
             keys[r] = ((unsigned long long)addr << 32) | (wi & 0xffffffffull);
