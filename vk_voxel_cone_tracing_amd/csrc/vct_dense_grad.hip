@@ -73,10 +73,12 @@ __device__ __forceinline__ Corner corner_of(const Pyramid& P, int m, const float
     return c;
 }
 
-// LDS accumulation is in double: on gfx950 ds_add_f64 runs about as fast as
-// the integer LDS atomics while ds_add_f32 is more than twenty times slower.
-// A window that fits only at float width accumulates in float, still far
-// cheaper than scattered global atomics.
+// LDS windows accumulate in 32-bit fixed point: on gfx950 ds_add_u32 costs
+// about half of ds_add_f64 per wave instruction and ds_add_f32 is more than
+// twenty times slower. Each (wave, cone) carries the largest magnitude of its
+// sample gradients per channel, which bounds every window sum (see
+// fixed_scale); the per-workgroup accumulators of the coarse levels, shared
+// by many cones, stay in double.
 constexpr int kBwdThreads = 256;  // one 16 x 16 pixel tile, four waves of 8 x 8
 constexpr int kPoolUnits = 1024;  // a wave's LDS pool for its two live level windows, in 16 B units (16 KB)
 constexpr int kMaxCopies = 16;    // lane-striped copies of a small window
@@ -132,15 +134,27 @@ __device__ __forceinline__ Box box_of(const Pyramid& P, int m, const float blo[3
 // A wave's LDS window for one level: one box, or two when the wave's start
 // points split across the middle of their longest axis (a depth edge) and the
 // two boxes fit where the one does not. Layout in LDS: per part, `copies`
-// blocks of 4 channel planes of `vol` texels from the part's base (in
-// elements: doubles, or floats when f32); lds is false when nothing fits in
-// `cap` 16-byte units; `used` is the units it takes.
+// blocks of 4 channel planes of `vol` 32-bit fixed-point texels from the
+// part's base (in elements); lds is false when nothing fits in `cap` 16-byte
+// units; `used` is the units it takes; q[c] is channel c's fixed-point scale.
 struct Window {
     Box part[2];
     int parts, axis, copies, used;
     float mid;
-    bool lds, f32;
+    bool lds;
+    float q[4];
 };
+
+// Fixed-point scale for a window of a (wave, cone) whose sample gradients are
+// at most gmax in magnitude, over nsteps steps: a texel element takes at most
+// 64 lanes x nsteps contributions, each at most gmax (trilinear and level
+// weights are at most 1), so the sum of the scaled contributions stays below
+// 2^30 in magnitude. Contributions are truncated to integers: the error per
+// contribution is below one unit, gmax * 64 * nsteps / 2^30.
+__device__ __forceinline__ float fixed_scale(float gmax, int nsteps)
+{
+    return gmax > 1e-30f ? 1073741824.0f / (gmax * 64.0f * (float)nsteps) : 0.0f;
+}
 
 __device__ __forceinline__ Window window_of(const Pyramid& P, int m, const float start[3], bool lane_in,
                                             const float blo[3], const float bhi[3], const float dir[3], float ta,
@@ -176,12 +190,13 @@ __device__ __forceinline__ Window window_of(const Pyramid& P, int m, const float
         }
     }
     const int total = W.part[0].vol + W.part[1].vol;
-    W.f32 = 2 * total > cap;
     W.lds = total > 0 && total <= cap;
     int copies = kMaxCopies;
-    while (copies > 1 && (W.f32 ? 1 : 2) * copies * total > cap) copies >>= 1;
+    while (copies > 1 && copies * total > cap) copies >>= 1;
     W.copies = copies;
-    W.used = W.lds ? (W.f32 ? 1 : 2) * copies * total : 0;
+    W.used = W.lds ? copies * total : 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) W.q[c] = 0.0f;
     W.part[0].base = base;
     W.part[1].base = base + 4 * copies * W.part[0].vol;
     return W;
@@ -239,31 +254,22 @@ __device__ __forceinline__ void add_channel(float* __restrict__ grad, double* __
         unsafeAtomicAdd(grad + 4ull * (idx + (unsigned long long)face[a] * P.block[m]) + ch, w[a] * v);
 }
 
-// Adds v to texel element t of a window whose channel planes are `plane`
-// elements apart, in the window's precision. Channel planes (not 4 channels
-// per texel) let one LDS instruction's 64 lanes spread over all banks.
-__device__ __forceinline__ void lds_add4(double* __restrict__ win, bool f32, int t, int plane, const float v[4])
+// Adds v (in fixed-point units) to texel element t of a window whose channel
+// planes are `plane` elements apart. Channel planes (not 4 channels per
+// texel) let one LDS instruction's 64 lanes spread over all banks.
+__device__ __forceinline__ void lds_add4(int* __restrict__ win, int t, int plane, const float v[4])
 {
-    if (f32) {
-        float* f = (float*)win + t;
-        atomicAdd(f, v[0]);
-        atomicAdd(f + plane, v[1]);
-        atomicAdd(f + 2 * plane, v[2]);
-        atomicAdd(f + 3 * plane, v[3]);
-    } else {
-        double* d = win + t;
-        atomicAdd(d, (double)v[0]);
-        atomicAdd(d + plane, (double)v[1]);
-        atomicAdd(d + 2 * plane, (double)v[2]);
-        atomicAdd(d + 3 * plane, (double)v[3]);
-    }
+    atomicAdd(win + t, (int)v[0]);
+    atomicAdd(win + t + plane, (int)v[1]);
+    atomicAdd(win + t + 2 * plane, (int)v[2]);
+    atomicAdd(win + t + 3 * plane, (int)v[3]);
 }
 
 // Gradient of one step's samples on level m: into the wave's LDS window when
 // it has one, else straight to add_channel. A lane visits its 8 corners in
 // the order j ^ r with r set by its pixel position in the 8 x 8 wave, so the
 // lanes of one footprint rarely meet in the same cell in one LDS instruction.
-__device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __restrict__ win,
+__device__ __forceinline__ void splat_level(float* __restrict__ grad, int* __restrict__ win,
                                             double* __restrict__ coarse, const Coarse& C, const Pyramid& P, int m,
                                             const Window& W, float scale, const float start[3], const float pos[3],
                                             const f4& gs, const int face[3], const float w[3], int lane)
@@ -280,6 +286,7 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __
     B.vol = second ? W.part[1].vol : W.part[0].vol;
     B.base = second ? W.part[1].base : W.part[0].base;
     const float g[4] = {gs.x * scale, gs.y * scale, gs.z * scale, gs.w * scale};
+    const float gq[4] = {g[0] * W.q[0], g[1] * W.q[1], g[2] * W.q[2], g[3] * W.q[3]};
     const int px = lane & 7, py = lane >> 3;
     const int r = (px & 1) | ((py & 1) << 1) | ((((px >> 1) ^ (py >> 1)) & 1) << 2);
     const int rx = c.i[0] - B.lo[0], ry = c.i[1] - B.lo[1], rz = c.i[2] - B.lo[2];
@@ -297,8 +304,8 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __
         for (int j = 0; j < 8; ++j) {
             const int jx = j & 1, jy = (j >> 1) & 1, jz = j >> 2;
             const float k = wx[jx] * wy[jy] * wz[jz];
-            const float v[4] = {k * g[0], k * g[1], k * g[2], k * g[3]};
-            lds_add4(win, W.f32, t0 + (jx ? sx : 0) + (jy ? sy : 0) + (jz ? sz : 0), B.vol, v);
+            const float v[4] = {k * gq[0], k * gq[1], k * gq[2], k * gq[3]};
+            lds_add4(win, t0 + (jx ? sx : 0) + (jy ? sy : 0) + (jz ? sz : 0), B.vol, v);
         }
         return;
     }
@@ -310,13 +317,14 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __
         const int x = c.i[0] + qx, y = c.i[1] + qy, z = c.i[2] + qz;
         if (x < 1 || x > n || y < 1 || y > n || z < 1 || z > n) continue;
         const float k = (qx ? c.f[0] : 1.0f - c.f[0]) * (qy ? c.f[1] : 1.0f - c.f[1]) * (qz ? c.f[2] : 1.0f - c.f[2]);
-        const float v[4] = {k * g[0], k * g[1], k * g[2], k * g[3]};
         const int ux = x - B.lo[0], uy = y - B.lo[1], uz = z - B.lo[2];
         if (W.lds && (unsigned)ux < (unsigned)B.d[0] && (unsigned)uy < (unsigned)B.d[1] &&
-            (unsigned)uz < (unsigned)B.d[2])
-            lds_add4(win, W.f32, B.base + cp + (uz * B.d[1] + uy) * B.d[0] + ux, B.vol, v);
-        else
-            for (int ch = 0; ch < 4; ++ch) add_channel(grad, coarse, C, P, m, x, y, z, ch, face, w, v[ch]);
+            (unsigned)uz < (unsigned)B.d[2]) {
+            const float v[4] = {k * gq[0], k * gq[1], k * gq[2], k * gq[3]};
+            lds_add4(win, B.base + cp + (uz * B.d[1] + uy) * B.d[0] + ux, B.vol, v);
+        } else {
+            for (int ch = 0; ch < 4; ++ch) add_channel(grad, coarse, C, P, m, x, y, z, ch, face, w, k * g[ch]);
+        }
     }
 }
 
@@ -325,12 +333,15 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, double* __
 // entries l, l + 64, ...: consecutive lanes add to consecutive floats of a row
 // of the gradient pyramid. Texel coordinates come from float reciprocals,
 // exact here since a window holds at most 2^10 texels.
-__device__ __forceinline__ void flush_window(float* __restrict__ grad, double* __restrict__ win,
+__device__ __forceinline__ void flush_window(float* __restrict__ grad, int* __restrict__ win,
                                              double* __restrict__ coarse, const Coarse& C, const Pyramid& P, int m,
                                              const Window& W, const int face[3], const float w[3], int lane)
 {
     const int n = P.n[m], D = n + 2;
     const float ws = w[0] + w[1] + w[2];
+    float inv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) inv[c] = W.q[c] > 0.0f ? 1.0f / W.q[c] : 0.0f;
 #pragma unroll
     for (int pp = 0; pp < 2; ++pp) {
         if (pp >= W.parts) break;
@@ -343,24 +354,19 @@ __device__ __forceinline__ void flush_window(float* __restrict__ grad, double* _
         float* gf[3];
         for (int a = 0; a < 3; ++a) gf[a] = g0 + (m == 0 ? 0ull : 4ull * (unsigned long long)face[a] * P.block[m]);
         for (int e = lane; e < 4 * B.vol; e += 64) {
-            double v = 0.0;
+            int acc = 0;
             const int t = e >> 2, ch = e & 3;
             for (int cc = 0; cc < W.copies; ++cc) {
                 const int li = B.base + (4 * cc + ch) * B.vol + t;
-                if (W.f32) {
-                    v += ((float*)win)[li];
-                    ((float*)win)[li] = 0.0f;
-                } else {
-                    v += win[li];
-                    win[li] = 0.0;
-                }
+                acc += win[li];
+                win[li] = 0;
             }
-            if (v == 0.0) continue;
+            if (acc == 0) continue;
             const int z = (int)(((float)t + 0.5f) * inv_d01);
             const int rem = t - z * d01;
             const int y = (int)(((float)rem + 0.5f) * inv_d0);
             const int x = rem - y * d0;
-            const float fv = (float)v;
+            const float fv = (float)acc * (ch == 0 ? inv[0] : ch == 1 ? inv[1] : ch == 2 ? inv[2] : inv[3]);
             if (m >= C.first) {
                 const int n3 = n * n * n;
                 const int tc = ((B.lo[2] + z - 1) * n + (B.lo[1] + y - 1)) * n + (B.lo[0] + x - 1);
@@ -409,16 +415,18 @@ __device__ __forceinline__ int step_table(const Pyramid& P, const March& M, floa
 
 // One (wave, cone) of the reverse sweep, handed from the sweep kernel to the
 // splat kernel: its sample gradients are records base + k * 64 + lane for
-// steps k < kc; `mask` holds the lanes that trace the cone and blo / bhi
-// bound their start points.
+// steps k < kc; `mask` holds the lanes that trace the cone, blo / bhi bound
+// their start points and gmax[c] bounds the magnitude of channel c of every
+// record.
 struct SweepHeader {
     uint32_t cone, kc, base;
     int px0, py0;  // the wave's 8 x 8 pixel block
     uint32_t pad;
     unsigned long long mask;
     float blo[3], bhi[3];
+    float gmax[4];
 };
-static_assert(sizeof(SweepHeader) == 56, "SweepHeader layout");
+static_assert(sizeof(SweepHeader) == 72, "SweepHeader layout");
 
 // Sweep: one 16 x 16 pixel tile per workgroup, each wave (8 x 8 pixels) on
 // its own, no LDS beyond the step table. Per cone every lane replays its
@@ -545,11 +553,13 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
             }
             H.pad = 0u;
             H.mask = mask;
+            for (int c = 0; c < 4; ++c) H.gmax[c] = 0.0f;  // set after the reverse sweep
             hdr[h] = H;
         }
         f4 gR = {go.x * inv * albedo[0] * M.diffuse_intensity * ct, go.y * inv * albedo[1] * M.diffuse_intensity * ct,
                  go.z * inv * albedo[2] * M.diffuse_intensity * ct, 0.0f};
         float gO = -(go.w * inv * M.ao_factor * ct);
+        float amax[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int k = kc - 1; k >= 0; --k) {
             float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (k < n) {
@@ -579,7 +589,17 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
                 gR.w = gR.w * (1.0f - op) - c * dot;
                 gO = gO * (1.0f - op / q);
             }
+            amax[0] = fmaxf(amax[0], fabsf(out.x));
+            amax[1] = fmaxf(amax[1], fabsf(out.y));
+            amax[2] = fmaxf(amax[2], fabsf(out.z));
+            amax[3] = fmaxf(amax[3], fabsf(out.w));
             rec[base + (uint32_t)k * 64u + (uint32_t)lane] = out;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) amax[c] = wave_max(amax[c]);
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) hdr[h].gmax[c] = amax[c];
         }
     }
 }
@@ -604,8 +624,8 @@ __global__ __launch_bounds__(256) void k_dense_bwd_splat(Pyramid P, March M, con
     __shared__ float s_step[kAdjMaxSteps], s_seg[kAdjMaxSteps], s_lvl[kAdjMaxSteps];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Coarse C = coarse_layout(P);
-    double* win = s_win[wave];
-    for (int i = lane; i < 2 * kPoolUnits; i += 64) win[i] = 0.0;
+    int* win = (int*)s_win[wave];
+    for (int i = lane; i < 4 * kPoolUnits; i += 64) win[i] = 0;
     for (int i = tid; i < kCoarseFloats; i += 256) s_coarse[i] = 0.0;
     if (tid == 0) step_table(P, M, s_step, s_seg, s_lvl);
     __syncthreads();
@@ -671,10 +691,12 @@ __global__ __launch_bounds__(256) void k_dense_bwd_splat(Pyramid P, March M, con
                     }
                     w1 = window_of(P, m, start, lane_in, blo, bhi, dir, lead + s_step[k1], lead + stp, 0,
                                    kPoolUnits - (m0 >= 0 ? w0.used : 0));
-                    // the top of the pool, in the window's elements (4 floats or 2 doubles per unit)
-                    const int shift = (w1.f32 ? 4 : 2) * (kPoolUnits - w1.used);
+                    // the top of the pool, in elements (4 per 16-byte unit)
+                    const int shift = 4 * (kPoolUnits - w1.used);
                     w1.part[0].base += shift;
                     w1.part[1].base += shift;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) w1.q[c] = fixed_scale(H.gmax[c], k - k1 + 1);
                     m1 = m;
                 } else {
                     if (m0 == m) continue;
@@ -685,6 +707,8 @@ __global__ __launch_bounds__(256) void k_dense_bwd_splat(Pyramid P, March M, con
                     }
                     w0 = window_of(P, m, start, lane_in, blo, bhi, dir, lead + s_step[k1], lead + stp, 0,
                                    kPoolUnits - (m1 >= 0 ? w1.used : 0));
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) w0.q[c] = fixed_scale(H.gmax[c], k - k1 + 1);
                     m0 = m;
                 }
             }
