@@ -413,6 +413,36 @@ __device__ __forceinline__ int step_table(const Pyramid& P, const March& M, floa
     return step < M.max_dist ? -1 : cnt;
 }
 
+// Per-step constants of accumulate() (vct_dense_march.hpp), with the same
+// arithmetic: the opacity correction seg / vs_l, the occlusion divisor
+// 1 + (t + vs_l) decay and its reciprocal.
+__device__ __forceinline__ void step_coefs(const Pyramid& P, const March& M, int cnt, const float* step_t,
+                                           const float* seg_t, const float* lvl_t, float* corr_t, float* q_t,
+                                           float* iq_t)
+{
+    for (int k = 0; k < cnt; ++k) {
+        const float vsl = P.vs * exp2f(lvl_t[k]);
+        corr_t[k] = seg_t[k] / vsl;
+        q_t[k] = 1.0f + (step_t[k] + vsl) * M.decay;
+        iq_t[k] = 1.0f / q_t[k];
+    }
+}
+
+// accumulate() with the step's constants from the table; returns
+// (1 - s.a)^corr for the reverse sweep.
+__device__ __forceinline__ float replay_step(f4& result, float& occlusion, const f4& s, float corr, float q)
+{
+    const float pc = powf(1.0f - s.w, corr);
+    const float op = fminf(fmaxf(1.0f - pc, 0.0f), 1.0f);
+    const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
+    result.x += blend * (s.x * corr);
+    result.y += blend * (s.y * corr);
+    result.z += blend * (s.z * corr);
+    result.w += blend * op;
+    occlusion += (1.0f - occlusion) * op / q;
+    return pc;
+}
+
 // One (wave, cone) of the reverse sweep, handed from the sweep kernel to the
 // splat kernel: its sample gradients are records base + k * 64 + lane for
 // steps k < kc; `mask` holds the lanes that trace the cone, blo / bhi bound
@@ -453,6 +483,7 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
                                                          uint32_t* __restrict__ status)
 {
     __shared__ float s_step[kAdjMaxSteps], s_seg[kAdjMaxSteps], s_lvl[kAdjMaxSteps];
+    __shared__ float s_corr[kAdjMaxSteps], s_q[kAdjMaxSteps], s_iq[kAdjMaxSteps];
     __shared__ int s_nsteps;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned tiles_x = ((unsigned)width + 15u) / 16u;
@@ -460,7 +491,11 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
     const int px0 = (int)(tile % tiles_x) * 16 + (wave & 1) * 8, py0 = (int)(tile / tiles_x) * 16 + (wave >> 1) * 8;
     const int px = px0 + (lane & 7), py = py0 + (lane >> 3);
     const float vs = P.vs;
-    if (tid == 0) s_nsteps = step_table(P, M, s_step, s_seg, s_lvl);
+    if (tid == 0) {
+        const int cnt = step_table(P, M, s_step, s_seg, s_lvl);
+        s_nsteps = cnt;
+        step_coefs(P, M, cnt, s_step, s_seg, s_lvl, s_corr, s_q, s_iq);
+    }
     bool on = px < width && py < height;
     size_t idx = 0;
     float start[3] = {0.0f, 0.0f, 0.0f}, normal[3] = {0.0f, 0.0f, 1.0f}, albedo[3] = {0.0f, 0.0f, 0.0f};
@@ -494,7 +529,7 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
 #pragma unroll
         for (int k = 0; k < 3; ++k) cstart[k] = start[k] + dir[k] * vs * M.start_offset * 0.5f;
         // forward replay, two steps' samples in flight
-        float st_rw[kAdjMaxSteps], st_occ[kAdjMaxSteps];
+        float st_rw[kAdjMaxSteps], st_occ[kAdjMaxSteps], st_pc[kAdjMaxSteps];
         f4 smp[kStore ? kAdjMaxSteps : 1];
         int n = 0;
         if (lane_on) {
@@ -513,13 +548,13 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
                 st_rw[k] = result.w;
                 st_occ[k] = occlusion;
                 if (kStore) smp[kStore ? k : 0] = sa;
-                accumulate(result, occlusion, sa, s_lvl[k], s_step[k], s_seg[k], vs, M.decay);
+                st_pc[k] = replay_step(result, occlusion, sa, s_corr[k], s_q[k]);
                 n = k + 1;
                 if (!two || !(occlusion < 1.0f)) break;
                 st_rw[k + 1] = result.w;
                 st_occ[k + 1] = occlusion;
                 if (kStore) smp[kStore ? k + 1 : 0] = sb;
-                accumulate(result, occlusion, sb, s_lvl[k + 1], s_step[k + 1], s_seg[k + 1], vs, M.decay);
+                st_pc[k + 1] = replay_step(result, occlusion, sb, s_corr[k + 1], s_q[k + 1]);
                 n = k + 2;
             }
         }
@@ -573,21 +608,19 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
                     for (int a = 0; a < 3; ++a) pos[a] = cstart[a] + dir[a] * stp;
                     s = level_sample<true>(data, P, pos, lvl, face, w);
                 }
-                const float vsl = vs * exp2f(lvl);
-                const float c = s_seg[k] / vsl;
-                const float q = 1.0f + (stp + vsl) * M.decay;
+                const float c = s_corr[k], iq = s_iq[k];
                 const float base_a = 1.0f - s.w;
-                const float pc = powf(base_a, c);
+                const float pc = st_pc[k];  // base_a^c, from the replay
                 const float op = 1.0f - pc;
                 const float b = 1.0f - st_rw[k];
-                const float g_op = gR.w * b + gO * (1.0f - st_occ[k]) / q;
+                const float g_op = gR.w * b + gO * (1.0f - st_occ[k]) * iq;
                 out.x = gR.x * b * c;
                 out.y = gR.y * b * c;
                 out.z = gR.z * b * c;
-                out.w = base_a > 0.0f ? g_op * c * powf(base_a, c - 1.0f) : 0.0f;
+                out.w = base_a > 0.0f ? g_op * c * (pc / base_a) : 0.0f;  // d/da of 1 - base_a^c
                 const float dot = gR.x * s.x + gR.y * s.y + gR.z * s.z;
                 gR.w = gR.w * (1.0f - op) - c * dot;
-                gO = gO * (1.0f - op / q);
+                gO = gO * (1.0f - op * iq);
             }
             amax[0] = fmaxf(amax[0], fabsf(out.x));
             amax[1] = fmaxf(amax[1], fabsf(out.y));
