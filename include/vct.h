@@ -445,8 +445,10 @@ vct_status vct_dense_texels(vct_dense* grid, size_t* texels);
 /* albedo: res^3 x 3 floats, opacity: res^3 floats (device, x fastest). */
 vct_status vct_dense_set_level0(vct_dense* grid, vct_stream stream, const float* albedo, const float* opacity);
 /* grad_out: d loss / d out of vct_dense_trace (RGBA32F per pixel). `status`
- * (device uint32, zeroed by the caller) is set non-zero if a march outgrew
- * the per-lane state (only possible for step counts the call rejects). */
+ * (device uint32, zeroed by the caller) gets bit 0 if a march outgrew the
+ * per-lane state (only possible for step counts the call rejects) and bit 1
+ * if the sweep's gradient records would have outgrown their buffer (never
+ * with the library's sizing); the gradient is then incomplete. */
 vct_status vct_dense_trace_backward(vct_dense* grid, vct_stream stream, const vct_gbuffer* gbuffer,
                                     const vct_camera_ubo* camera, const vct_dense_trace_desc* desc,
                                     const float* grad_out, float* grad_pyramid, uint32_t flags, uint32_t* status);
@@ -457,13 +459,15 @@ vct_status vct_dense_level0_grad(vct_dense* grid, vct_stream stream, const float
 /* ---- batched dense method ------------------------------------------------
  * No reference counterpart (the reference renders one scene): many scenes of
  * uneven size, each voxelized into its own dense grid (vct_dense_voxelize's
- * rule), mipped (vct_dense_mipmap) and traced (vct_dense_trace) in ONE
- * persistent launch. The launch pulls tasks (vertex chunks, raster chunks,
- * mip bricks, pixel tiles) from a queue ordered by phase; a task waits on its
+ * rule), mipped (vct_dense_mipmap) and traced (vct_dense_trace), captured
+ * once in a hipGraph and replayed by vct_batch_run. Two schedules over one
+ * task table (vertex chunks, raster chunks, mip bricks, pixel tiles), chosen
+ * by vct_batch_tuning.workgroups_per_cu: 0 (the default) launches each phase
+ * over every scene as one grid; 1, 2 or 4 run ONE persistent launch whose
+ * workgroups pull tasks from a queue ordered by phase, a task waiting on its
  * scene's completion counter of the previous phase, so small scenes trace
- * while large ones still voxelize. The launch (counter reset + persistent
- * kernel) is captured once in a hipGraph and replayed by vct_batch_run.
- * Results are bit-identical to the per-scene vct_dense_* calls. */
+ * while large ones still voxelize. Results are bit-identical to the
+ * per-scene vct_dense_* calls under every schedule. */
 typedef struct vct_batch vct_batch;
 
 typedef struct vct_batch_scene {

@@ -210,7 +210,9 @@ public:
         auto go = torch::empty_like(opacity);
         check(vct_dense_level0_grad(slot.grid, s, gpyr.data_ptr<float>(), ga.data_ptr<float>(), go.data_ptr<float>()),
               "vct_dense_level0_grad");
-        TORCH_CHECK(status.item<int32_t>() == 0, "adjoint march outgrew its per-lane state");
+        const int32_t st = status.item<int32_t>();
+        TORCH_CHECK(st == 0, "vct_dense_trace_backward status ", st,
+                    " (bit 0: a march outgrew its per-lane state; bit 1: the gradient records outgrew their buffer)");
         return {ga, go, torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(),
                 torch::Tensor(), torch::Tensor()};
     }
