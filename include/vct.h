@@ -291,12 +291,17 @@ vct_status vct_make_voxelization_desc(const vct_ctx* ctx, uint32_t level,
 vct_status vct_clear_volume(vct_ctx* ctx, vct_stream stream, vct_volume volume);
 
 /* K1: opacity voxelization of one clip level (VoxelizationPass::onUpdate,
- * VoxelizationPass.cpp:190-212; Voxelizer::cmdVoxelize, Voxelizer.h:36). */
+ * VoxelizationPass.cpp:190-212; Voxelizer::cmdVoxelize, Voxelizer.h:36).
+ * Stream-ordered; the host waits only when the draw list (mesh->primitives)
+ * differs from the one the context uploaded last, for its upload. */
 vct_status vct_voxelize_opacity(vct_ctx* ctx, vct_stream stream, const vct_mesh* mesh,
                                 const vct_voxelization_desc* desc, uint32_t level);
 
 /* K2: radiance injection of one clip level (RadianceInjectionPass::onUpdate,
- * RadianceInjectionPass.cpp:134-159). */
+ * RadianceInjectionPass.cpp:134-159). The host waits once per call for the
+ * record count that sizes the sort (and for a changed draw list, as in
+ * vct_voxelize_opacity); while the context's record buffer grows, the raster
+ * runs a second time. */
 vct_status vct_inject_radiance(vct_ctx* ctx, vct_stream stream, const vct_mesh* mesh,
                                const vct_shadow_map* shadow,
                                const vct_directional_light_desc* light,
