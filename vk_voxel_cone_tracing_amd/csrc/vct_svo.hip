@@ -186,23 +186,50 @@ __device__ __forceinline__ uint32_t walk(const uint2* __restrict__ pool, uint32_
 }
 
 // octreeNodeFlag.comp: mark the deepest existing node of every fragment.
+// The pass at `depth` would walk from the root along the previous pass's
+// path and one step further (the node a fragment reached then was flagged,
+// so it has children now): node_of keeps each fragment's node between passes
+// and the pass takes that one step. The child slot at depth d is bit
+// L - 1 - d of the position (z | x << 1 | y << 2, as in walk()); a coordinate
+// of 2^L takes walk()'s path for 2^L - 1.
 __global__ __launch_bounds__(256) void k_svo_flag(uint2* __restrict__ pool, const uint2* __restrict__ frags,
-                                                  uint32_t nfrag, uint32_t R, bool half)
+                                                  uint32_t nfrag, uint32_t L, uint32_t depth, bool half,
+                                                  uint32_t* __restrict__ node_of)
 {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nfrag) return;
-    uint32_t p[3], r;
+    uint32_t p[3];
     frag_pos(frags[f], half, p);
-    const uint32_t idx = walk(pool, p, R, 1u, r);
+    const uint32_t top = (1u << L) - 1u, b = L - 1u - depth;
+    const uint32_t x = min(p[0], top), y = min(p[1], top), z = min(p[2], top);
+    const uint32_t slot = ((z >> b) & 1u) | (((x >> b) & 1u) << 1) | (((y >> b) & 1u) << 2);
+    uint32_t idx = slot;
+    if (depth > 0) {
+        const uint32_t parent = node_of[f];
+        const uint32_t cur = pool[parent].x & 0x7fffffffu;
+        idx = cur ? cur + slot : parent;  // walk() stops at a node without children
+    }
+    node_of[f] = idx;
     // Setting a set bit changes nothing, so the pool ends the same when the
-    // lanes of a wave that reach the node of the wave's first active lane
-    // leave it to that lane, and when a flag already visible is not set
-    // again. Near the root nearly every fragment reaches the same node, and
-    // atomics on one address serialize.
+    // lanes of a wave that reach one node leave it to one of them, and when a
+    // flag already visible is not set again. Near the root the fragments of a
+    // wave reach a few nodes (nearly all fragments reach one node at the
+    // first depth), and atomics on one address serialize: up to four of a
+    // wave's nodes are flagged by one lane each, the rest lane by lane.
     const uint32_t lane = __lane_id();
-    if (idx == __builtin_amdgcn_readfirstlane(idx) && lane != __builtin_amdgcn_readfirstlane(lane)) return;
-    if (pool[idx].x & 0x80000000u) return;
-    atomicOr(&pool[idx].x, 0x80000000u);
+    bool todo = true;
+#pragma unroll
+    for (int round = 0; round < 4; ++round) {
+        const unsigned long long left = __ballot(todo);
+        if (left == 0ull) break;
+        const int leader = __ffsll((long long)left) - 1;
+        const uint32_t node = __builtin_amdgcn_readlane(idx, leader);
+        if (todo && idx == node) {
+            todo = false;
+            if ((int)lane == leader && !(pool[idx].x & 0x80000000u)) atomicOr(&pool[idx].x, 0x80000000u);
+        }
+    }
+    if (todo && !(pool[idx].x & 0x80000000u)) atomicOr(&pool[idx].x, 0x80000000u);
 }
 
 // octreeNodeAlloc.comp with ordered allocation. Pass 1: per-wave flagged counts.
@@ -431,6 +458,8 @@ vct_status vct_svo_build(vct_ctx* c, vct_stream stream)
     std::vector<uint32_t> level_begin, level_num;
     VCT_TRY(ensure(c->counts, 8u * ((cap >> 6) + 8)));
     VCT_TRY(ensure(c->offsets, 8u * ((cap >> 6) + 8)));
+    VCT_TRY(ensure(c->rec_idx[1], 4u * (size_t)(nfrag ? nfrag : 1)));
+    uint32_t* node_of = (uint32_t*)c->rec_idx[1].ptr;  // each fragment's node between flag passes
     for (uint32_t i = 1; i <= L; ++i) {
         if ((uint64_t)begin + num > cap) return fail(VCT_ERR_CAPACITY, "octree node pool overflow (%llu nodes)", (unsigned long long)cap);
         level_begin.push_back(begin);
@@ -439,7 +468,8 @@ vct_status vct_svo_build(vct_ctx* c, vct_stream stream)
         hipLaunchKernelGGL(k_fill_u2, dim3(blocks_for(num)), dim3(256), 0, s, pool + begin, num, make_uint2(0u, 0u));
         // octreeNodeFlag.comp
         if (nfrag)
-            hipLaunchKernelGGL(k_svo_flag, dim3(blocks_for(nfrag)), dim3(256), 0, s, pool, frags, nfrag, R, half);
+            hipLaunchKernelGGL(k_svo_flag, dim3(blocks_for(nfrag)), dim3(256), 0, s, pool, frags, nfrag, L, i - 1u, half,
+                               node_of);
         if (i == L) break;
         // octreeNodeAlloc.comp + octreeNodeModifyArg.comp
         const uint32_t waves = (num + 63) / 64;
