@@ -257,33 +257,33 @@ __global__ __launch_bounds__(256) void k_svo_alloc_assign(uint2* __restrict__ po
     pool[begin + i] = make_uint2(((base + rank + 1u) << 3u) | 0x80000000u, 0u);
 }
 
-// octreeNodeLeafWrite.comp, ordered: one (leaf, fragment) key per fragment
-// that reaches a leaf; keys of other fragments sort last.
+// octreeNodeLeafWrite.comp, ordered: one key (leaf << fbits | fragment) per
+// fragment that reaches a leaf; fragments that do not take `none`, above
+// every such key, and sort last.
 __global__ __launch_bounds__(256) void k_svo_leaf_keys(const uint2* __restrict__ pool, const uint2* __restrict__ frags,
-                                                       uint32_t nfrag, uint32_t R, bool half,
-                                                       unsigned long long* __restrict__ keys,
-                                                       uint32_t* __restrict__ vals)
+                                                       uint32_t nfrag, uint32_t R, bool half, uint32_t fbits,
+                                                       unsigned long long none, unsigned long long* __restrict__ keys)
 {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nfrag) return;
     uint32_t p[3], r;
     frag_pos(frags[f], half, p);
     const uint32_t idx = walk(pool, p, R, 1u, r);
-    keys[f] = r == 1u ? (((unsigned long long)idx << 32) | f) : ~0ull;
-    vals[f] = f;
+    keys[f] = r == 1u ? (((unsigned long long)idx << fbits) | f) : none;
 }
 
 __global__ __launch_bounds__(256) void k_svo_leaf_apply(uint2* __restrict__ pool, const uint2* __restrict__ frags,
-                                                        const unsigned long long* __restrict__ keys,
-                                                        const uint32_t* __restrict__ vals, uint32_t n)
+                                                        const unsigned long long* __restrict__ keys, uint32_t n,
+                                                        uint32_t fbits, unsigned long long none)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || keys[i] == ~0ull) return;
-    const unsigned long long leaf = keys[i] >> 32;
-    if (i > 0 && (keys[i - 1] >> 32) == leaf) return;
+    if (i >= n || keys[i] == none) return;
+    const unsigned long long leaf = keys[i] >> fbits;
+    if (i > 0 && (keys[i - 1] >> fbits) == leaf) return;
+    const unsigned long long fmask = (1ull << fbits) - 1ull;
     uint32_t t = pool[leaf].y;
-    for (uint32_t j = i; j < n && keys[j] != ~0ull && (keys[j] >> 32) == leaf; ++j)
-        t = rgba8_avg(t, unpack_raw(frags[vals[j]].y & 0x0fffffffu));
+    for (uint32_t j = i; j < n && keys[j] != none && (keys[j] >> fbits) == leaf; ++j)
+        t = rgba8_avg(t, unpack_raw(frags[(uint32_t)(keys[j] & fmask)].y & 0x0fffffffu));
     pool[leaf].y = t;
 }
 
@@ -499,22 +499,26 @@ vct_status vct_svo_build(vct_ctx* c, vct_stream stream)
         for (int b = 0; b < 2; ++b) {
             VCT_TRY(ensure(c->rec_key[b], 8u * (size_t)nfrag));
         }
-        VCT_TRY(ensure(c->rec_idx[0], 4u * (size_t)nfrag));
-        VCT_TRY(ensure(c->rec_idx[1], 4u * (size_t)nfrag));
+        // key bits: fragment index below, leaf index above; a leaf index is
+        // below total - 1, so the all-ones `none` is above every real key
+        auto bits = [](unsigned long long v) {
+            int b = 0;
+            while (b < 64 && (v >> b) != 0ull) ++b;
+            return b;
+        };
+        const uint32_t fbits = (uint32_t)std::max(1, bits(nfrag - 1u));
+        const int end_bit = (int)fbits + bits(total);
+        const unsigned long long none = end_bit >= 64 ? ~0ull : (1ull << end_bit) - 1ull;
+        unsigned long long* k0 = (unsigned long long*)c->rec_key[0].ptr;
+        unsigned long long* k1 = (unsigned long long*)c->rec_key[1].ptr;
         hipLaunchKernelGGL(k_svo_leaf_keys, dim3(blocks_for(nfrag)), dim3(256), 0, s, pool, frags, nfrag, R, half,
-                           (unsigned long long*)c->rec_key[0].ptr, (uint32_t*)c->rec_idx[0].ptr);
+                           fbits, none, k0);
         size_t tmp = 0;
-        VCT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (unsigned long long*)c->rec_key[0].ptr,
-                                                   (unsigned long long*)c->rec_key[1].ptr,
-                                                   (uint32_t*)c->rec_idx[0].ptr, (uint32_t*)c->rec_idx[1].ptr,
-                                                   (int)nfrag, 0, 64, s));
+        VCT_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, k0, k1, (int)nfrag, 0, end_bit, s));
         VCT_TRY(ensure(c->sort_tmp, tmp));
-        VCT_HIP(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp.ptr, tmp, (unsigned long long*)c->rec_key[0].ptr,
-                                                   (unsigned long long*)c->rec_key[1].ptr,
-                                                   (uint32_t*)c->rec_idx[0].ptr, (uint32_t*)c->rec_idx[1].ptr,
-                                                   (int)nfrag, 0, 64, s));
-        hipLaunchKernelGGL(k_svo_leaf_apply, dim3(blocks_for(nfrag)), dim3(256), 0, s, pool, frags,
-                           (const unsigned long long*)c->rec_key[1].ptr, (const uint32_t*)c->rec_idx[1].ptr, nfrag);
+        VCT_HIP(hipcub::DeviceRadixSort::SortKeys(c->sort_tmp.ptr, tmp, k0, k1, (int)nfrag, 0, end_bit, s));
+        hipLaunchKernelGGL(k_svo_leaf_apply, dim3(blocks_for(nfrag)), dim3(256), 0, s, pool, frags, k1, nfrag, fbits,
+                           none);
     }
     // octreeNodeMipmapWrite.comp for i = 2..L (OctreeBuilder.cpp:328-344). The
     // target of pass i is the node whose descent resolution equals i (quirk:
