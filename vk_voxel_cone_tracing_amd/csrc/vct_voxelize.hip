@@ -106,6 +106,15 @@ __global__ void k_total(const unsigned long long* __restrict__ counts, const uns
 
 __global__ void k_reset_records(unsigned long long* __restrict__ total) { total[1] = 0ull; }
 
+// Bits of the work index in an injection record key: enough for n - 1 (n
+// work items). The key is (texel index within the level) << bits | index.
+__host__ __device__ inline uint32_t work_bits(unsigned long long n)
+{
+    uint32_t b = 1;
+    while (n > 1ull && b < 32 && ((n - 1ull) >> b) != 0ull) ++b;  // n <= 2^32 (checked)
+    return b;
+}
+
 __global__ __launch_bounds__(256) void k_raster_opacity(const float* __restrict__ pos,
                                                         const unsigned long long* __restrict__ offsets, uint32_t ntris,
                                                         const unsigned long long* __restrict__ total, VoxParams P,
@@ -151,6 +160,8 @@ __global__ __launch_bounds__(256) void k_raster_inject(const float* __restrict__
 #pragma clang fp contract(off)
     const unsigned long long n = total[0];
     if (n > 0xffffffffull) return;  // record keys hold 32 bits of work index; the host reports it
+    const uint32_t wbits = work_bits(n);
+    const size_t level_base = vol_index(P.D, P.clip_level, 0, 0, 0, 0);
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     const float box_lo[3] = {box_lo0, box_lo1, box_lo2};
     const float box_hi[3] = {box_hi0, box_hi1, box_hi2};
@@ -213,8 +224,8 @@ __global__ __launch_bounds__(256) void k_raster_inject(const float* __restrict__
         for (int j = 0; j < nout; ++j) {
             const unsigned long long r = first + (unsigned long long)j;
             if (r >= cap) break;  // capacity is checked on the host from total[1]
-            const size_t addr = vol_index(P.D, P.clip_level, faces[j], ic[0], ic[1], ic[2]);
-            keys[r] = ((unsigned long long)addr << 32) | (wi & 0xffffffffull);
+            const size_t addr = vol_index(P.D, P.clip_level, faces[j], ic[0], ic[1], ic[2]) - level_base;
+            keys[r] = ((unsigned long long)addr << wbits) | wi;
             idx[r] = (uint32_t)r;
             vals[r] = out[j];
         }
@@ -224,23 +235,25 @@ __global__ __launch_bounds__(256) void k_raster_inject(const float* __restrict__
 template <bool kHalf>
 __global__ __launch_bounds__(256) void k_apply(const unsigned long long* __restrict__ keys,
                                                const uint32_t* __restrict__ idx, const Record* __restrict__ vals,
-                                               uint32_t n, void* __restrict__ vol, bool keep_alpha)
+                                               uint32_t n, void* __restrict__ vol, bool keep_alpha, uint32_t wbits,
+                                               size_t level_base)
 {
 #pragma clang fp contract(off)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const unsigned long long addr = keys[i] >> 32;
-    if (i > 0 && (keys[i - 1] >> 32) == addr) return;  // not the head of its texel's run
+    const unsigned long long local = keys[i] >> wbits;
+    if (i > 0 && (keys[i - 1] >> wbits) == local) return;  // not the head of its texel's run
+    const unsigned long long addr = level_base + local;
     if (!kHalf) {
         uint32_t t = ((uint32_t*)vol)[addr];
-        for (uint32_t j = i; j < n && (keys[j] >> 32) == addr; ++j) {
+        for (uint32_t j = i; j < n && (keys[j] >> wbits) == local; ++j) {
             const Record v = vals[idx[j]];
             t = rgba8_avg(t, f4{v.r * 255.0f, v.g * 255.0f, v.b * 255.0f, 1.0f});
         }
         ((uint32_t*)vol)[addr] = t;
     } else {
         float sr = 0.0f, sg = 0.0f, sb = 0.0f, cnt = 0.0f;
-        for (uint32_t j = i; j < n && (keys[j] >> 32) == addr; ++j) {
+        for (uint32_t j = i; j < n && (keys[j] >> wbits) == local; ++j) {
             const Record v = vals[idx[j]];
             sr = sr + v.r;
             sg = sg + v.g;
@@ -406,8 +419,12 @@ vct_status vct_inject_radiance(vct_ctx* c, vct_stream stream, const vct_mesh* me
     }
     const uint32_t nrec = (uint32_t)tot[1];
     if (!nrec) return VCT_OK;
-    int end_bit = 32;
-    while ((1ull << (end_bit - 32)) < texels_per_volume(c)) ++end_bit;
+    // key bits: the texel within the level (6 faces x D^3) above the work index
+    const uint32_t wbits = work_bits(tot[0]);
+    const size_t level_texels = (size_t)kFaces * (size_t)c->D * (size_t)c->D * (size_t)c->D;
+    const size_t level_base = (size_t)level * level_texels;
+    int end_bit = (int)wbits;
+    while (end_bit < 64 && ((level_texels - 1) >> (end_bit - (int)wbits)) != 0) ++end_bit;
     size_t tmp = 0;
     VCT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (unsigned long long*)c->rec_key[0].ptr,
                                                (unsigned long long*)c->rec_key[1].ptr, (uint32_t*)c->rec_idx[0].ptr,
@@ -420,11 +437,11 @@ vct_status vct_inject_radiance(vct_ctx* c, vct_stream stream, const vct_mesh* me
     if (half)
         hipLaunchKernelGGL(k_apply<true>, dim3((nrec + 255) / 256), dim3(256), 0, s,
                            (const unsigned long long*)c->rec_key[1].ptr, (const uint32_t*)c->rec_idx[1].ptr,
-                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr, keep);
+                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr, keep, wbits, level_base);
     else
         hipLaunchKernelGGL(k_apply<false>, dim3((nrec + 255) / 256), dim3(256), 0, s,
                            (const unsigned long long*)c->rec_key[1].ptr, (const uint32_t*)c->rec_idx[1].ptr,
-                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr, false);
+                           (const Record*)c->rec_val.ptr, nrec, c->radiance.ptr, false, wbits, level_base);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
