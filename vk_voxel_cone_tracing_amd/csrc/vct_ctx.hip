@@ -310,6 +310,7 @@ vct_status vct_destroy(vct_ctx* c)
                             &c->svo.frags, &c->svo.pool, &c->svo.counters, &c->spec_perm};
     for (DeviceBuffer* b : bufs) release(*b);
     if (c->pinned) hipHostFree(c->pinned);
+    if (c->pinned_word) hipHostFree(c->pinned_word);
     delete c;
     return VCT_OK;
 }

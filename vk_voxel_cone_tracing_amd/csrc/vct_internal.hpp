@@ -96,6 +96,7 @@ struct vct_ctx {
     size_t pinned_bytes = 0;
     const vct::DeviceBuffer* pinned_dst = nullptr;  // buffer that holds the pinned block's first pinned_valid bytes
     size_t pinned_valid = 0;
+    void* pinned_word = nullptr;  // pinned landing slot of small device-to-host reads
 
     vct::SvoState svo;
 };

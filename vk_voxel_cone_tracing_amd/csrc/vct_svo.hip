@@ -257,6 +257,14 @@ __global__ __launch_bounds__(256) void k_svo_alloc_assign(uint2* __restrict__ po
     pool[begin + i] = make_uint2(((base + rank + 1u) << 3u) | 0x80000000u, 0u);
 }
 
+// octreeNodeModifyArg.comp: the child blocks handed out so far, on the device
+// (the last wave's offset plus its count), read back once per level.
+__global__ void k_svo_advance(const uint32_t* __restrict__ wave_offsets, const uint32_t* __restrict__ wave_counts,
+                              uint32_t waves, uint32_t* __restrict__ counter)
+{
+    counter[0] += wave_offsets[waves - 1] + wave_counts[waves - 1];
+}
+
 // octreeNodeLeafWrite.comp, ordered: one key (leaf << fbits | fragment) per
 // fragment that reaches a leaf; fragments that do not take `none`, above
 // every such key, and sort last.
@@ -460,6 +468,11 @@ vct_status vct_svo_build(vct_ctx* c, vct_stream stream)
     VCT_TRY(ensure(c->offsets, 8u * ((cap >> 6) + 8)));
     VCT_TRY(ensure(c->rec_idx[1], 4u * (size_t)(nfrag ? nfrag : 1)));
     uint32_t* node_of = (uint32_t*)c->rec_idx[1].ptr;  // each fragment's node between flag passes
+    VCT_TRY(ensure(c->svo.counters, 16));
+    uint32_t* dev_counter = (uint32_t*)c->svo.counters.ptr;
+    VCT_HIP(hipMemsetAsync(dev_counter, 0, 4, s));
+    if (!c->pinned_word) VCT_HIP(hipHostMalloc(&c->pinned_word, 64, hipHostMallocDefault));
+    volatile uint32_t* host_counter = (volatile uint32_t*)c->pinned_word;
     for (uint32_t i = 1; i <= L; ++i) {
         if ((uint64_t)begin + num > cap) return fail(VCT_ERR_CAPACITY, "octree node pool overflow (%llu nodes)", (unsigned long long)cap);
         level_begin.push_back(begin);
@@ -482,11 +495,10 @@ vct_status vct_svo_build(vct_ctx* c, vct_stream stream)
         VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, wave_counts, wave_offsets, (int)waves, s));
         hipLaunchKernelGGL(k_svo_alloc_assign, dim3(blocks_for(num)), dim3(256), 0, s, pool, begin, num, wave_offsets,
                            counter);
-        uint32_t lw[2];
-        VCT_HIP(hipMemcpyAsync(&lw[0], wave_offsets + waves - 1, 4, hipMemcpyDeviceToHost, s));
-        VCT_HIP(hipMemcpyAsync(&lw[1], wave_counts + waves - 1, 4, hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_svo_advance, dim3(1), dim3(1), 0, s, wave_offsets, wave_counts, waves, dev_counter);
+        VCT_HIP(hipMemcpyAsync((void*)host_counter, dev_counter, 4, hipMemcpyDeviceToHost, s));
         VCT_HIP(hipStreamSynchronize(s));
-        counter += lw[0] + lw[1];
+        counter = *host_counter;
         begin += num;
         num = (counter << 3) + 8u - begin;
     }

@@ -391,6 +391,8 @@ vct_status vct_inject_radiance(vct_ctx* c, vct_stream stream, const vct_mesh* me
     // buffer and runs the raster again, which happens while the capacity
     // warms up. Record keys carry the work index, so the sorted order does
     // not depend on which run stored them.
+    if (!c->pinned_word) VCT_HIP(hipHostMalloc(&c->pinned_word, 64, hipHostMallocDefault));
+    volatile unsigned long long* landed = (volatile unsigned long long*)c->pinned_word;
     unsigned long long tot[2] = {0, 0};
     for (int attempt = 0;; ++attempt) {
         const uint32_t cap = c->inject_cap;
@@ -400,8 +402,10 @@ vct_status vct_inject_radiance(vct_ctx* c, vct_stream stream, const vct_mesh* me
                            box_hi[1], box_hi[2], (unsigned long long*)c->rec_key[0].ptr, (uint32_t*)c->rec_idx[0].ptr,
                            (Record*)c->rec_val.ptr, cap);
         VCT_HIP(hipGetLastError());
-        VCT_HIP(hipMemcpyAsync(tot, c->totals.ptr, 16, hipMemcpyDeviceToHost, s));
+        VCT_HIP(hipMemcpyAsync((void*)landed, c->totals.ptr, 16, hipMemcpyDeviceToHost, s));
         VCT_HIP(hipStreamSynchronize(s));
+        tot[0] = landed[0];
+        tot[1] = landed[1];
         if (tot[0] > 0xffffffffull) return fail(VCT_ERR_CAPACITY, "raster work %llu exceeds 2^32", tot[0]);
         if (tot[1] <= cap) break;
         if (tot[1] > 0xfffffff0ull || attempt > 0)
