@@ -58,6 +58,14 @@ __global__ void k_svo_count(const float* __restrict__ npos, uint32_t ntris, SvoR
     counts[t] = n;
 }
 
+// The total of an exclusive scan: its last element plus the last input.
+template <typename T>
+__global__ void k_scan_total(const T* __restrict__ scan, const T* __restrict__ in, uint32_t n,
+                             unsigned long long* __restrict__ out)
+{
+    out[0] = (unsigned long long)scan[n - 1] + (unsigned long long)in[n - 1];
+}
+
 // covered[wi] = 1 for every covered sample: each is one fragment invocation
 // and consumes one fragment-list slot (voxelizer.frag:55).
 __global__ __launch_bounds__(256) void k_svo_cover(const float* __restrict__ npos,
@@ -405,11 +413,14 @@ vct_status vct_svo_voxelize(vct_ctx* c, vct_stream stream, const vct_mesh* mesh,
     VCT_TRY(ensure(c->scan_tmp, tmp));
     VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, (unsigned long long*)c->counts.ptr,
                                              (unsigned long long*)c->offsets.ptr, (int)soup.count, s));
-    unsigned long long last[2];
-    VCT_HIP(hipMemcpyAsync(&last[0], (unsigned long long*)c->offsets.ptr + soup.count - 1, 8, hipMemcpyDeviceToHost, s));
-    VCT_HIP(hipMemcpyAsync(&last[1], (unsigned long long*)c->counts.ptr + soup.count - 1, 8, hipMemcpyDeviceToHost, s));
+    if (!c->pinned_word) VCT_HIP(hipHostMalloc(&c->pinned_word, 64, hipHostMallocDefault));
+    volatile unsigned long long* landed = (volatile unsigned long long*)c->pinned_word;
+    hipLaunchKernelGGL(k_scan_total<unsigned long long>, dim3(1), dim3(1), 0, s,
+                       (const unsigned long long*)c->offsets.ptr, (const unsigned long long*)c->counts.ptr, soup.count,
+                       (unsigned long long*)c->totals.ptr);
+    VCT_HIP(hipMemcpyAsync((void*)landed, c->totals.ptr, 8, hipMemcpyDeviceToHost, s));
     VCT_HIP(hipStreamSynchronize(s));
-    const unsigned long long nwork64 = last[0] + last[1];
+    const unsigned long long nwork64 = landed[0];
     if (nwork64 >= 0xffffffffull) return fail(VCT_ERR_CAPACITY, "raster work %llu exceeds 2^32", nwork64);
     const uint32_t nwork = (uint32_t)nwork64;
     if (!nwork) return VCT_OK;
@@ -423,11 +434,11 @@ vct_status vct_svo_voxelize(vct_ctx* c, vct_stream stream, const vct_mesh* mesh,
     VCT_TRY(ensure(c->scan_tmp, tmp));
     VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, covered, slot, (int)nwork, s));
     // the count pass's host read-back (SparseVoxelizer.cpp:279)
-    uint32_t lastc[2];
-    VCT_HIP(hipMemcpyAsync(&lastc[0], slot + nwork - 1, 4, hipMemcpyDeviceToHost, s));
-    VCT_HIP(hipMemcpyAsync(&lastc[1], covered + nwork - 1, 4, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_scan_total<uint32_t>, dim3(1), dim3(1), 0, s, (const uint32_t*)slot, (const uint32_t*)covered,
+                       nwork, (unsigned long long*)c->totals.ptr);
+    VCT_HIP(hipMemcpyAsync((void*)landed, c->totals.ptr, 8, hipMemcpyDeviceToHost, s));
     VCT_HIP(hipStreamSynchronize(s));
-    const uint32_t nfrag = lastc[0] + lastc[1];
+    const uint32_t nfrag = (uint32_t)landed[0];
     VCT_TRY(ensure(c->svo.frags, 8u * (size_t)(nfrag ? nfrag : 1)));
     if (nfrag) {
         hipLaunchKernelGGL(k_fill_u2, dim3(blocks_for(nfrag)), dim3(256), 0, s, (uint2*)c->svo.frags.ptr, nfrag,
