@@ -468,8 +468,8 @@ struct vct_batch {
 
 static void drop_graph(vct_batch* b)
 {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
     b->graph_valid = false;
@@ -568,8 +568,8 @@ static vct_status capture_graph(vct_batch* b, unsigned long long* samples)
                                                                 b->capture);
                 if (e != hipSuccess) {
                     hipGraph_t g0 = nullptr;
-                    hipStreamEndCapture(b->capture, &g0);
-                    if (g0) hipGraphDestroy(g0);
+                    (void)hipStreamEndCapture(b->capture, &g0);
+                    if (g0) (void)hipGraphDestroy(g0);
                     return fail(VCT_ERR_HIP, "batched raster scan: %s", hipGetErrorString(e));
                 }
                 hipLaunchKernelGGL(k_phase_total, dim3(1), dim3(1), 0, b->capture, counts, offsets, nt, total);
@@ -765,7 +765,7 @@ extern "C" {
 vct_status vct_batch_destroy(vct_batch* b)
 {
     if (!b) return VCT_OK;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     drop_graph(b);
     if (b->capture) hipStreamDestroy(b->capture);
     DeviceBuffer* bufs[] = {&b->scene_table, &b->prims,  &b->tasks,  &b->counters, &b->soup_pos, &b->soup_mat,

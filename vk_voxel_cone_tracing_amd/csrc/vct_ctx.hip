@@ -43,7 +43,7 @@ vct_status ensure(DeviceBuffer& b, size_t bytes)
 {
     if (bytes <= b.bytes && b.ptr) return VCT_OK;
     if (b.ptr) {
-        hipFree(b.ptr);
+        (void)hipFree(b.ptr);
         b.ptr = nullptr;
         b.bytes = 0;
     }
@@ -60,7 +60,7 @@ vct_status ensure(DeviceBuffer& b, size_t bytes)
 
 void release(DeviceBuffer& b)
 {
-    if (b.ptr) hipFree(b.ptr);
+    if (b.ptr) (void)hipFree(b.ptr);
     b.ptr = nullptr;
     b.bytes = 0;
 }
@@ -301,7 +301,7 @@ vct_status vct_create(vct_ctx** out_ctx, const vct_config* cfg)
 vct_status vct_destroy(vct_ctx* c)
 {
     if (!c) return VCT_OK;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     release(c->opacity);
     release(c->radiance);
     DeviceBuffer* bufs[] = {&c->soup_pos, &c->soup_nrm, &c->soup_mat, &c->prims, &c->prim_first, &c->counts,
@@ -309,8 +309,8 @@ vct_status vct_destroy(vct_ctx* c)
                             &c->rec_idx[1], &c->rec_val, &c->sort_tmp, &c->staging, &c->materials_flat,
                             &c->svo.frags, &c->svo.pool, &c->svo.counters, &c->spec_perm};
     for (DeviceBuffer* b : bufs) release(*b);
-    if (c->pinned) hipHostFree(c->pinned);
-    if (c->pinned_word) hipHostFree(c->pinned_word);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->pinned_word) (void)hipHostFree(c->pinned_word);
     delete c;
     return VCT_OK;
 }

@@ -189,7 +189,7 @@ vct_status vct_dense_create(vct_dense** out, const vct_dense_desc* d)
 vct_status vct_dense_destroy(vct_dense* g)
 {
     if (!g) return VCT_OK;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     release(g->data);
     release(g->winner);
     release(g->bwd_rec);

@@ -79,7 +79,6 @@ __device__ __forceinline__ Corner corner_of(const Pyramid& P, int m, const float
 // sample gradients per channel, which bounds every window sum (see
 // fixed_scale); the per-workgroup accumulators of the coarse levels, shared
 // by many cones, stay in double.
-constexpr int kBwdThreads = 256;  // one 16 x 16 pixel tile, four waves of 8 x 8
 constexpr int kPoolUnits = 1024;  // a wave's LDS pool for its two live level windows, in 16 B units (16 KB)
 constexpr int kMaxCopies = 16;    // lane-striped copies of a small window
 constexpr int kCoarseN = 4;       // levels with at most 4^3 interior cells stay in LDS per tile
