@@ -183,10 +183,9 @@ def test_radiance_injection_bit_exact(fmt, quirks):
 def test_scene_changes_between_calls():
     # One context voxelizes and injects a small scene, a larger one, then the
     # small one again: the device copy of the draw list must follow every
-    # change of the list, the injection record buffer grows from the small
-    # scene's capacity, and the larger scene's 80 000 triangles take the
-    # device-wide scan instead of the one-workgroup scan of small draw lists.
-    # Every pass stays bit-exact against the oracle.
+    # change of the list, and the injection record buffer grows from the
+    # small scene's capacity (the larger scene has 80 000 triangles). Every
+    # pass stays bit-exact against the oracle.
     import torch
     import vk_voxel_cone_tracing_amd as V
     from vk_voxel_cone_tracing_amd import scenes

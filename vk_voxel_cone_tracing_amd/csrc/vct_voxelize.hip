@@ -106,44 +106,6 @@ __global__ void k_total(const unsigned long long* __restrict__ counts, const uns
 
 __global__ void k_reset_records(unsigned long long* __restrict__ total) { total[1] = 0ull; }
 
-// Exclusive scan of up to kSmallScan counts by one workgroup, with the total
-// (what k_total writes) at the end: one launch instead of a device-wide scan
-// and k_total for a draw list of tens of thousands of triangles. Each thread
-// sums a contiguous run, the runs' sums are scanned across the workgroup, and
-// each thread writes its run's offsets. Integer sums: the offsets equal any
-// other scan's.
-constexpr uint32_t kSmallScan = 1024u * 64u;
-__global__ __launch_bounds__(1024) void k_scan_small(const unsigned long long* __restrict__ counts, uint32_t n,
-                                                     unsigned long long* __restrict__ offsets,
-                                                     unsigned long long* __restrict__ total)
-{
-    __shared__ unsigned long long s_wave[16];
-    const uint32_t t = threadIdx.x, per = (n + 1023u) / 1024u;
-    const uint32_t b = min(t * per, n), e = min(b + per, n);
-    unsigned long long sum = 0;
-    for (uint32_t i = b; i < e; ++i) sum += counts[i];
-    const int lane = (int)(t & 63u), w = (int)(t >> 6);
-    unsigned long long x = sum;  // inclusive scan over the wave
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long y = __shfl_up(x, (unsigned)o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) s_wave[w] = x;
-    __syncthreads();
-    if (t == 0)
-        for (int k = 1; k < 16; ++k) s_wave[k] += s_wave[k - 1];
-    __syncthreads();
-    unsigned long long run = (w ? s_wave[w - 1] : 0ull) + x - sum;
-    for (uint32_t i = b; i < e; ++i) {
-        offsets[i] = run;
-        run += counts[i];
-    }
-    if (t == 0) {
-        total[0] = s_wave[15];
-        total[1] = 0ull;
-    }
-}
-
 // Bits of the work index in an injection record key: enough for n - 1 (n
 // work items). The key is (texel index within the level) << bits | index.
 __host__ __device__ inline uint32_t work_bits(unsigned long long n)
@@ -339,14 +301,6 @@ static vct_status expand(vct_ctx* c, hipStream_t s, const TriSoup& soup, const V
     VCT_TRY(ensure(c->counts, 8u * (size_t)(n ? n : 1)));
     VCT_TRY(ensure(c->offsets, 8u * (size_t)(n ? n : 1)));
     VCT_TRY(ensure(c->totals, 64));
-    if (n && n <= kSmallScan) {
-        hipLaunchKernelGGL(k_count, dim3((n + 255) / 256), dim3(256), 0, s, soup.pos, n, P,
-                           (unsigned long long*)c->counts.ptr);
-        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, (const unsigned long long*)c->counts.ptr, n,
-                           (unsigned long long*)c->offsets.ptr, (unsigned long long*)c->totals.ptr);
-        VCT_HIP(hipGetLastError());
-        return VCT_OK;
-    }
     if (n) {
         hipLaunchKernelGGL(k_count, dim3((n + 255) / 256), dim3(256), 0, s, soup.pos, n, P,
                            (unsigned long long*)c->counts.ptr);
