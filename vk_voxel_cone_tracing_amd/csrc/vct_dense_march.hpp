@@ -70,6 +70,9 @@ __device__ __forceinline__ const float* cone_dir(const March& M, int i)
     return M.cones32 ? c_dense_dirs32[i] : c_dense_dirs16[i];
 }
 
+// A texel's channels: the floats of an RGBA32F grid, or the bytes (0..255,
+// not yet divided by 255) of an RGBA8 grid; trilinear() applies the 1/255
+// once to its result instead of to each of the 8 texels.
 template <bool kF32>
 __device__ __forceinline__ f4 texel(const void* __restrict__ data, unsigned long long i)
 {
@@ -78,8 +81,7 @@ __device__ __forceinline__ f4 texel(const void* __restrict__ data, unsigned long
         return {t.x, t.y, t.z, t.w};
     }
     const uint32_t t = ((const uint32_t*)data)[i];
-    const float s = 1.0f / 255.0f;
-    return {(float)(t & 0xffu) * s, (float)((t >> 8) & 0xffu) * s, (float)((t >> 16) & 0xffu) * s, (float)(t >> 24) * s};
+    return {(float)(t & 0xffu), (float)((t >> 8) & 0xffu), (float)((t >> 16) & 0xffu), (float)(t >> 24)};
 }
 
 // Footprint of one trilinear fetch of level m (bordered indices).
@@ -137,6 +139,13 @@ __device__ __forceinline__ f4 trilinear(const void* __restrict__ data, const Foo
     VCT_T(z);
     VCT_T(w);
 #undef VCT_T
+    if (!kF32) {
+        const float s = 1.0f / 255.0f;
+        r.x *= s;
+        r.y *= s;
+        r.z *= s;
+        r.w *= s;
+    }
     return r;
 }
 
