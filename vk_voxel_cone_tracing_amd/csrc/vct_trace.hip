@@ -153,10 +153,9 @@ __device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restr
             t011 = unpack_half(make_uint2(q3.x, q3.y));
             t111 = unpack_half(make_uint2(q3.z, q3.w));
         } else {
-            const float sc = 1.0f / 255.0f;
-            auto u8 = [sc](uint32_t t) -> f4 {
-                return {(float)(t & 0xffu) * sc, (float)((t >> 8) & 0xffu) * sc, (float)((t >> 16) & 0xffu) * sc,
-                        (float)(t >> 24) * sc};
+            // raw bytes; the 1/255 is applied once to the level's sum below
+            auto u8 = [](uint32_t t) -> f4 {
+                return {(float)(t & 0xffu), (float)((t >> 8) & 0xffu), (float)((t >> 16) & 0xffu), (float)(t >> 24)};
             };
             const vct_u2 q0 = __builtin_amdgcn_raw_buffer_load_b64(R, voff, 0, 0);
             const vct_u2 q1 = __builtin_amdgcn_raw_buffer_load_b64(R, voff, sy, 0);
@@ -180,6 +179,13 @@ __device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restr
         VCT_TRI(z);
         VCT_TRI(w);
 #undef VCT_TRI
+    }
+    if (!kHalf) {
+        const float sc = 1.0f / 255.0f;
+        acc.x *= sc;
+        acc.y *= sc;
+        acc.z *= sc;
+        acc.w *= sc;
     }
     return acc;
 }
