@@ -98,23 +98,28 @@ __global__ void k_begin_radiance(uint2* __restrict__ rad, const uint32_t* __rest
 }
 
 // ---------------------------------------------------------------- K7
+// One thread per texel of one border plane of one (level, face): the border
+// texels are those of [0, n)^3 with a coordinate outside [1, res], i.e. on a
+// plane at 0 or res + 1; each takes the interior texel its coordinates wrap to
+// (borderWrapping.comp:20-36), so no copy reads a texel another one writes.
+// A texel on two planes gets the same value twice.
 template <typename T>
-__global__ void k_wrap_border(T* __restrict__ vol, int D, int res, int n, int levels, int faces, int rwb)
+__global__ __launch_bounds__(256) void k_wrap_border(T* __restrict__ vol, int D, int res, int n, int faces)
 {
-    uint32_t gx = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t gy = blockIdx.y * blockDim.y + threadIdx.y;
-    uint32_t gz = blockIdx.z * blockDim.z + threadIdx.z;
-    if (gx >= (uint32_t)n || gy >= (uint32_t)n || gz >= (uint32_t)n) return;
-    if (gx >= (uint32_t)rwb || gy >= (uint32_t)rwb || gz >= (uint32_t)rwb) return;
-    // borderWrapping.comp:20-22: interior texels return
-    if (gx < (uint32_t)res + 1 && gy < (uint32_t)res + 1 && gz < (uint32_t)res + 1 && gx > 0 && gy > 0 && gz > 0)
-        return;
-    int rx = (((int)gx + res - 1) & (res - 1)) + 1;
-    int ry = (((int)gy + res - 1) & (res - 1)) + 1;
-    int rz = (((int)gz + res - 1) & (res - 1)) + 1;
-    for (int l = 0; l < levels; ++l)
-        for (int f = 0; f < faces; ++f)
-            vol[vol_index(D, l, f, (int)gx, (int)gy, (int)gz)] = vol[vol_index(D, l, f, rx, ry, rz)];
+    const int u = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int v = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    const int plane = (int)blockIdx.z % 6, lf = (int)blockIdx.z / 6;
+    const int l = lf / faces, f = lf % faces;
+    const int axis = plane >> 1, c = (plane & 1) ? res + 1 : 0;
+    if (u >= n || v >= n || c >= n) return;
+    int g[3];
+    g[axis] = c;
+    g[axis == 0 ? 1 : 0] = u;
+    g[axis == 2 ? 1 : 2] = v;
+    int r[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = ((g[k] + res - 1) & (res - 1)) + 1;
+    vol[vol_index(D, l, f, g[0], g[1], g[2])] = vol[vol_index(D, l, f, r[0], r[1], r[2])];
 }
 
 // ---------------------------------------------------------------- K4 / K5
@@ -302,16 +307,17 @@ vct_status vct_wrap_border(vct_ctx* c, vct_stream stream, vct_volume v, const vc
     const int rwb = c->res + 2;
     // BorderWrapper.cpp:139-140 dispatches ((res + border) >> 3) groups of 8.
     const int n = (c->cfg.quirks & VCT_QUIRK_BORDER_SKIP_FAR) ? 8 * (rwb >> 3) : rwb;
-    dim3 block(64, 2, 2);
-    dim3 grid((n + 63) / 64, (n + 1) / 2, (n + 1) / 2);
+    if (d->clip_region_count == 0 || d->face_count == 0) return VCT_OK;
+    dim3 block(64, 4, 1);
+    dim3 grid((n + 63) / 64, (n + 3) / 4, 6u * d->clip_region_count * d->face_count);
     hipStream_t s = (hipStream_t)stream;
     if (is_half(c, v))
         hipLaunchKernelGGL(k_wrap_border<uint2>, grid, block, 0, s, (uint2*)c->radiance.ptr, c->D, c->res, n,
-                           (int)d->clip_region_count, (int)d->face_count, rwb);
+                           (int)d->face_count);
     else
         hipLaunchKernelGGL(k_wrap_border<uint32_t>, grid, block, 0, s,
                            (uint32_t*)(v == VCT_VOLUME_OPACITY ? c->opacity.ptr : c->radiance.ptr), c->D, c->res, n,
-                           (int)d->clip_region_count, (int)d->face_count, rwb);
+                           (int)d->face_count);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
