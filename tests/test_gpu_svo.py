@@ -163,3 +163,50 @@ def test_svo_trace_matches_oracle():
             err = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-12)
             print(f"mode={mode} {name}: rel L2 {err:.2e}")
             assert err <= 1e-3, (mode, name, err)
+
+
+def test_empty_draw_list():
+    # A mesh with no primitives: voxelization and injection leave the volumes
+    # untouched, the SVO has no fragments and its pool is the 8 root children
+    # (as the oracle builds it), and the context still works afterwards.
+    import torch
+    import vk_voxel_cone_tracing_amd as V
+    from vk_voxel_cone_tracing_amd import scenes
+
+    scene = _scene()
+    pos, nrm, mat = scene.world_triangles()
+    ctx = V.Context(resolution=16, levels=2, quirks=0x3F, raster_samples=1, downsample_region_size=1, extent_l0=1.0)
+    ctx.update_clip_regions(np.full(3, 0.5))
+    ds = V.DeviceScene(scene)
+    light = scenes.light(1, scale=1.0)
+    depth = O.raycast_shadow(pos, nrm, mat, light, 128, 128)
+    sm = V.shadow_struct(torch.from_numpy(depth).cuda())
+    ld, ls = V.light_structs(light)
+    full_count = ds.mesh.primitive_count
+    ds.mesh.primitive_count = 0
+    ctx.clear_volume(V.VOLUME_OPACITY)
+    ctx.clear_volume(V.VOLUME_RADIANCE)
+    for level in range(2):
+        desc = ctx.voxelization_desc(level)
+        ctx.voxelize_opacity(ds, desc, level)
+        ctx.inject_radiance(ds, sm, ld, ls, desc, level)
+    assert not ctx.read_volume(V.VOLUME_OPACITY).any() and not ctx.read_volume(V.VOLUME_RADIANCE).any()
+    nfrag = C.c_uint32(7)
+    V._abi.call("vct_svo_voxelize", ctx._h, None, C.byref(ds.mesh), C.byref(sm), C.byref(ld), C.byref(ls), LEVEL,
+                C.byref(nfrag))
+    assert nfrag.value == 0
+    V._abi.call("vct_svo_build", ctx._h, None)
+    n = C.c_size_t(0)
+    V._abi.call("vct_svo_read_nodes", ctx._h, None, None, 0, C.byref(n))
+    nodes = np.zeros((n.value, 2), np.uint32)
+    V._abi.call("vct_svo_read_nodes", ctx._h, None, nodes.ctypes.data_as(C.c_void_p), n.value, C.byref(n))
+    opool = np.zeros((1000000, 2), np.uint32)
+    ocount = C.c_uint32(0)
+    assert O.lib().orc_svo_build(O.vptr(np.zeros((1, 2), np.uint32)), 0, LEVEL, 0x3F, O.vptr(opool), len(opool),
+                                 C.byref(ocount)) == 0
+    np.testing.assert_array_equal(nodes, opool[:ocount.value])
+    assert len(nodes) == 8 and not nodes.any()
+    # the same context with the draw list back
+    ds.mesh.primitive_count = full_count
+    ctx.voxelize_opacity(ds, ctx.voxelization_desc(0), 0)
+    assert ctx.read_volume(V.VOLUME_OPACITY).any()

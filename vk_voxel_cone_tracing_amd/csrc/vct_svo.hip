@@ -485,6 +485,7 @@ vct_status vct_svo_build(vct_ctx* c, vct_stream stream)
     if (!c->pinned_word) VCT_HIP(hipHostMalloc(&c->pinned_word, 64, hipHostMallocDefault));
     volatile uint32_t* host_counter = (volatile uint32_t*)c->pinned_word;
     for (uint32_t i = 1; i <= L; ++i) {
+        if (num == 0) break;  // no node was flagged: the tree ends here (the reference's zero-group dispatches)
         if ((uint64_t)begin + num > cap) return fail(VCT_ERR_CAPACITY, "octree node pool overflow (%llu nodes)", (unsigned long long)cap);
         level_begin.push_back(begin);
         level_num.push_back(num);
