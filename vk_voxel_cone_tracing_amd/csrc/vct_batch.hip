@@ -373,10 +373,16 @@ __global__ __launch_bounds__(256) void k_phase_raster_flat(const SceneDev* __res
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < n; base += stride) {
         const TriRange tr = block_triangles(offsets, ntris, base, min(base + blockDim.x - 1, n - 1));
+        // the scenes of the block's first and last triangles bound every lane's scene
+        const uint32_t sc_lo = scene_of(tri_start, nscenes, tr.lo), sc_hi = scene_of(tri_start, nscenes, tr.hi);
         const unsigned long long wi = base + threadIdx.x;
         if (wi >= n) continue;
         const uint32_t g = work_triangle(offsets, tr.lo, tr.hi, wi);
-        const uint32_t sc = scene_of(tri_start, nscenes, g);
+        uint32_t sc = sc_lo;
+        for (uint32_t hi = sc_hi + 1; hi - sc > 1;) {  // last scene in [sc_lo, sc_hi] starting at or before g
+            const uint32_t mid = (sc + hi) >> 1;
+            if (tri_start[mid] <= g) sc = mid; else hi = mid;
+        }
         const SceneDev& S = scenes[sc];
         const uint32_t t = g - tri_start[sc];
         float p[9];
