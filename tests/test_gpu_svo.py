@@ -26,7 +26,7 @@ def _scene():
     return scenes.room(1, scale=1.0, n_boxes=16, n_spheres=2, thin_total=6000, emissive_boxes=1)
 
 
-def _run(quirks, samples=1):
+def _run(quirks, samples=1, level=LEVEL):
     import torch
     import vk_voxel_cone_tracing_amd as V
     from vk_voxel_cone_tracing_amd import scenes
@@ -41,7 +41,7 @@ def _run(quirks, samples=1):
     sm = V.shadow_struct(torch.from_numpy(depth).cuda())
     ld, ls = V.light_structs(light)
     nfrag = C.c_uint32(0)
-    V._abi.call("vct_svo_voxelize", ctx._h, None, C.byref(ds.mesh), C.byref(sm), C.byref(ld), C.byref(ls), LEVEL,
+    V._abi.call("vct_svo_voxelize", ctx._h, None, C.byref(ds.mesh), C.byref(sm), C.byref(ld), C.byref(ls), level,
                 C.byref(nfrag))
     n = C.c_size_t(0)
     frags = np.zeros((nfrag.value, 2), np.uint32)
@@ -58,11 +58,11 @@ def _run(quirks, samples=1):
     bbmin = (C.c_float * 3)(*scene.bb_min.tolist())
     bbmax = (C.c_float * 3)(*scene.bb_max.tolist())
     onf = L.orc_svo_fragments(C.byref(O.tris_struct(pos, nrm, mat)), O.materials_struct(scene), C.byref(osm),
-                              C.byref(O.light_struct(light)), bbmin, bbmax, 1 << LEVEL, samples, quirks,
+                              C.byref(O.light_struct(light)), bbmin, bbmax, 1 << level, samples, quirks,
                               O.vptr(ofr), cap)
     opool = np.zeros((max(1000000, 8 * onf), 2), np.uint32)
     ocount = C.c_uint32(0)
-    assert L.orc_svo_build(O.vptr(ofr), onf, LEVEL, quirks, O.vptr(opool), len(opool), C.byref(ocount)) == 0
+    assert L.orc_svo_build(O.vptr(ofr), onf, level, quirks, O.vptr(opool), len(opool), C.byref(ocount)) == 0
     return dict(ctx=ctx, ds=ds, sm=sm, ld=ld, ls=ls, scene=scene, light=light, depth=depth, frags=frags,
                 nodes=nodes, ofrags=ofr[:onf], onodes=opool[:ocount.value], pos=pos, nrm=nrm, mat=mat)
 
@@ -72,6 +72,16 @@ def test_fragment_list_and_pool_bit_exact(quirks, samples):
     r = _run(quirks, samples)
     print(f"quirks={quirks:#x} samples={samples}: {len(r['frags'])} fragments, {len(r['nodes'])} nodes")
     assert len(r["frags"]) > 10000
+    np.testing.assert_array_equal(r["frags"], r["ofrags"])
+    np.testing.assert_array_equal(r["nodes"], r["onodes"])
+
+
+@pytest.mark.parametrize("level,quirks", [(1, 0x3F), (2, 0x00), (9, 0x3F), (9, 0x00)])
+def test_fragment_list_and_pool_bit_exact_at_depth_extremes(level, quirks):
+    # the shallowest octrees (one and two levels) and a 512^3 one
+    r = _run(quirks, 1, level)
+    print(f"level={level} quirks={quirks:#x}: {len(r['frags'])} fragments, {len(r['nodes'])} nodes")
+    assert len(r["frags"]) > 0 and len(r["nodes"]) >= 8
     np.testing.assert_array_equal(r["frags"], r["ofrags"])
     np.testing.assert_array_equal(r["nodes"], r["onodes"])
 
