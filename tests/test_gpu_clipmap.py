@@ -414,3 +414,43 @@ def test_fused_radiance_begin_equals_clear_inject_copy_alpha():
         results.append(ctx.read_volume(V.VOLUME_RADIANCE))
     assert (results[0].reshape(results[0].shape[:3] + (-1,)) != 0).any(-1).sum() > 1000
     np.testing.assert_array_equal(results[0], results[1])
+
+
+def test_voxelize_and_inject_at_128():
+    # The benchmark frame's resolution (128^3 per level, RGBA16F radiance) on
+    # two levels: opacity and injection bit-exact against the oracle.
+    import torch
+    import vk_voxel_cone_tracing_amd as V
+    from vk_voxel_cone_tracing_amd import scenes
+
+    res, levels = 128, 2
+    scene = scenes.room(0, scale=SCALE, n_boxes=24, n_spheres=4, emissive_boxes=2)
+    ctx = V.Context(resolution=res, levels=levels, radiance_format=1, quirks=0x3F, raster_samples=1,
+                    downsample_region_size=10, extent_l0=EXTENT_L0)
+    ctx.update_clip_regions(np.full(3, 0.5 * SCALE) + np.array([0.13, -0.07, 0.21]))
+    ds = V.DeviceScene(scene)
+    pos, nrm, mat = scene.world_triangles()
+    light, depth, depth_d = _light_and_shadow(scene, pos, nrm, mat)
+    ld, ls = V.light_structs(light)
+    sm = V.shadow_struct(depth_d)
+    torch.cuda.synchronize()
+    ctx.clear_volume(V.VOLUME_OPACITY)
+    ctx.clear_volume(V.VOLUME_RADIANCE)
+    ref_o, ref_r = O.new_volume(res, levels), O.new_volume(res, levels, 1)
+    L = O.lib()
+    tris, mats = O.tris_struct(pos, nrm, mat), O.materials_struct(scene)
+    osm = O.Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
+    olight = O.light_struct(light)
+    for level in range(levels):
+        desc = ctx.voxelization_desc(level)
+        odesc = _desc_to_oracle(desc, ctx.regions[level])
+        ctx.voxelize_opacity(ds, desc, level)
+        ctx.inject_radiance(ds, sm, ld, ls, desc, level)
+        L.orc_voxelize_opacity(C.byref(O.vol_struct(ref_o, res, levels)), C.byref(tris), C.byref(odesc), 1, 0x3F)
+        L.orc_inject_radiance(C.byref(O.vol_struct(ref_r, res, levels)), C.byref(tris), mats, C.byref(osm),
+                              C.byref(olight), C.byref(odesc), 1, 0x3F)
+    occupied = int((ref_o != 0).sum())
+    print(f"128^3 x {levels}: {occupied} occupied face texels")
+    assert occupied > 100000
+    np.testing.assert_array_equal(ctx.read_volume(V.VOLUME_OPACITY), ref_o)
+    np.testing.assert_array_equal(ctx.read_volume(V.VOLUME_RADIANCE), ref_r)
