@@ -20,13 +20,13 @@ EXTENT_L0 = 4.0
 SCALE = 4.0
 
 
-def _setup_case(quirks, samples=1, fmt=0, seed=0):
+def _setup_case(quirks, samples=1, fmt=0, seed=0, res=RES, levels=LEVELS):
     import torch
     import vk_voxel_cone_tracing_amd as V
     from vk_voxel_cone_tracing_amd import scenes
 
     scene = scenes.room(seed, scale=SCALE, n_boxes=24, n_spheres=4, emissive_boxes=2)
-    ctx = V.Context(resolution=RES, levels=LEVELS, radiance_format=fmt, quirks=quirks, raster_samples=samples,
+    ctx = V.Context(resolution=res, levels=levels, radiance_format=fmt, quirks=quirks, raster_samples=samples,
                     downsample_region_size=2, extent_l0=EXTENT_L0)
     cam = np.full(3, 0.5 * SCALE) + np.array([0.13, -0.07, 0.21])
     ctx.update_clip_regions(cam)
@@ -227,39 +227,41 @@ def test_scene_changes_between_calls():
     assert counts[0] == counts[2] and counts[1][0] > 65536 and counts[1][1] > counts[0][1]
 
 
-def _pipeline(ctx, ds, scene, sm, ld, ls):
+def _pipeline(ctx, ds, scene, sm, ld, ls, levels=LEVELS):
     """A full clipmap frame on the device: voxelize, mip, wrap, inject, copy alpha, mip."""
     import vk_voxel_cone_tracing_amd as V
 
     ctx.clear_volume(V.VOLUME_OPACITY)
     ctx.clear_volume(V.VOLUME_RADIANCE)
     regions = ctx.regions
-    for level in range(LEVELS):
+    for level in range(levels):
         ctx.voxelize_opacity(ds, ctx.voxelization_desc(level), level)
-    for level in range(1, LEVELS):
+    for level in range(1, levels):
         ctx.downsample(V.VOLUME_OPACITY, level, regions[level - 1][0], region_size=2)
     ctx.wrap_border(V.VOLUME_OPACITY)
-    for level in range(LEVELS):
+    for level in range(levels):
         ctx.inject_radiance(ds, sm, ld, ls, ctx.voxelization_desc(level), level)
-    for level in range(LEVELS):
+    for level in range(levels):
         ctx.copy_alpha(level)
-    for level in range(1, LEVELS):
+    for level in range(1, levels):
         ctx.downsample(V.VOLUME_RADIANCE, level, regions[level - 1][0], region_size=2)
 
 
-@pytest.mark.parametrize("fmt,mode,cones32,W,H", [(0, 8, 0, 48, 32), (1, 8, 0, 48, 32), (0, 7, 1, 48, 32),
-                                                  (0, 5, 0, 48, 32), (1, 6, 0, 48, 32), (1, 8, 0, 136, 72)])
-def test_trace_matches_oracle(fmt, mode, cones32, W, H):
+@pytest.mark.parametrize("fmt,mode,cones32,W,H,res,levels",
+                         [(0, 8, 0, 48, 32, RES, LEVELS), (1, 8, 0, 48, 32, RES, LEVELS), (0, 7, 1, 48, 32, RES, LEVELS),
+                          (0, 5, 0, 48, 32, RES, LEVELS), (1, 6, 0, 48, 32, RES, LEVELS), (1, 8, 0, 136, 72, RES, LEVELS),
+                          (1, 8, 0, 48, 32, 128, 6)])  # the last: the benchmark frame's 6 x 128^3 RGBA16F clipmap
+def test_trace_matches_oracle(fmt, mode, cones32, W, H, res, levels):
     import torch
     import vk_voxel_cone_tracing_amd as V
     from vk_voxel_cone_tracing_amd import scenes
 
-    scene, ctx, ds = _setup_case(0x3F, fmt=fmt)
+    scene, ctx, ds = _setup_case(0x3F, fmt=fmt, res=res, levels=levels)
     pos, nrm, mat = scene.world_triangles()
     light, depth, depth_d = _light_and_shadow(scene, pos, nrm, mat)
     ld, ls = V.light_structs(light)
     sm = V.shadow_struct(depth_d)
-    _pipeline(ctx, ds, scene, sm, ld, ls)
+    _pipeline(ctx, ds, scene, sm, ld, ls, levels)
     vp, vp_inv, eye = scenes.camera(0, W, H, scale=SCALE)
     g = O.raycast_gbuffer(scene, pos, nrm, mat, vp, vp_inv, W, H)
     gd = {k: torch.from_numpy(v.view(np.int32) if v.dtype == np.uint32 else (v.view(np.int16) if v.dtype == np.uint16 else v)).cuda() for k, v in g.items()}
@@ -267,10 +269,10 @@ def test_trace_matches_oracle(fmt, mode, cones32, W, H):
     vs0 = reg0[2]
     center = [(reg0[0][k] * vs0) + (reg0[1][k] * vs0) * 0.5 for k in range(3)]
     fields = dict(volume_center=[float(np.float32(c)) for c in center], rendering_mode=mode, voxel_size=vs0,
-                  volume_dimension=float(RES), trace_start_offset=1.0, indirect_diffuse_intensity=8.0,
+                  volume_dimension=float(res), trace_start_offset=1.0, indirect_diffuse_intensity=8.0,
                   ambient_occlusion_factor=0.5, min_trace_step_factor=1.0, indirect_specular_intensity=3.0,
                   occlusion_decay=2.0, enable_32_cones=cones32)
-    desc = V._abi.ConeTracingDesc(V._abi.f3(*fields["volume_center"]), mode, vs0, float(RES), 1.0, 8.0, 0.5, 1.0, 3.0,
+    desc = V._abi.ConeTracingDesc(V._abi.f3(*fields["volume_center"]), mode, vs0, float(res), 1.0, 8.0, 0.5, 1.0, 3.0,
                                   2.0, cones32)
     cam = V.camera_ubo(vp, vp_inv, eye)
     dout = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
@@ -281,18 +283,18 @@ def test_trace_matches_oracle(fmt, mode, cones32, W, H):
     # oracle on the same volumes
     rad = ctx.read_volume(V.VOLUME_RADIANCE)
     L = O.lib()
-    p = O.trace_params(fields, vp_inv, eye, LEVELS)
+    p = O.trace_params(fields, vp_inv, eye, levels)
     og = O.gbuf_struct(g)
     osm = O.Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
     olight = O.light_struct(light)
     rd = np.zeros((H, W, 4), np.float32)
     rs = np.zeros((H, W, 4), np.float32)
     n_ref = C.c_uint64(0)
-    L.orc_trace(C.byref(O.vol_struct(rad, RES, LEVELS)), C.byref(og), C.byref(p), C.byref(osm), C.byref(olight),
+    L.orc_trace(C.byref(O.vol_struct(rad, res, levels)), C.byref(og), C.byref(p), C.byref(osm), C.byref(olight),
                 0x3F, 0, H, O.vptr(rd), O.vptr(rs), C.byref(n_ref))
     rd64 = np.zeros((H, W, 4), np.float64)
     rs64 = np.zeros((H, W, 4), np.float64)
-    L.orc_trace_f64(C.byref(O.vol_struct(rad, RES, LEVELS)), C.byref(og), C.byref(p), C.byref(osm), C.byref(olight),
+    L.orc_trace_f64(C.byref(O.vol_struct(rad, res, levels)), C.byref(og), C.byref(p), C.byref(osm), C.byref(olight),
                     0x3F, 0, H, O.vptr(rd64), O.vptr(rs64), None)
     gd_out, gs_out = dout.cpu().numpy(), sout.cpu().numpy()
     for name, got, ref, ref64 in (("diffuse", gd_out, rd, rd64), ("specular", gs_out, rs, rs64)):
