@@ -268,6 +268,11 @@ vct_status vct_get_clip_regions(const vct_ctx* ctx, vct_clipmap_region* regions,
  * VoxelizationPass::calculateChangeDelta (VoxelizationPass.cpp:438-449) from the
  * initial regions of createVoxelClipmap. Writes the regions into ctx. */
 vct_status vct_update_clip_regions(vct_ctx* ctx, const float camera_pos[3]);
+/* The same regions computed on the host without a context (no device call):
+ * config->voxel_resolution, clip_levels and extent_l0 are read; `count` must
+ * equal clip_levels. */
+vct_status vct_compute_clip_regions(const vct_config* config, const float camera_pos[3],
+                                    vct_clipmap_region* regions, uint32_t count);
 /* Incremental (toroidal) update for a moving camera:
  * VoxelizationPass::fillRevoxelizationRegions (VoxelizationPass.cpp:450-494)
  * with calculateChangeDelta (:438-449) taken from the CURRENT regions. Moves

@@ -191,6 +191,8 @@ void orc_copy_alpha(orc_volume* radiance, const orc_volume* opacity, int clip_le
 {
     const int res = radiance->res;
     const int rwb = res + 2;
+    /* texels are independent */
+#pragma omp parallel for collapse(2) schedule(static)
     for (int gz = 0; gz < res; ++gz)
         for (int gy = 0; gy < res; ++gy)
             for (int gx = 0; gx < res; ++gx) {
@@ -294,6 +296,9 @@ static void downsample_common(orc_volume* vol, const int prev_min[3], int clip_l
     const int res = vol->res;
     const int half = res >> 1;
     const int rwb = res + 2;
+    /* every destination texel reads level clip_level - 1 and writes only
+     * itself, so the texels are independent */
+#pragma omp parallel for collapse(2) schedule(static)
     for (int gz = 0; gz < half; ++gz)
         for (int gy = 0; gy < half; ++gy)
             for (int gx = 0; gx < half; ++gx) {

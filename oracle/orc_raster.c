@@ -298,12 +298,6 @@ static void normalize3(const float in[3], float out[3])
     out[2] = in[2] / len;
 }
 
-static size_t texel_linear(const orc_volume* v, int x, int y, int z)
-{
-    size_t w = (size_t)(v->res + 2) * 6u;
-    size_t h = (size_t)(v->res + 2) * (size_t)v->levels;
-    return ((size_t)z * h + (size_t)y) * w + (size_t)x;
-}
 
 /* imageAtomicRGBA8Avg (msaaInjectRadiance.frag:185-201) for RGBA8 volumes,
  * serialised in fragment order; for RGBA16F volumes an FP32 (sum, count)
@@ -315,7 +309,11 @@ static void sink(inj_ctx* c, int x, int y, int z, const float color[4])
         uint32_t* t = orc_texel_u32(c->vol, x, y, z);
         *t = orc_rgba8_avg(*t, v);
     } else {
-        float* a = c->acc + 4u * texel_linear(c->vol, x, y, z);
+        /* the accumulator holds the clip level's 6 face blocks only */
+        const int rwb = c->vol->res + 2;
+        const size_t face = (size_t)(x / rwb), xi = (size_t)(x % rwb);
+        const size_t j = (size_t)(y - c->d->clip_level * rwb);
+        float* a = c->acc + 4u * ((((face * rwb + (size_t)z) * rwb) + j) * rwb + xi);
         a[0] = a[0] + color[0];
         a[1] = a[1] + color[1];
         a[2] = a[2] + color[2];
@@ -384,20 +382,25 @@ void orc_inject_radiance(orc_volume* vol, const orc_tris* tris, const orc_materi
         c.box_lo[k] = (float)lo[k] * d->voxel_size;
         c.box_hi[k] = (float)(hi[k] + 1) * d->voxel_size;
     }
-    size_t n = orc_volume_texels(vol->res, vol->levels);
+    const size_t rwb = (size_t)vol->res + 2u;
+    const size_t n = 6u * rwb * rwb * rwb; /* the level's texels, (face, z, y, x) */
     if (vol->fmt != ORC_FMT_RGBA8) c.acc = (float*)calloc(n * 4u, sizeof(float));
     for (uint32_t t = 0; t < tris->count; ++t)
         orc_raster_triangle(tris->pos + 9u * t, d->voxel_size, lo, hi, samples, inject_frag, &c, t);
     if (c.acc) {
         /* resolve: mean colour, alpha 1 marks an injected texel */
         uint16_t* tex = (uint16_t*)vol->texels;
+        const size_t W = 6u * rwb, Hh = rwb * (size_t)vol->levels;
         for (size_t i = 0; i < n; ++i) {
             const float* a = c.acc + 4u * i;
             if (a[3] == 0.0f) continue;
-            tex[4u * i + 0] = orc_float_to_half(a[0] / a[3]);
-            tex[4u * i + 1] = orc_float_to_half(a[1] / a[3]);
-            tex[4u * i + 2] = orc_float_to_half(a[2] / a[3]);
-            tex[4u * i + 3] = orc_float_to_half(1.0f);
+            const size_t xi = i % rwb, j = (i / rwb) % rwb, z = (i / (rwb * rwb)) % rwb, face = i / (rwb * rwb * rwb);
+            const size_t y = (size_t)d->clip_level * rwb + j;
+            const size_t at = (z * Hh + y) * W + face * rwb + xi;
+            tex[4u * at + 0] = orc_float_to_half(a[0] / a[3]);
+            tex[4u * at + 1] = orc_float_to_half(a[1] / a[3]);
+            tex[4u * at + 2] = orc_float_to_half(a[2] / a[3]);
+            tex[4u * at + 3] = orc_float_to_half(1.0f);
         }
         free(c.acc);
     }

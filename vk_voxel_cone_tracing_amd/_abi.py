@@ -294,6 +294,8 @@ _PROTOS = {
     "vct_set_clip_regions": (C.c_int, [P, C.POINTER(ClipmapRegion), C.c_uint32]),
     "vct_get_clip_regions": (C.c_int, [P, C.POINTER(ClipmapRegion), C.c_uint32]),
     "vct_update_clip_regions": (C.c_int, [P, C.POINTER(C.c_float)]),
+    "vct_compute_clip_regions": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_float), C.POINTER(ClipmapRegion),
+                                           C.c_uint32]),
     "vct_move_clip_regions": (C.c_int, [P, C.POINTER(C.c_float), C.POINTER(ClipmapRegion), C.POINTER(C.c_uint32)]),
     "vct_make_voxelization_desc": (C.c_int, [P, C.c_uint32, C.POINTER(ClipmapRegion), C.POINTER(VoxelizationDesc)]),
     "vct_clear_volume": (C.c_int, [P, P, C.c_int]),

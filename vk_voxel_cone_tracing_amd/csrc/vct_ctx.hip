@@ -338,18 +338,23 @@ vct_status vct_get_clip_regions(const vct_ctx* c, vct_clipmap_region* regions, u
     return VCT_OK;
 }
 
-vct_status vct_update_clip_regions(vct_ctx* c, const float cam[3])
+vct_status vct_compute_clip_regions(const vct_config* cfg, const float cam[3], vct_clipmap_region* regions,
+                                    uint32_t count)
 {
-    if (!c || !cam) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!cfg || !cam || !regions) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    const uint32_t r = cfg->voxel_resolution;
+    if (r < 8 || r > 256 || (r & (r - 1))) return fail(VCT_ERR_INVALID_ARGUMENT, "voxel_resolution");
+    if (count != cfg->clip_levels || count > (uint32_t)kMaxLevels)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "expected %u regions", cfg->clip_levels);
+    if (!(cfg->extent_l0 > 0.0f)) return fail(VCT_ERR_INVALID_ARGUMENT, "extent_l0");
     // Application::updateClipRegionBoundingBox (Application.cpp:116-128) and
     // VoxelizationPass::calculateChangeDelta (VoxelizationPass.cpp:438-449)
     // applied to the initial regions of createVoxelClipmap (:335-357).
     static const int32_t kMinChange[8] = {2, 2, 2, 2, 2, 1, 1, 1}; // VoxelizationPass.h:57
-    const uint32_t r = (uint32_t)c->res;
-    for (int i = 0; i < c->levels; ++i) {
+    for (uint32_t i = 0; i < count; ++i) {
         vct_clipmap_region reg;
-        float vs = (c->cfg.extent_l0 * (float)(1u << i)) / (float)r;
-        float half = (float)((((uint32_t)c->cfg.extent_l0) >> 1) * (1u << i));
+        float vs = (cfg->extent_l0 * (float)(1u << i)) / (float)r;
+        float half = (float)((((uint32_t)cfg->extent_l0) >> 1) * (1u << i));
         for (int k = 0; k < 3; ++k) {
             reg.min_corner[k] = -(int32_t)(r >> 1);
             reg.extent[k] = r;
@@ -360,8 +365,17 @@ vct_status vct_update_clip_regions(vct_ctx* c, const float cam[3])
             reg.min_corner[k] += delta;
         }
         reg.voxel_size = vs;
-        c->regions[i] = reg;
+        regions[i] = reg;
     }
+    return VCT_OK;
+}
+
+vct_status vct_update_clip_regions(vct_ctx* c, const float cam[3])
+{
+    if (!c || !cam) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    vct_config cfg = c->cfg;
+    cfg.clip_levels = (uint32_t)c->levels;
+    VCT_TRY(vct_compute_clip_regions(&cfg, cam, c->regions, (uint32_t)c->levels));
     c->regions_set = true;
     return VCT_OK;
 }
