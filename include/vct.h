@@ -512,8 +512,9 @@ vct_status vct_batch_run(vct_batch* batch, vct_stream stream, unsigned long long
 /* Synchronizes and reports whether any task of the previous runs gave up
  * waiting on a dependency (VCT_ERR_HIP with a message), then clears it. */
 vct_status vct_batch_status(vct_batch* batch);
-/* Times every candidate schedule `reps` times on `stream`, keeps the fastest
- * and returns it (and its median milliseconds). */
+/* Times candidate schedules `reps` times on `stream` (coordinate descent over
+ * raster chunk, mip brick and trace schedule, phased launches only), keeps
+ * the fastest and returns it (and its median milliseconds). */
 vct_status vct_batch_autotune(vct_batch* batch, vct_stream stream, uint32_t reps, vct_batch_tuning* best,
                               float* best_ms);
 /* Level `level` of scene `scene` in the vct_dense_level layout. */
@@ -521,6 +522,24 @@ vct_status vct_batch_level(vct_batch* batch, uint32_t scene, uint32_t level, voi
                            uint32_t* faces);
 vct_status vct_batch_read_level(vct_batch* batch, vct_stream stream, uint32_t scene, uint32_t level, void* dst,
                                 size_t bytes);
+/* The task queue both schedules run, computed on the host without a batch
+ * (no device call): for scenes of these shapes and tuning.raster_chunk /
+ * tuning.mip_brick, *n_tasks tasks of 4 uint32 each {type << 24 | scene, a,
+ * b, level} (type 0 vertex chunk [a, b) of triangles, 1 raster chunk [a, b),
+ * 2 mip brick a of level `level`, 3 pixel tile a) in queue order, and
+ * need[scene * VCT_BATCH_PHASES + phase] = the task count of each phase
+ * (0 vertex, 1 raster, 1 + m mip level m); a raster task waits for phase 0 of
+ * its scene, a level-m brick for phase m, a tile for the scene's last level.
+ * With tasks == NULL only *n_tasks (and need, if non-NULL) are written. */
+#define VCT_BATCH_PHASES 12
+typedef struct vct_batch_scene_shape {
+    uint32_t triangles;
+    uint32_t resolution;
+    uint32_t width;
+    uint32_t height;
+} vct_batch_scene_shape;
+vct_status vct_batch_plan(const vct_batch_scene_shape* scenes, uint32_t count, const vct_batch_tuning* tuning,
+                          uint32_t* tasks, size_t cap_tasks, size_t* n_tasks, uint32_t* need);
 
 /* ---- the step after the trace ---------------------------------------------
  * SpecularFilterPass (SpecularFilterPass.cpp:75-90, specularFilter.frag:25-53):

@@ -38,6 +38,7 @@ QUIRK_SVO_UNTEXTURED_BLACK = 1 << 5
 QUIRK_ALL = 0x3F
 ADJOINT_RECOMPUTE = 1
 REVOX_PER_LEVEL = 4  # VCT_REVOX_PER_LEVEL: regions per level written by vct_move_clip_regions
+BATCH_PHASES = 12  # VCT_BATCH_PHASES: need[] entries per scene written by vct_batch_plan
 
 MODE_DIFFUSE = 0
 MODE_SPECULAR = 1
@@ -262,6 +263,15 @@ class BatchTuning(C.Structure):
     ]
 
 
+class BatchSceneShape(C.Structure):
+    _fields_ = [
+        ("triangles", C.c_uint32),
+        ("resolution", C.c_uint32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+    ]
+
+
 STRUCT_SIZES = {
     "vct_config": (Config, 32),
     "vct_clipmap_region": (ClipmapRegion, 28),
@@ -281,6 +291,7 @@ STRUCT_SIZES = {
     "vct_gbuffer": (GBuffer, 48),
     "vct_batch_scene": (BatchScene, 336),
     "vct_batch_tuning": (BatchTuning, 16),
+    "vct_batch_scene_shape": (BatchSceneShape, 16),
     "vct_specular_filter_desc": (SpecularFilterDesc, 16),
 }
 
@@ -362,6 +373,8 @@ _PROTOS = {
     "vct_batch_level": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
                                   C.POINTER(C.c_uint32)]),
     "vct_batch_read_level": (C.c_int, [P, P, C.c_uint32, C.c_uint32, P, C.c_size_t]),
+    "vct_batch_plan": (C.c_int, [C.POINTER(BatchSceneShape), C.c_uint32, C.POINTER(BatchTuning), P, C.c_size_t,
+                                 C.POINTER(C.c_size_t), P]),
     "vct_specular_filter": (C.c_int, [P, P, P, C.c_uint32, C.c_uint32, C.POINTER(SpecularFilterDesc), P]),
     "vct_volume_bytes": (C.c_int, [P, C.c_int, C.POINTER(C.c_size_t)]),
     "vct_read_volume": (C.c_int, [P, P, C.c_int, P, C.c_size_t]),

@@ -42,6 +42,7 @@
 
 static_assert(sizeof(vct_batch_tuning) == 16, "vct_batch_tuning layout");
 static_assert(sizeof(vct_batch_scene) == 336, "vct_batch_scene layout");
+static_assert(sizeof(vct_batch_scene_shape) == 16, "vct_batch_scene_shape layout");
 
 namespace vct {
 namespace batch {
@@ -50,6 +51,7 @@ enum TaskType : uint32_t { kVertex = 0, kRaster = 1, kMip = 2, kTrace = 3 };
 
 // Phase p of a scene: 0 vertex, 1 raster, 1 + m mip level m (m >= 1).
 constexpr int kPhases = 1 + dense::kMaxLevels;
+static_assert(kPhases == VCT_BATCH_PHASES, "phases per scene");
 constexpr int kTile = 16;                    // trace tile edge (pixels)
 constexpr double kWaitSeconds = 2.0;         // a dependency wait gives up (and aborts the launch) after this
 
@@ -481,45 +483,72 @@ static void drop_graph(vct_batch* b)
     b->graph_valid = false;
 }
 
-// Task table for the current tuning. Within a phase, larger work first for
-// voxelization and mips; trace tiles of the scenes whose mips finish first
-// (smaller grids) first.
-static vct_status build_tasks(vct_batch* b)
+// The shape of one scene as the planner sees it.
+struct PlanScene {
+    uint32_t tri_count, res, width, height;
+};
+
+// Pyramid levels of a res^3 grid: log2(res) + 1.
+static int plan_levels(uint32_t res)
 {
-    const uint32_t ns = (uint32_t)b->scenes.size();
+    int nl = 0;
+    while ((1u << nl) < res) ++nl;
+    return nl + 1;
+}
+
+// Task table for raster chunk rc and mip brick B, and the task count of every
+// (scene, phase). Phases in queue order: vertex chunks of every scene, raster
+// chunks, mip level 1 of every scene, level 2, ..., pixel tiles. Within a
+// phase, larger work first for voxelization and mips; trace tiles of the
+// scenes whose mips finish first (smaller grids) first. Pure host code.
+static void plan_tasks(const PlanScene* sc, uint32_t ns, uint32_t rc, uint32_t B, std::vector<Task>& t,
+                       std::vector<uint32_t>& need)
+{
     std::vector<uint32_t> by_tris(ns), by_res_desc(ns), by_res_asc(ns);
     for (uint32_t i = 0; i < ns; ++i) by_tris[i] = by_res_desc[i] = by_res_asc[i] = i;
     std::stable_sort(by_tris.begin(), by_tris.end(),
-                     [&](uint32_t x, uint32_t y) { return b->dev[x].tri_count > b->dev[y].tri_count; });
-    std::stable_sort(by_res_desc.begin(), by_res_desc.end(),
-                     [&](uint32_t x, uint32_t y) { return b->dev[x].res > b->dev[y].res; });
+                     [&](uint32_t x, uint32_t y) { return sc[x].tri_count > sc[y].tri_count; });
+    std::stable_sort(by_res_desc.begin(), by_res_desc.end(), [&](uint32_t x, uint32_t y) { return sc[x].res > sc[y].res; });
     std::stable_sort(by_res_asc.begin(), by_res_asc.end(), [&](uint32_t x, uint32_t y) {
-        return b->dev[x].res != b->dev[y].res ? b->dev[x].res < b->dev[y].res
-                                               : b->dev[x].tri_count < b->dev[y].tri_count;
+        return sc[x].res != sc[y].res ? sc[x].res < sc[y].res : sc[x].tri_count < sc[y].tri_count;
     });
-    std::vector<Task> t;
-    for (auto& d : b->dev) std::fill(std::begin(d.need), std::end(d.need), 0u);
-    const uint32_t rc = b->tuning.raster_chunk, B = b->tuning.mip_brick;
+    t.clear();
+    need.assign((size_t)ns * kPhases, 0u);
     for (uint32_t s : by_tris)
-        for (uint32_t a = 0; a < b->dev[s].tri_count; a += 256, ++b->dev[s].need[0])
-            t.push_back({(kVertex << 24) | s, a, std::min(a + 256u, b->dev[s].tri_count), 0});
+        for (uint32_t a = 0; a < sc[s].tri_count; a += 256, ++need[(size_t)s * kPhases + 0])
+            t.push_back({(kVertex << 24) | s, a, std::min(a + 256u, sc[s].tri_count), 0});
     for (uint32_t s : by_tris)
-        for (uint32_t a = 0; a < b->dev[s].tri_count; a += rc, ++b->dev[s].need[1])
-            t.push_back({(kRaster << 24) | s, a, std::min(a + rc, b->dev[s].tri_count), 0});
+        for (uint32_t a = 0; a < sc[s].tri_count; a += rc, ++need[(size_t)s * kPhases + 1])
+            t.push_back({(kRaster << 24) | s, a, std::min(a + rc, sc[s].tri_count), 0});
     int max_levels = 0;
-    for (auto& d : b->dev) max_levels = std::max(max_levels, d.pyr.nlevels);
+    for (uint32_t s = 0; s < ns; ++s) max_levels = std::max(max_levels, plan_levels(sc[s].res));
     for (int m = 1; m < max_levels; ++m)
         for (uint32_t s : by_res_desc) {
-            SceneDev& d = b->dev[s];
-            if (m >= d.pyr.nlevels) continue;
-            const uint32_t n = (uint32_t)(d.res >> m), be = std::min(n, B), nb = (n + be - 1) / be;
-            for (uint32_t k = 0; k < nb * nb * nb; ++k, ++d.need[1 + m]) t.push_back({(kMip << 24) | s, k, 0, (uint32_t)m});
+            if (m >= plan_levels(sc[s].res)) continue;
+            const uint32_t n = sc[s].res >> m, be = std::min(n, B), nb = (n + be - 1) / be;
+            for (uint32_t k = 0; k < nb * nb * nb; ++k, ++need[(size_t)s * kPhases + 1 + m])
+                t.push_back({(kMip << 24) | s, k, 0, (uint32_t)m});
         }
     for (uint32_t s : by_res_asc) {
-        const SceneDev& d = b->dev[s];
-        const uint32_t tiles = (uint32_t)d.tiles_x * (uint32_t)((d.height + kTile - 1) / kTile);
+        const uint32_t tiles = ((sc[s].width + kTile - 1) / kTile) * ((sc[s].height + kTile - 1) / kTile);
         for (uint32_t k = 0; k < tiles; ++k) t.push_back({(kTrace << 24) | s, k, 0, 0});
     }
+}
+
+// Task table for the current tuning, uploaded with the scene table (whose
+// need[] counts it sets).
+static vct_status build_tasks(vct_batch* b)
+{
+    const uint32_t ns = (uint32_t)b->scenes.size();
+    std::vector<PlanScene> shapes(ns);
+    for (uint32_t i = 0; i < ns; ++i)
+        shapes[i] = {b->dev[i].tri_count, (uint32_t)b->dev[i].res, (uint32_t)b->dev[i].width,
+                     (uint32_t)b->dev[i].height};
+    std::vector<Task> t;
+    std::vector<uint32_t> need;
+    plan_tasks(shapes.data(), ns, b->tuning.raster_chunk, b->tuning.mip_brick, t, need);
+    for (uint32_t i = 0; i < ns; ++i)
+        for (int p = 0; p < kPhases; ++p) b->dev[i].need[p] = need[(size_t)i * kPhases + p];
     b->ntasks = (uint32_t)t.size();
     b->phases.clear();
     for (uint32_t i = 0; i < b->ntasks; ++i) {
@@ -659,6 +688,17 @@ static vct_status batch_init(vct_batch* b, const vct_batch_scene* scenes, uint32
         if (!sc.gbuffer.diffuse || !sc.gbuffer.normal || !sc.gbuffer.depth || !sc.gbuffer.width || !sc.gbuffer.height ||
             !sc.out)
             return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u: G-buffer and output are required", i);
+        {
+            std::vector<uint8_t> textured(m.material_count ? m.material_count : 1);
+            VCT_TRY(read_textured_flags(nullptr, m.materials, m.material_count, textured.data()));
+            for (uint32_t k = 0; k < m.primitive_count; ++k)
+                if (m.primitives[k].index_count && m.primitives[k].material_index < m.material_count &&
+                    textured[m.primitives[k].material_index])
+                    return fail(VCT_ERR_UNSUPPORTED,
+                                "scene %u primitive %u: the material references a base-colour, emissive or "
+                                "occlusion texture; the voxel passes use material factors only",
+                                i, k);
+        }
         SceneDev& d = b->dev[i];
         std::memset(&d, 0, sizeof(d));
         d.positions = m.positions;
@@ -822,7 +862,40 @@ vct_status vct_batch_status(vct_batch* b)
     if (st) {
         const uint32_t zero = 0;
         VCT_HIP(hipMemcpy((uint32_t*)b->counters.ptr + 2, &zero, 4, hipMemcpyHostToDevice));
+        // an aborted launch may have skipped the mip level-1 tasks that return
+        // the winners to empty; the next launch needs them empty
+        size_t winner_total = 0;
+        for (const SceneDev& d : b->dev) winner_total += (size_t)d.res * d.res * d.res;
+        VCT_HIP(hipMemset(b->winner.ptr, 0xff, 4u * winner_total));
+        VCT_HIP(hipDeviceSynchronize());
         return fail(VCT_ERR_HIP, "a batched task gave up waiting on its dependency; results are incomplete");
+    }
+    return VCT_OK;
+}
+
+vct_status vct_batch_plan(const vct_batch_scene_shape* shapes, uint32_t count, const vct_batch_tuning* tuning,
+                          uint32_t* tasks, size_t cap_tasks, size_t* n_tasks, uint32_t* need)
+{
+    if (!shapes || !count || !tuning || !n_tasks) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (tuning->raster_chunk != 64 && tuning->raster_chunk != 128 && tuning->raster_chunk != 256)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "raster_chunk must be 64, 128 or 256");
+    if (tuning->mip_brick != 4 && tuning->mip_brick != 8) return fail(VCT_ERR_INVALID_ARGUMENT, "mip_brick must be 4 or 8");
+    std::vector<PlanScene> sc(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t r = shapes[i].resolution;
+        if (r < 2 || r > 1024 || (r & (r - 1)))
+            return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u: resolution must be a power of two in [2, 1024]", i);
+        if (!shapes[i].width || !shapes[i].height) return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u: image size", i);
+        sc[i] = {shapes[i].triangles, r, shapes[i].width, shapes[i].height};
+    }
+    std::vector<Task> t;
+    std::vector<uint32_t> nd;
+    plan_tasks(sc.data(), count, tuning->raster_chunk, tuning->mip_brick, t, nd);
+    *n_tasks = t.size();
+    if (need) std::memcpy(need, nd.data(), sizeof(uint32_t) * nd.size());
+    if (tasks) {
+        if (cap_tasks < t.size()) return fail(VCT_ERR_CAPACITY, "the plan holds %zu tasks", t.size());
+        std::memcpy(tasks, t.data(), sizeof(Task) * t.size());
     }
     return VCT_OK;
 }
@@ -854,14 +927,17 @@ vct_status vct_batch_autotune(vct_batch* b, vct_stream stream, uint32_t reps, vc
         return vct_batch_status(b);
     };
     // coordinate descent over the four knobs, two sweeps
-    static const uint32_t chunks[] = {64, 128, 256}, bricks[] = {4, 8}, scheds[] = {0, 1}, occ[] = {0, 1, 2, 4};
+    // the persistent schedules (workgroups_per_cu 1, 2, 4) are not candidates:
+    // measured at a third of the phased schedule's speed; vct_batch_set_tuning
+    // still selects them
+    static const uint32_t chunks[] = {64, 128, 256}, bricks[] = {4, 8}, scheds[] = {0, 1}, occ[] = {0};
     vct_batch_tuning cur = b->tuning;
     float cur_ms = 0.0f;
     vct_status st = time_it(cur, &cur_ms);
     for (int sweep = 0; st == VCT_OK && sweep < 2; ++sweep) {
         for (int knob = 0; st == VCT_OK && knob < 4; ++knob) {
             const uint32_t* vals = knob == 0 ? chunks : knob == 1 ? bricks : knob == 2 ? scheds : occ;
-            const int nv = knob == 0 ? 3 : knob == 1 ? 2 : knob == 2 ? 2 : 4;
+            const int nv = knob == 0 ? 3 : knob == 1 ? 2 : knob == 2 ? 2 : 1;
             for (int k = 0; st == VCT_OK && k < nv; ++k) {
                 vct_batch_tuning t = cur;
                 uint32_t* field = knob == 0 ? &t.raster_chunk : knob == 1 ? &t.mip_brick

@@ -136,12 +136,64 @@ __global__ void k_vertex_stage(const float* __restrict__ positions, const float*
     out_mat[t] = pr.material_index;
 }
 
+vct_status read_textured_flags(hipStream_t s, const vct_gltf_material* mats, uint32_t count, uint8_t* flags)
+{
+    if (!count) return VCT_OK;
+    vct_gltf_material* host = new (std::nothrow) vct_gltf_material[count];
+    if (!host) return fail(VCT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    hipError_t e = hipMemcpyAsync(host, mats, sizeof(vct_gltf_material) * count, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        delete[] host;
+        return fail(VCT_ERR_HIP, "reading the material table failed: %s", hipGetErrorString(e));
+    }
+    // msaaVoxelizer.frag:64, msaaInjectRadiance.frag:73-82,132-135 and
+    // voxelizer.frag:52-76 sample these three textures
+    for (uint32_t i = 0; i < count; ++i)
+        flags[i] = host[i].base_color_texture > -1 || host[i].emissive_texture > -1 || host[i].occlusion_texture > -1;
+    delete[] host;
+    return VCT_OK;
+}
+
+static vct_status check_untextured(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh)
+{
+    if (ctx->mat_table != (const void*)mesh->materials || ctx->mat_count != mesh->material_count) {
+        delete[] ctx->mat_textured;
+        ctx->mat_textured = nullptr;
+        ctx->mat_table = nullptr;
+        ctx->mat_count = 0;
+        uint8_t* flags = new (std::nothrow) uint8_t[mesh->material_count ? mesh->material_count : 1];
+        if (!flags) return fail(VCT_ERR_OUT_OF_MEMORY, "host allocation failed");
+        vct_status st = read_textured_flags(s, mesh->materials, mesh->material_count, flags);
+        if (st != VCT_OK) {
+            delete[] flags;
+            return st;
+        }
+        ctx->mat_textured = flags;
+        ctx->mat_table = mesh->materials;
+        ctx->mat_count = mesh->material_count;
+    }
+    for (uint32_t i = 0; i < mesh->primitive_count; ++i) {
+        const vct_primitive& p = mesh->primitives[i];
+        if (p.index_count && p.material_index < ctx->mat_count && ctx->mat_textured[p.material_index])
+            return fail(VCT_ERR_UNSUPPORTED,
+                        "primitive %u: material %u references a base-colour, emissive or occlusion texture; the "
+                        "voxel passes use material factors only",
+                        i, p.material_index);
+    }
+    return VCT_OK;
+}
+
 vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, TriSoup* out)
 {
     if (!mesh || !mesh->positions || !mesh->normals || !mesh->indices || !mesh->node_matrices || !mesh->materials)
         return fail(VCT_ERR_INVALID_ARGUMENT, "mesh buffers must be non-null");
     if (!mesh->primitives && mesh->primitive_count)
         return fail(VCT_ERR_INVALID_ARGUMENT, "mesh primitive list is null");
+    for (uint32_t i = 0; i < mesh->primitive_count; ++i)
+        if (mesh->primitives[i].material_index >= mesh->material_count)
+            return fail(VCT_ERR_INVALID_ARGUMENT, "primitive %u material out of range", i);
+    VCT_TRY(check_untextured(ctx, s, mesh));
     // host side of GLTFScene::cmdDraw: flatten the draws
     static thread_local DevPrim* hp = nullptr;
     static thread_local size_t hp_cap = 0;
@@ -311,6 +363,7 @@ vct_status vct_destroy(vct_ctx* c)
     for (DeviceBuffer* b : bufs) release(*b);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->pinned_word) (void)hipHostFree(c->pinned_word);
+    delete[] c->mat_textured;
     delete c;
     return VCT_OK;
 }

@@ -99,6 +99,12 @@ struct vct_ctx {
     void* pinned_word = nullptr;  // pinned landing slot of small device-to-host reads
 
     vct::SvoState svo;
+
+    // Which materials of the last material table read back reference a
+    // texture (read once per change of the table's pointer or count).
+    const void* mat_table = nullptr;
+    uint32_t mat_count = 0;
+    uint8_t* mat_textured = nullptr;
 };
 
 namespace vct {
@@ -143,7 +149,14 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n)
 }
 
 // Mesh vertex stage (msaaVoxelizer.vert:31-37): world-space triangles.
+// Refuses (VCT_ERR_UNSUPPORTED) a draw list whose materials reference a
+// base-colour, emissive or occlusion texture: the kernels use the material
+// factors only.
 vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, TriSoup* out);
+
+// Host copy of a device material table's "references a texture the voxel
+// passes sample" flags (base colour, emissive or occlusion), one per material.
+vct_status read_textured_flags(hipStream_t s, const vct_gltf_material* mats, uint32_t count, uint8_t* flags);
 
 // Pinned staging copy of a small host array to a device buffer (stream ordered).
 vct_status upload_small(vct_ctx* ctx, hipStream_t s, DeviceBuffer& dst, const void* src, size_t bytes);

@@ -314,13 +314,22 @@ def batch(seed: int = 3, count: int = 64, tri_min: int = 5_000, tri_max: int = 5
     mip pyramid needs a power of two). Each scene is the room with as many of
     its 16 spheres as fit the budget, topped up with thin triangles.
     Returns a list of (Scene, resolution)."""
-    rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     room_fixed = 12 + 64 * 12
-    for i in range(count):
-        tris = int(round(np.exp(rng.uniform(np.log(tri_min), np.log(tri_max)))))
-        res = 1 << int(round(rng.uniform(6.0, 8.0)))
+    for i, (tris, res) in enumerate(batch_sizes(seed, count, tri_min, tri_max)):
         spheres = int(min(16, max(0, (tris - room_fixed) // 1280)))
         scene = room(10_000 + 100 * seed + i, scale=1.0, n_boxes=64, n_spheres=spheres, thin_total=tris)
         out.append((scene, res))
+    return out
+
+
+def batch_sizes(seed: int = 3, count: int = 64, tri_min: int = 5_000, tri_max: int = 500_000):
+    """The (triangle count, grid resolution) pairs of batch() without building
+    the geometry."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        tris = int(round(np.exp(rng.uniform(np.log(tri_min), np.log(tri_max)))))
+        res = 1 << int(round(rng.uniform(6.0, 8.0)))
+        out.append((tris, res))
     return out
