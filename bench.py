@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=30, help="rows of the 1080p image traced by the CPU baseline")
+    ap.add_argument("--cpu-rows", type=int, default=32, help="rows of the 1080p image traced by the CPU baseline")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the plain-VXAO, SVO, batched and adjoint workloads reported beside the frame")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -31,6 +31,14 @@ typedef struct {
     float light_dir[3];
 } svo_ctx;
 
+/* voxelizer.frag:99-100: x | y << 12 | (z & 0xff) << 24 and
+ * (z >> 8) << 28 | (RGBA8 & 0x0fffffff) */
+static void pack_fragment(const uint32_t vp[3], uint32_t rgba8, uint32_t out[2])
+{
+    out[0] = vp[0] | (vp[1] << 12u) | ((vp[2] & 0xffu) << 24u);
+    out[1] = ((vp[2] >> 8u) << 28u) | (rgba8 & 0x0fffffffu);
+}
+
 static void svo_frag(void* user, uint32_t tri, const float pos[3], const double lam[3])
 {
     svo_ctx* c = (svo_ctx*)user;
@@ -71,8 +79,7 @@ static void svo_frag(void* user, uint32_t tri, const float pos[3], const double 
         vp[k] = u;
     }
     float v255[4] = {rad[0] * 255.0f, rad[1] * 255.0f, rad[2] * 255.0f, rad[3] * 255.0f};
-    c->frags[2u * cur + 0] = vp[0] | (vp[1] << 12u) | ((vp[2] & 0xffu) << 24u);
-    c->frags[2u * cur + 1] = ((vp[2] >> 8u) << 28u) | (orc_pack_rgba8(v255) & 0x0fffffffu);
+    pack_fragment(vp, orc_pack_rgba8(v255), c->frags + 2u * cur);
 }
 
 uint32_t orc_svo_fragments(const orc_tris* tris, const orc_material* mats, const orc_shadow* sm,
@@ -206,4 +213,16 @@ int orc_svo_build(const uint32_t* frags, uint32_t nfrag, int level, uint32_t qui
     }
     if (node_count) *node_count = total;
     return 0;
+}
+
+/* test hooks: the fragment word pair of voxelizer.frag:99-100 and the
+ * position decode of octreeNodeFlag.comp:28-32 over arrays */
+void orc_svo_pack_fragments(const uint32_t* xyz, const uint32_t* rgba8, uint32_t n, uint32_t* out)
+{
+    for (uint32_t i = 0; i < n; ++i) pack_fragment(xyz + 3u * i, rgba8[i], out + 2u * i);
+}
+
+void orc_svo_unpack_fragments(const uint32_t* frags, uint32_t n, uint32_t quirks, uint32_t* xyz)
+{
+    for (uint32_t i = 0; i < n; ++i) frag_pos(frags + 2u * i, quirks, xyz + 3u * i);
 }

@@ -313,6 +313,13 @@ void orc_svo_trace(const uint32_t* pool, const orc_gbuffer* g, const orc_trace_p
     if (samples) *samples = total;
 }
 
+/* test hook: one sampleSVO of voxelConeTracing_Octree.frag:319-346 */
+void orc_sample_svo(const uint32_t* pool, const orc_trace_params* p, const float bb_min[3], const float bb_max[3],
+                    const float pos[3], uint32_t level, float out[4])
+{
+    sample_svo(pool, p, bb_min, bb_max, pos, level, out);
+}
+
 /* test hook: one sampleClipmap of voxelConeTracing.frag:313-327 */
 void orc_sample_clipmap(const orc_volume* vol, const orc_trace_params* p, const float pos[3], int level,
                         const float dir[3], float out[4])

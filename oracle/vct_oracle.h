@@ -190,6 +190,12 @@ uint32_t orc_svo_fragments(const orc_tris* tris, const orc_material* mats, const
                            int resolution, int samples, uint32_t quirks, uint32_t* frags, uint32_t cap);
 int orc_svo_build(const uint32_t* frags, uint32_t nfrag, int level, uint32_t quirks,
                   uint32_t* pool, uint32_t pool_cap, uint32_t* node_count);
+/* test hooks: fragment words (voxelizer.frag:99-100) and their position decode
+ * (octreeNodeFlag.comp:28-32, >> 1 with ORC_QUIRK_SVO_HALF_COORDS); one sampleSVO */
+void orc_svo_pack_fragments(const uint32_t* xyz, const uint32_t* rgba8, uint32_t n, uint32_t* out);
+void orc_svo_unpack_fragments(const uint32_t* frags, uint32_t n, uint32_t quirks, uint32_t* xyz);
+void orc_sample_svo(const uint32_t* pool, const orc_trace_params* p, const float bb_min[3], const float bb_max[3],
+                    const float pos[3], uint32_t level, float out[4]);
 void orc_svo_trace(const uint32_t* pool, const orc_gbuffer* g, const orc_trace_params* p,
                    const float bb_min[3], const float bb_max[3], const orc_shadow* sm,
                    const orc_light* light, uint32_t quirks, int row_begin, int row_end,
