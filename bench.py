@@ -25,7 +25,6 @@ import os
 import statistics
 import sys
 import time
-from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 # the CPU baseline's OpenMP threads stay on neighbouring cores (read when the
@@ -72,76 +71,16 @@ def copy_probe(mib: int) -> None:
 
 
 def setup():
-    import torch
-    import vk_voxel_cone_tracing_amd as V
-    from vk_voxel_cone_tracing_amd import passes, scenes
+    from vk_voxel_cone_tracing_amd import workloads
 
-    W, H, S = 1920, 1080, 16.0
-    scene = scenes.room(2, scale=S)
-    ds = V.DeviceScene(scene)
-    ctx = V.Context(resolution=128, levels=6, radiance_format=V.FORMAT_RGBA16F, quirks=V.QUIRK_ALL,
-                    raster_samples=1, downsample_region_size=10, extent_l0=16.0)
-    vp, vp_inv, eye = scenes.camera(2, W, H, scale=S)
-    ctx.update_clip_regions(eye)
-    light = scenes.light(2, scale=S)
-    ld, ls = V.light_structs(light)
-    dev = "cuda"
-    shadow_depth = torch.empty((2048, 2048), dtype=torch.float32, device=dev)
-    V._abi.call("vct_raycast_shadow", ctx._h, C.c_void_p(torch.cuda.current_stream().cuda_stream),
-                C.byref(ds.mesh), C.byref(ls), C.c_void_p(shadow_depth.data_ptr()), 2048, 2048)
-    g = dict(
-        diffuse=torch.empty((H, W), dtype=torch.int32, device=dev),
-        normal=torch.empty((H, W, 4), dtype=torch.int16, device=dev),
-        specular=torch.empty((H, W), dtype=torch.int32, device=dev),
-        emission=torch.empty((H, W, 4), dtype=torch.int16, device=dev),
-        depth=torch.empty((H, W), dtype=torch.float32, device=dev),
-    )
-    surface = torch.from_numpy(np.ascontiguousarray(scene.surface)).to(dev)
-    cam = V.camera_ubo(vp, vp_inv, eye)
-    V._abi.call("vct_raycast_gbuffer", ctx._h, C.c_void_p(torch.cuda.current_stream().cuda_stream),
-                C.byref(ds.mesh), C.c_void_p(surface.data_ptr()), C.byref(cam), C.c_void_p(g["diffuse"].data_ptr()),
-                C.c_void_p(g["normal"].data_ptr()), C.c_void_p(g["specular"].data_ptr()),
-                C.c_void_p(g["emission"].data_ptr()), C.c_void_p(g["depth"].data_ptr()), W, H)
-    # per-pixel perceptual roughness U[0.1, 0.8] in the diffuse alpha byte
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(2)
-    rough = (torch.rand((H, W), generator=gen, device=dev) * 0.7 + 0.1)
-    rough_b = torch.round(rough * 255.0).to(torch.int32)
-    g["diffuse"] = (g["diffuse"] & 0x00FFFFFF) | (rough_b << 24)
-    torch.cuda.synchronize()
-    sm = V.shadow_struct(shadow_depth)
-    gb = V.gbuffer_struct(g)
-    dout = torch.zeros((H, W, 4), dtype=torch.float32, device=dev)
-    sout = torch.zeros((H, W, 4), dtype=torch.float32, device=dev)
-    mgr = passes.RenderPassManager()
-    mgr.add_render_pass("VoxelizationPass", passes.VoxelizationPass(ctx, ds))
-    inj = passes.RadianceInjectionPass(ctx, ds, sm, ld, ls)
-    mgr.add_render_pass("RadianceInjectionPass", inj)
-    vct_pass = passes.VoxelConeTracingPass(ctx, gb, cam, sm, ld, ls, dout, sout)
-    vct_pass.samples = torch.zeros(1, dtype=torch.int64, device=dev)
-    mgr.add_render_pass("VoxelConeTracingPass", vct_pass)
-    return dict(scene=scene, ds=ds, ctx=ctx, vp=vp, vp_inv=vp_inv, eye=eye, light=light, shadow=shadow_depth, g=g,
-                mgr=mgr, inj=inj, vct=vct_pass, W=W, H=H, dout=dout, sout=sout)
+    return workloads.clipmap_frame()
 
 
 def frame(st, ev=None):
     """One GI frame; with `ev` records per-pass events."""
-    import vk_voxel_cone_tracing_amd as V
+    from vk_voxel_cone_tracing_amd import workloads
 
-    f = V.Frame()
-    st["inj"].frame_index = 0  # every clip level is due in the measured frame
-    st["vct"].samples.zero_()
-    if ev is not None:
-        ev[0].record()
-    st["mgr"].draw_single_render_pass("VoxelizationPass", f)
-    if ev is not None:
-        ev[1].record()
-    st["mgr"].draw_single_render_pass("RadianceInjectionPass", f)
-    if ev is not None:
-        ev[2].record()
-    st["mgr"].draw_single_render_pass("VoxelConeTracingPass", f)
-    if ev is not None:
-        ev[3].record()
+    workloads.clipmap_frame_run(st, ev)
 
 
 def extras() -> dict:
@@ -186,20 +125,6 @@ def _rel_l2(got, ref) -> float:
     return float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
 
 
-def _vox_desc(O, ctx, regions, lvl, res):
-    d = ctx.voxelization_desc(lvl)
-    od = O.VoxDesc()
-    od.region_cell_min = (C.c_int * 3)(*regions[lvl][0])
-    od.region_cell_extent = (C.c_int * 3)(*regions[lvl][1])
-    od.region_min = (C.c_float * 3)(*d.region_min_corner)
-    od.region_max = (C.c_float * 3)(*d.region_max_corner)
-    od.clip_max_extent = d.clip_max_extent
-    od.voxel_size = d.voxel_size
-    od.clip_level = lvl
-    od.clipmap_resolution = res
-    return od
-
-
 def cpu_baseline(st, rows: int, runs: int = 5):
     """The oracle (the CPU restatement of the reference shaders) on the
     frame's own inputs: all voxel passes of the frame (opacity voxelization
@@ -215,22 +140,13 @@ def cpu_baseline(st, rows: int, runs: int = 5):
 
     ctx = st["ctx"]
     scene = st["scene"]
-    pos, nrm, mat = scene.world_triangles()
-    L = O.lib()
-    L.orc_trace_rowlist.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
-                                    C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     RES, LV = 128, 6
-    tris = O.tris_struct(pos, nrm, mat)
     depth = st["shadow"].cpu().numpy()
-    osm = O.Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
-    ol = O.light_struct(st["light"])
-    mats = O.materials_struct(scene)
     regions = ctx.regions
-    descs = [_vox_desc(O, ctx, regions, lvl, RES) for lvl in range(LV)]
+    descs = [O.voxel_desc_from_library(ctx.voxelization_desc(lvl), regions[lvl]) for lvl in range(LV)]
     g = {k: v.cpu().numpy() for k, v in st["g"].items()}
     g = {k: (v.view(np.uint32) if v.dtype == np.int32 else (v.view(np.uint16) if v.dtype == np.int16 else v))
          for k, v in g.items()}
-    og = O.gbuf_struct(g)
     vdesc = st["vct"].desc()
     fields = dict(volume_center=list(vdesc.volume_center), rendering_mode=8, voxel_size=vdesc.voxel_size,
                   volume_dimension=128.0, trace_start_offset=1.0, indirect_diffuse_intensity=8.0,
@@ -242,42 +158,19 @@ def cpu_baseline(st, rows: int, runs: int = 5):
     row_list = np.asarray(list(range(step // 2, H, step))[:rows], dtype=np.int32)
 
     def voxel_passes():
-        opa = O.new_volume(RES, LV, 0)
-        rad = O.new_volume(RES, LV, 1)
-        vo, vr = O.vol_struct(opa, RES, LV), O.vol_struct(rad, RES, LV)
-
-        def level(lvl):  # each level writes only its own slice of the volumes
-            L.orc_voxelize_opacity(C.byref(vo), C.byref(tris), C.byref(descs[lvl]), 1, 0x3F)
-            L.orc_inject_radiance(C.byref(vr), C.byref(tris), mats, C.byref(osm), C.byref(ol), C.byref(descs[lvl]),
-                                  1, 0x3F)
-
         t0 = time.perf_counter()
-        with ThreadPoolExecutor(max_workers=LV) as ex:
-            list(ex.map(level, range(LV)))
-        for lvl in range(1, LV):
-            L.orc_downsample_opacity(C.byref(vo), (C.c_int * 3)(*regions[lvl - 1][0]), lvl, 10)
-        L.orc_wrap_border(C.byref(vo), 2, 6, LV, 1)
-        for lvl in range(LV):
-            L.orc_copy_alpha(C.byref(vr), C.byref(vo), lvl, 6)
-        for lvl in range(1, LV):
-            L.orc_downsample_radiance(C.byref(vr), (C.c_int * 3)(*regions[lvl - 1][0]), lvl, 10)
+        _, rad = O.clipmap_voxel_passes(scene, regions, descs, depth, st["light"], RES, LV, 10)
         return time.perf_counter() - t0, rad
 
-    def trace_rows(rad, rd, rs):
-        n = C.c_uint64(0)
+    def trace_rows(rad):
         t0 = time.perf_counter()
-        L.orc_trace_rowlist(C.byref(O.vol_struct(rad, RES, LV)), C.byref(og), C.byref(p), C.byref(osm), C.byref(ol),
-                            0x3F, row_list.ctypes.data, len(row_list), O.vptr(rd), O.vptr(rs), C.byref(n))
-        return time.perf_counter() - t0, n.value
+        rd, rs, n = O.trace_rows(rad, RES, LV, g, p, depth, st["light"], row_list)
+        return time.perf_counter() - t0, n, rd, rs
 
-    rd = np.zeros((H, W, 4), np.float32)
-    rs = np.zeros_like(rd)
     t_pass, t_rows = [], []
-    nsamp = 0
-    rad = None
     for run in range(runs + 1):  # run 0 is the warm-up
         tp, rad = voxel_passes()
-        tt, nsamp = trace_rows(rad, rd, rs)
+        tt, nsamp, rd, rs = trace_rows(rad)
         if run:
             t_pass.append(tp)
             t_rows.append(tt)
@@ -289,11 +182,7 @@ def cpu_baseline(st, rows: int, runs: int = 5):
     parity = dict(rows=int(len(row_list)), diffuse_rel_l2_vs_fp32=_rel_l2(dout, rd[row_list]),
                   specular_rel_l2_vs_fp32=_rel_l2(sout, rs[row_list]))
     sub = row_list[::4]
-    d64 = np.zeros((H, W, 4), np.float64)
-    s64 = np.zeros_like(d64)
-    for r in sub:
-        L.orc_trace_f64(C.byref(O.vol_struct(rad, RES, LV)), C.byref(og), C.byref(p), C.byref(osm), C.byref(ol), 0x3F,
-                        int(r), int(r) + 1, O.vptr(d64), O.vptr(s64), None)
+    d64, s64, _ = O.trace_rows(rad, RES, LV, g, p, depth, st["light"], sub, fp64=True)
     pos_sub = [int(np.where(row_list == r)[0][0]) for r in sub]
     parity.update(fp64_rows=int(len(sub)), diffuse_rel_l2_vs_fp64=_rel_l2(dout[pos_sub], d64[sub]),
                   specular_rel_l2_vs_fp64=_rel_l2(sout[pos_sub], s64[sub]),

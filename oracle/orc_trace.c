@@ -97,6 +97,20 @@ void orc_trace_rowlist(const orc_volume* rad, const orc_gbuffer* g, const orc_tr
     if (samples) *samples = total;
 }
 
+void orc_trace_rowlist_f64(const orc_volume* rad, const orc_gbuffer* g, const orc_trace_params* p,
+                           const orc_shadow* sm, const orc_light* light, uint32_t quirks, const int* rows,
+                           int nrows, double* diffuse_out, double* specular_out, uint64_t* samples)
+{
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+    for (int i = 0; i < nrows; ++i) {
+        uint64_t s = 0;
+        trace_rows_f64(rad, g, p, sm, light, quirks, rows[i], rows[i] + 1, diffuse_out, specular_out, &s);
+        total += s;
+    }
+    if (samples) *samples = total;
+}
+
 void orc_trace_f64(const orc_volume* rad, const orc_gbuffer* g, const orc_trace_params* p,
                    const orc_shadow* sm, const orc_light* light, uint32_t quirks, int row_begin,
                    int row_end, double* diffuse_out, double* specular_out, uint64_t* samples)

@@ -161,6 +161,9 @@ void orc_trace(const orc_volume* rad, const orc_gbuffer* g, const orc_trace_para
 void orc_trace_rowlist(const orc_volume* rad, const orc_gbuffer* g, const orc_trace_params* p,
                        const orc_shadow* sm, const orc_light* light, uint32_t quirks, const int* rows,
                        int nrows, float* diffuse_out, float* specular_out, uint64_t* samples);
+void orc_trace_rowlist_f64(const orc_volume* rad, const orc_gbuffer* g, const orc_trace_params* p,
+                           const orc_shadow* sm, const orc_light* light, uint32_t quirks, const int* rows,
+                           int nrows, double* diffuse_out, double* specular_out, uint64_t* samples);
 void orc_trace_f64(const orc_volume* rad, const orc_gbuffer* g, const orc_trace_params* p,
                    const orc_shadow* sm, const orc_light* light, uint32_t quirks,
                    int row_begin, int row_end, double* diffuse_out, double* specular_out,

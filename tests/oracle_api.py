@@ -129,6 +129,10 @@ def lib() -> C.CDLL:
         _L.orc_svo_build.restype = C.c_int
         _L.orc_specular_filter.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
                                            C.c_int, C.c_void_p]
+        rowlist = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int,
+                   C.c_void_p, C.c_void_p, C.c_void_p]
+        _L.orc_trace_rowlist.argtypes = rowlist
+        _L.orc_trace_rowlist_f64.argtypes = rowlist
         _L.orc_adj_create.restype = C.c_void_p
         _L.orc_adj_create.argtypes = [C.c_int, C.c_float, C.POINTER(C.c_float)]
         for fn in ("orc_adj_free", "orc_adj_mip"):
@@ -375,3 +379,78 @@ def specular_filter(diffuse, specular, gamma=2.2, exposure=0.1, tonemap=0, metho
     out = np.zeros((H, W, 4), np.float32)
     lib().orc_specular_filter(d.ctypes.data, s.ctypes.data, W, H, gamma, exposure, tonemap, method, out.ctypes.data)
     return out
+
+
+# --------------------------------------------------------------------------
+# the clipmap frame's voxel passes, as the benchmark frame runs them
+
+
+def voxel_desc_from_library(desc, region) -> VoxDesc:
+    """The oracle's descriptor for a vct_voxelization_desc and its region."""
+    d = VoxDesc()
+    mn, ext, _ = region
+    d.region_cell_min = (C.c_int * 3)(*mn)
+    d.region_cell_extent = (C.c_int * 3)(*ext)
+    d.region_min = (C.c_float * 3)(*desc.region_min_corner)
+    d.region_max = (C.c_float * 3)(*desc.region_max_corner)
+    d.clip_max_extent = desc.clip_max_extent
+    d.voxel_size = desc.voxel_size
+    d.clip_level = desc.clip_level
+    d.clipmap_resolution = desc.clipmap_resolution
+    return d
+
+
+def clipmap_voxel_passes(scene, regions, descs, shadow_depth, light, res, levels, region_size, quirks=0x3F,
+                         radiance_fmt=FMT_RGBA16F):
+    """VoxelizationPass (opacity voxelization of every level, opacity mips,
+    border wrap) and RadianceInjectionPass (injection of every level, copy
+    alpha, radiance mips) on the oracle. Voxelization and injection of the
+    levels run as one thread per level (each writes only its own level);
+    the mips and copy-alpha use OpenMP inside the oracle. Returns
+    (opacity, radiance) in the reference atlas layout."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    L = lib()
+    pos, nrm, mat = scene.world_triangles()
+    tris = tris_struct(pos, nrm, mat)
+    mats = materials_struct(scene)
+    depth = np.ascontiguousarray(shadow_depth, np.float32)
+    osm = Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
+    ol = light_struct(light)
+    opa = new_volume(res, levels, FMT_RGBA8)
+    rad = new_volume(res, levels, radiance_fmt)
+    vo, vr = vol_struct(opa, res, levels), vol_struct(rad, res, levels)
+
+    def level(lvl):
+        L.orc_voxelize_opacity(C.byref(vo), C.byref(tris), C.byref(descs[lvl]), 1, quirks)
+        L.orc_inject_radiance(C.byref(vr), C.byref(tris), mats, C.byref(osm), C.byref(ol), C.byref(descs[lvl]), 1,
+                              quirks)
+
+    with ThreadPoolExecutor(max_workers=levels) as ex:
+        list(ex.map(level, range(levels)))
+    for lvl in range(1, levels):
+        L.orc_downsample_opacity(C.byref(vo), (C.c_int * 3)(*regions[lvl - 1][0]), lvl, region_size)
+    L.orc_wrap_border(C.byref(vo), 2, 6, levels, 1 if quirks & 2 else 0)
+    for lvl in range(levels):
+        L.orc_copy_alpha(C.byref(vr), C.byref(vo), lvl, 6)
+    for lvl in range(1, levels):
+        L.orc_downsample_radiance(C.byref(vr), (C.c_int * 3)(*regions[lvl - 1][0]), lvl, region_size)
+    return opa, rad
+
+
+def trace_rows(rad, res, levels, g: dict, params, shadow_depth, light, rows, quirks=0x3F, fp64=False):
+    """orc_trace (or orc_trace_f64) over `rows` of the image, OpenMP over the
+    rows. Returns (diffuse, specular, samples); rows not listed stay zero."""
+    H, W = g["depth"].shape
+    rows = np.ascontiguousarray(rows, np.int32)
+    dt = np.float64 if fp64 else np.float32
+    d = np.zeros((H, W, 4), dt)
+    s = np.zeros((H, W, 4), dt)
+    depth = np.ascontiguousarray(shadow_depth, np.float32)
+    osm = Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
+    n = C.c_uint64(0)
+    fn = lib().orc_trace_rowlist_f64 if fp64 else lib().orc_trace_rowlist
+    fn(C.addressof(vol_struct(rad, res, levels)), C.addressof(gbuf_struct(g)), C.addressof(params),
+       C.addressof(osm), C.addressof(light_struct(light)), quirks, rows.ctypes.data, len(rows), d.ctypes.data,
+       s.ctypes.data, C.addressof(n))
+    return d, s, n.value

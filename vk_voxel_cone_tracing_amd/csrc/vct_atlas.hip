@@ -245,6 +245,7 @@ vct_status vct_clear_region(vct_ctx* c, vct_stream stream, vct_volume v, const v
         if (d->clip_max_extent[k] > (uint32_t)c->res) return fail(VCT_ERR_INVALID_ARGUMENT, "extent > resolution");
     }
     if (!d->clip_max_extent[0] || !d->clip_max_extent[1] || !d->clip_max_extent[2] || !d->face_count) return VCT_OK;
+    if (v == VCT_VOLUME_RADIANCE) radiance_written(c);
     dim3 block(64, 2, 2);
     dim3 grid((d->clip_max_extent[0] + 63) / 64, (d->clip_max_extent[1] + 1) / 2, (d->clip_max_extent[2] + 1) / 2);
     int3 minc = make_int3(d->region_min_corner[0], d->region_min_corner[1], d->region_min_corner[2]);
@@ -270,6 +271,7 @@ vct_status vct_begin_radiance_level(vct_ctx* c, vct_stream stream, uint32_t leve
     hipStream_t s = (hipStream_t)stream;
     dim3 block(64, 2, 2);
     dim3 grid((c->res + 63) / 64, (c->res + 1) / 2, (c->res + 1) / 2);
+    radiance_written(c);
     hipLaunchKernelGGL(k_begin_radiance, grid, block, 0, s, (uint2*)c->radiance.ptr, (const uint32_t*)c->opacity.ptr,
                        c->D, c->res, (int)level);
     VCT_HIP(hipGetLastError());
@@ -286,6 +288,7 @@ vct_status vct_copy_alpha(vct_ctx* c, vct_stream stream, const vct_copy_alpha_de
     dim3 block(64, 2, 2);
     dim3 grid((c->res + 63) / 64, (c->res + 1) / 2, (c->res + 1) / 2);
     hipStream_t s = (hipStream_t)stream;
+    radiance_written(c);
     if (is_half(c, VCT_VOLUME_RADIANCE))
         hipLaunchKernelGGL(k_copy_alpha<true>, grid, block, 0, s, (uint2*)c->radiance.ptr,
                            (const uint32_t*)c->opacity.ptr, c->D, c->res, d->clip_level, d->face_count);
@@ -308,6 +311,7 @@ vct_status vct_wrap_border(vct_ctx* c, vct_stream stream, vct_volume v, const vc
     // BorderWrapper.cpp:139-140 dispatches ((res + border) >> 3) groups of 8.
     const int n = (c->cfg.quirks & VCT_QUIRK_BORDER_SKIP_FAR) ? 8 * (rwb >> 3) : rwb;
     if (d->clip_region_count == 0 || d->face_count == 0) return VCT_OK;
+    if (v == VCT_VOLUME_RADIANCE) radiance_written(c);
     dim3 block(64, 4, 1);
     dim3 grid((n + 63) / 64, (n + 3) / 4, 6u * d->clip_region_count * d->face_count);
     hipStream_t s = (hipStream_t)stream;
@@ -336,6 +340,7 @@ vct_status vct_downsample(vct_ctx* c, vct_stream stream, vct_volume v, const vct
     dim3 grid((half + block.x - 1) / block.x, (half + 1) / 2, (half + 1) / 2);
     int3 pm = make_int3(d->prev_region_min_corner[0], d->prev_region_min_corner[1], d->prev_region_min_corner[2]);
     hipStream_t s = (hipStream_t)stream;
+    if (v == VCT_VOLUME_RADIANCE) radiance_written(c);
     if (v == VCT_VOLUME_OPACITY)
         hipLaunchKernelGGL((k_downsample<false, false>), grid, block, 0, s, (uint32_t*)c->opacity.ptr, c->D, c->res,
                            d->clip_level, pm, d->downsample_region_size);

@@ -329,7 +329,15 @@ vct_status vct_create(vct_ctx** out_ctx, const vct_config* cfg)
     if (c->levels) s = ensure(c->opacity, texels_per_volume(c) * 4u);
     if (s == VCT_OK && c->levels) s = ensure(c->radiance, texels_per_volume(c) * texel_bytes(c, VCT_VOLUME_RADIANCE));
     if (s == VCT_OK && c->levels) {
-        hipError_t e = hipMemset(c->opacity.ptr, 0, texels_per_volume(c) * 4u);
+        const uint32_t D = (uint32_t)c->D, D3 = D * D * D;
+        c->occ_words = ((D3 + 63u) / 64u) * 2u;
+        c->occ_pad = (D * D + D + 1u) / 32u + 3u;
+        s = ensure(c->occ_raw, 4u * (size_t)c->levels * (c->occ_words + c->occ_pad));
+        if (s == VCT_OK) s = ensure(c->occ_mask, 4u * (size_t)c->levels * c->occ_words);
+    }
+    if (s == VCT_OK && c->levels) {
+        hipError_t e = hipMemset(c->occ_raw.ptr, 0, c->occ_raw.bytes);
+        if (e == hipSuccess) e = hipMemset(c->opacity.ptr, 0, texels_per_volume(c) * 4u);
         if (e == hipSuccess) e = hipMemset(c->radiance.ptr, 0, texels_per_volume(c) * texel_bytes(c, VCT_VOLUME_RADIANCE));
         if (e == hipSuccess) e = hipDeviceSynchronize();
         if (e != hipSuccess) s = fail(VCT_ERR_HIP, "initial clear failed: %s", hipGetErrorString(e));
@@ -356,6 +364,8 @@ vct_status vct_destroy(vct_ctx* c)
     (void)hipDeviceSynchronize();
     release(c->opacity);
     release(c->radiance);
+    release(c->occ_raw);
+    release(c->occ_mask);
     DeviceBuffer* bufs[] = {&c->soup_pos, &c->soup_nrm, &c->soup_mat, &c->prims, &c->prim_first, &c->counts,
                             &c->offsets, &c->scan_tmp, &c->totals, &c->rec_key[0], &c->rec_key[1], &c->rec_idx[0],
                             &c->rec_idx[1], &c->rec_val, &c->sort_tmp, &c->staging, &c->materials_flat,
@@ -526,6 +536,7 @@ vct_status vct_clear_volume(vct_ctx* c, vct_stream stream, vct_volume v)
         return fail(VCT_ERR_INVALID_ARGUMENT, "bad argument");
     DeviceBuffer& b = v == VCT_VOLUME_OPACITY ? c->opacity : c->radiance;
     VCT_HIP(hipMemsetAsync(b.ptr, 0, texels_per_volume(c) * texel_bytes(c, v), (hipStream_t)stream));
+    if (v == VCT_VOLUME_RADIANCE) radiance_written(c);
     return VCT_OK;
 }
 
@@ -559,6 +570,7 @@ static vct_status convert(vct_ctx* c, hipStream_t s, vct_volume v, void* host, s
         VCT_HIP(hipMemcpyAsync(host, c->staging.ptr, n * tb, hipMemcpyDeviceToHost, s));
         VCT_HIP(hipStreamSynchronize(s));
     } else {
+        if (v == VCT_VOLUME_RADIANCE) radiance_written(c);
         VCT_HIP(hipMemcpyAsync(c->staging.ptr, host, n * tb, hipMemcpyHostToDevice, s));
         if (tb == 4)
             hipLaunchKernelGGL(k_from_atlas<uint32_t>, dim3(blocks), dim3(threads), 0, s,

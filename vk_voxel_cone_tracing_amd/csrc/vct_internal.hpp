@@ -100,6 +100,16 @@ struct vct_ctx {
 
     vct::SvoState svo;
 
+    // Occupancy of the radiance volume for the trace's empty-space skip:
+    // occ_raw holds one bit per texel position of a level (any face nonzero),
+    // occ_words words per level followed by occ_pad zero words; occ_mask the
+    // same bits dilated to the 2x2x2 footprint (occ_words per level). Every
+    // entry point that writes radiance bumps radiance_version; the trace
+    // rebuilds the mask when mask_version differs.
+    vct::DeviceBuffer occ_raw, occ_mask;
+    uint32_t occ_words = 0, occ_pad = 0;
+    uint64_t radiance_version = 1, mask_version = 0;
+
     // Which materials of the last material table read back reference a
     // texture (read once per change of the table's pointer or count).
     const void* mat_table = nullptr;
@@ -108,6 +118,9 @@ struct vct_ctx {
 };
 
 namespace vct {
+
+// Records a write to the radiance volume (the occupancy mask is stale).
+inline void radiance_written(vct_ctx* c) { ++c->radiance_version; }
 
 inline size_t texels_per_volume(const vct_ctx* c)
 {

@@ -74,6 +74,11 @@ struct TraceParams {
     float inv_voxel_size;  // 1 / voxel_size
     float inv_min_radius;  // 1 / (voxel_size * dim / 2)
     uint32_t vol_bytes;    // size of the radiance volume (buffer-load range)
+    // occupancy of the radiance volume: bit (z*D + y)*D + x of level m's
+    // occ_words words is set when the 2x2x2 footprint with that corner holds
+    // a nonzero texel in any face
+    const uint32_t* occ;
+    uint32_t occ_words;
 };
 
 template <bool kHalf>
@@ -100,19 +105,15 @@ struct Vol<true> {
 typedef unsigned int vct_u4 __attribute__((ext_vector_type(4)));
 typedef unsigned int vct_u2 __attribute__((ext_vector_type(2)));
 
-// Trilinear sample of the three selected faces of one level, weighted.
-// Texels are fetched with buffer loads: a 32-bit byte offset per lane (the
-// volume is far below 4 GiB) and the y / z / yz neighbour strides as scalar
-// offsets, so the 2x2x2 footprint of a face costs one offset computation and
-// four loads of a texel pair (x, x + 1) each. A fetch past the end of the
-// volume reads zero instead of faulting.
-template <bool kHalf>
-__device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P,
-                                           const float pos[3], int level, const int face[3], const float w[3])
+// The trilinear footprint of `pos` at `level`: the block coordinate
+// u = fract(p / extent) * res + 0.5 per axis, whose floor is the corner texel
+// (clamped to [0, res], so the 2x2x2 footprint stays inside the block) and
+// whose fraction is the weight. Returns the corner's texel index within a
+// (level, face) block.
+__device__ __forceinline__ uint32_t footprint(const TraceParams& P, const float pos[3], int level, float fr[3])
 {
     const float inv_extent = ldexpf(P.inv_extent0, -level);  // 1 / (voxel_size * 2^level * dim), exact
     int i0[3];
-    float fr[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float q = pos[k] * inv_extent;
@@ -123,11 +124,31 @@ __device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restr
         i = i < 0 ? 0 : (i > P.res ? P.res : i);
         i0[k] = i;
     }
+    const uint32_t D = (uint32_t)P.D;
+    return ((uint32_t)i0[2] * D + (uint32_t)i0[1]) * D + (uint32_t)i0[0];
+}
+
+// Whether the footprint with corner o000 of `level` holds a nonzero texel.
+__device__ __forceinline__ bool occupied(const TraceParams& P, int level, uint32_t o000)
+{
+    return (P.occ[(size_t)level * P.occ_words + (o000 >> 5)] >> (o000 & 31u)) & 1u;
+}
+
+// Trilinear sample of the three selected faces of one level, weighted.
+// Texels are fetched with buffer loads: a 32-bit byte offset per lane (the
+// volume is far below 4 GiB) and the y / z / yz neighbour strides as scalar
+// offsets, so the 2x2x2 footprint of a face costs one offset computation and
+// four loads of a texel pair (x, x + 1) each. A fetch past the end of the
+// volume reads zero instead of faulting.
+template <bool kHalf>
+__device__ __forceinline__ f4 sample_level(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P,
+                                           int level, uint32_t o000, const float fr[3], const int face[3],
+                                           const float w[3])
+{
     constexpr uint32_t TB = kHalf ? 8u : 4u;  // bytes per texel
     const __amdgpu_buffer_rsrc_t R =
         __builtin_amdgcn_make_buffer_rsrc((void*)vol, (short)0, (int)P.vol_bytes, 0x00020000);
     const uint32_t D = (uint32_t)P.D;
-    const uint32_t o000 = ((uint32_t)i0[2] * D + (uint32_t)i0[1]) * D + (uint32_t)i0[0];
     const int sy = (int)(D * TB), sz = (int)(D * D * TB), syz = sy + sz;
     const float ax = fr[0], ay = fr[1], az = fr[2];
     const float w000 = (1.0f - ax) * (1.0f - ay) * (1.0f - az), w100 = ax * (1.0f - ay) * (1.0f - az);
@@ -202,6 +223,11 @@ __device__ __forceinline__ float cone_level(const TraceParams& P, const float po
     return fminf(fmaxf(fmaxf(start_level, cur), min_level), max_level);
 }
 
+// sampleClipmapLinear (voxelConeTracing.frag:329-339). When no lane of the
+// wave has a nonzero texel in its footprints, every lane's sample is exactly
+// zero (a sum of zero texels), which the march accumulates as no change at
+// all (opacity 1 - (1 - 0)^c = 0, colour + 0), so the fetches and the
+// filtering are skipped and zero is returned.
 template <bool kHalf>
 __device__ __forceinline__ f4 cone_sample(const typename Vol<kHalf>::T* __restrict__ vol, const TraceParams& P,
                                           const float pos[3], float cur, const int face[3], const float w[3])
@@ -209,9 +235,19 @@ __device__ __forceinline__ f4 cone_sample(const typename Vol<kHalf>::T* __restri
     const int lower = (int)floorf(cur);
     const int upper = (int)ceilf(cur);
     const float fl = cur - floorf(cur);
-    f4 s = sample_level<kHalf>(vol, P, pos, lower, face, w);
+    float fr_lo[3], fr_hi[3];
+    const uint32_t o_lo = footprint(P, pos, lower, fr_lo);
+    bool occ = occupied(P, lower, o_lo);
+    uint32_t o_hi = 0;
     if (upper != lower) {
-        const f4 hi = sample_level<kHalf>(vol, P, pos, upper, face, w);
+        o_hi = footprint(P, pos, upper, fr_hi);
+        occ = occ || occupied(P, upper, o_hi);
+    }
+    f4 s = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (!__any(occ)) return s;
+    s = sample_level<kHalf>(vol, P, lower, o_lo, fr_lo, face, w);
+    if (upper != lower) {
+        const f4 hi = sample_level<kHalf>(vol, P, upper, o_hi, fr_hi, face, w);
         s.x = s.x * (1.0f - fl) + hi.x * fl;
         s.y = s.y * (1.0f - fl) + hi.y * fl;
         s.z = s.z * (1.0f - fl) + hi.z * fl;
@@ -902,6 +938,75 @@ __global__ __launch_bounds__(256) void k_svo_trace(const uint2* __restrict__ poo
     }
 }
 
+// ---- occupancy mask of the radiance volume (the trace's empty-space skip)
+
+__device__ __forceinline__ bool texel_nonzero(uint32_t t) { return t != 0u; }
+__device__ __forceinline__ bool texel_nonzero(uint2 t) { return (t.x | t.y) != 0u; }
+
+// Bit i of level blockIdx.y: texel position i = (z*D + y)*D + x of the level
+// holds a nonzero texel in any of the 6 faces (one lane per position, the
+// wave's ballot stored as two words; positions past D^3 are zero).
+template <typename T>
+__global__ __launch_bounds__(256) void k_occ_bits(const T* __restrict__ vol, uint32_t D3, uint32_t words,
+                                                  uint32_t raw_stride, uint32_t* __restrict__ raw)
+{
+    const uint32_t level = blockIdx.y;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    bool nz = false;
+    if (i < D3) {
+        const size_t base = (size_t)level * kFaces * D3 + i;
+#pragma unroll
+        for (int f = 0; f < kFaces; ++f) nz = nz || texel_nonzero(vol[base + (size_t)f * D3]);
+    }
+    const unsigned long long b = __ballot(nz);
+    const uint32_t lane = threadIdx.x & 63u, w = i >> 5;
+    if ((lane == 0u || lane == 32u) && w < words)
+        raw[(size_t)level * raw_stride + w] = lane == 0u ? (uint32_t)b : (uint32_t)(b >> 32);
+}
+
+// 32 bits of a bit array starting at bit b.
+__device__ __forceinline__ uint32_t bits_at(const uint32_t* __restrict__ r, uint32_t b)
+{
+    const uint32_t w = b >> 5, sh = b & 31u;
+    const uint32_t lo = r[w] >> sh;
+    return sh ? (lo | (r[w + 1] << (32u - sh))) : lo;
+}
+
+// mask bit i = OR of the raw bits at i + {0, 1} + {0, D} + {0, D^2}: the
+// 2x2x2 footprint with corner i. Reads stay within the level's words plus its
+// zero padding words.
+__global__ __launch_bounds__(256) void k_occ_dilate(const uint32_t* __restrict__ raw, uint32_t D, uint32_t words,
+                                                    uint32_t raw_stride, uint32_t* __restrict__ mask)
+{
+    const uint32_t level = blockIdx.y;
+    const uint32_t w = blockIdx.x * 256u + threadIdx.x;
+    if (w >= words) return;
+    const uint32_t* r = raw + (size_t)level * raw_stride;
+    const uint32_t b = w * 32u, D2 = D * D;
+    uint32_t m = 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) m |= bits_at(r, b + ((k & 4) ? D2 : 0u) + ((k & 2) ? D : 0u) + (uint32_t)(k & 1));
+    mask[(size_t)level * words + w] = m;
+}
+
+// Rebuilds the occupancy mask from the radiance volume on `s`.
+static vct_status build_occupancy(vct_ctx* c, hipStream_t s)
+{
+    const uint32_t D = (uint32_t)c->D, D3 = D * D * D;
+    const uint32_t raw_stride = c->occ_words + c->occ_pad;
+    uint32_t* raw = (uint32_t*)c->occ_raw.ptr;
+    if (c->cfg.radiance_format == VCT_FORMAT_RGBA16F)
+        hipLaunchKernelGGL(k_occ_bits<uint2>, dim3((D3 + 255u) / 256u, (unsigned)c->levels), dim3(256), 0, s,
+                           (const uint2*)c->radiance.ptr, D3, c->occ_words, raw_stride, raw);
+    else
+        hipLaunchKernelGGL(k_occ_bits<uint32_t>, dim3((D3 + 255u) / 256u, (unsigned)c->levels), dim3(256), 0, s,
+                           (const uint32_t*)c->radiance.ptr, D3, c->occ_words, raw_stride, raw);
+    hipLaunchKernelGGL(k_occ_dilate, dim3((c->occ_words + 255u) / 256u, (unsigned)c->levels), dim3(256), 0, s,
+                       (const uint32_t*)raw, D, c->occ_words, raw_stride, (uint32_t*)c->occ_mask.ptr);
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
 }  // namespace vct
 
 using namespace vct;
@@ -943,6 +1048,8 @@ static vct_status make_trace_params(vct_ctx* c, const vct_gbuffer* g, const vct_
         if (vb > 0xffffffffull) return fail(VCT_ERR_UNSUPPORTED, "radiance volume above 4 GiB");
         P.vol_bytes = (uint32_t)vb;
     }
+    P.occ = (const uint32_t*)c->occ_mask.ptr;
+    P.occ_words = c->occ_words;
     for (int k = 0; k < 16; ++k) P.vp_inv[k] = cam->view_proj_inv[k];
     for (int k = 0; k < 3; ++k) P.eye[k] = cam->eye_pos[k];
     P.width = (int)g->width;
@@ -970,24 +1077,31 @@ static vct_status make_trace_params(vct_ctx* c, const vct_gbuffer* g, const vct_
     return VCT_OK;
 }
 
+static bool traces_specular(const TraceParams& P) { return P.mode == 6u || P.mode == 8u; }
+
+// 64x64-pixel tiles of the specular work list of a width x height image.
+static size_t spec_tiles(const vct_gbuffer* g)
+{
+    return (size_t)((g->width + kSpecTile - 1) / kSpecTile) * (size_t)((g->height + kSpecTile - 1) / kSpecTile);
+}
+
 // The specular cone runs in its own roughness-binned pass whenever the mode
 // traces it (modes 6 and 8); the diffuse pass then leaves out_s alone.
+// `perm` holds spec_tiles(g) * 4096 entries when the mode traces it.
 template <bool kHalf>
-static vct_status launch_trace(vct_ctx* c, hipStream_t s, const typename Vol<kHalf>::T* vol, const TraceParams& P,
-                               const vct_gbuffer* g, float4* out_d, float4* out_s, unsigned long long* samples)
+static vct_status launch_trace(hipStream_t s, const typename Vol<kHalf>::T* vol, const TraceParams& P,
+                               const vct_gbuffer* g, float4* out_d, float4* out_s, unsigned long long* samples,
+                               uint32_t* perm)
 {
     const dim3 grid(((g->width + 15) / 16) * ((g->height + 15) / 16));
-    if (P.mode != 6u && P.mode != 8u) {
+    if (!traces_specular(P)) {
         hipLaunchKernelGGL((k_trace<kHalf, false>), grid, dim3(256), 0, s, vol, P, g->diffuse, g->normal, g->specular,
                            g->emission, g->depth, out_d, out_s, samples);
         VCT_HIP(hipGetLastError());
         return VCT_OK;
     }
     const int tiles_x = (int)((g->width + kSpecTile - 1) / kSpecTile);
-    const int tiles_y = (int)((g->height + kSpecTile - 1) / kSpecTile);
-    const size_t tiles = (size_t)tiles_x * (size_t)tiles_y;
-    VCT_TRY(ensure(c->spec_perm, 4u * tiles * kSpecTilePixels));
-    uint32_t* perm = (uint32_t*)c->spec_perm.ptr;
+    const size_t tiles = spec_tiles(g);
     hipLaunchKernelGGL(k_spec_bin, dim3((unsigned)tiles), dim3(256), 0, s, P, g->diffuse, g->specular, g->depth,
                        tiles_x, perm);
     hipLaunchKernelGGL((k_trace<kHalf, true>), grid, dim3(256), 0, s, vol, P, g->diffuse, g->normal, g->specular,
@@ -1010,11 +1124,20 @@ extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer
                               c ? c->levels : 0, &P));
     if (d->volume_dimension != (float)c->res)
         return fail(VCT_ERR_INVALID_ARGUMENT, "volume_dimension must equal the resolution");
+    uint32_t* perm = nullptr;
+    if (traces_specular(P)) {
+        VCT_TRY(ensure(c->spec_perm, 4u * spec_tiles(g) * kSpecTilePixels));
+        perm = (uint32_t*)c->spec_perm.ptr;
+    }
+    if (c->mask_version != c->radiance_version) {
+        VCT_TRY(build_occupancy(c, (hipStream_t)stream));
+        c->mask_version = c->radiance_version;
+    }
     if (c->cfg.radiance_format == VCT_FORMAT_RGBA16F)
-        return launch_trace<true>(c, (hipStream_t)stream, (const uint2*)c->radiance.ptr, P, g, (float4*)diffuse_out,
-                                  (float4*)specular_out, samples_out);
-    return launch_trace<false>(c, (hipStream_t)stream, (const uint32_t*)c->radiance.ptr, P, g, (float4*)diffuse_out,
-                               (float4*)specular_out, samples_out);
+        return launch_trace<true>((hipStream_t)stream, (const uint2*)c->radiance.ptr, P, g, (float4*)diffuse_out,
+                                  (float4*)specular_out, samples_out, perm);
+    return launch_trace<false>((hipStream_t)stream, (const uint32_t*)c->radiance.ptr, P, g, (float4*)diffuse_out,
+                               (float4*)specular_out, samples_out, perm);
 }
 
 // OctreeVoxelConeTracing::onUpdate (OctreeVoxelConeTracing.cpp:74-104). The

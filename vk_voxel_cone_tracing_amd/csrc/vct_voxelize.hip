@@ -353,6 +353,7 @@ vct_status vct_inject_radiance(vct_ctx* c, vct_stream stream, const vct_mesh* me
         return fail(VCT_ERR_INVALID_ARGUMENT, "light, light_shadow and shadow map are required");
     VoxParams P;
     VCT_TRY(make_params(c, d, level, &P));
+    radiance_written(c);
     // after vct_begin_radiance_level the texels already carry the opacity alpha
     const bool keep = c->keep_alpha[level];
     c->keep_alpha[level] = false;

@@ -83,26 +83,103 @@ def dense_trace_desc(max_distance: float):
 
 
 # --------------------------------------------------------------------------
+CLIPMAP_FRAME = dict(W=1920, H=1080, scale=16.0, seed=2, res=128, levels=6, shadow=2048)
+
+
+def clipmap_frame() -> dict:
+    """The 1080p clipmap GI frame: the seeded room scaled to 16 units (21 260
+    triangles), RGBA16F radiance + RGBA8 opacity clipmaps of 6 x 128^3, a
+    2048^2 light depth map ray cast from the seeded light, a 1920x1080
+    G-buffer ray cast from the seeded camera with per-pixel perceptual
+    roughness U[0.1, 0.8] (metallic > 0 everywhere, so every pixel traces the
+    specular cone), and the three passes registered with a RenderPassManager
+    (VoxelizationPass, RadianceInjectionPass, VoxelConeTracingPass in
+    CombinedGI mode)."""
+    from . import Context, FORMAT_RGBA16F, QUIRK_ALL, gbuffer_struct, light_structs, passes, shadow_struct
+
+    F = CLIPMAP_FRAME
+    W, H, S = F["W"], F["H"], F["scale"]
+    scene = scenes.room(F["seed"], scale=S)
+    ds = DeviceScene(scene)
+    ctx = Context(resolution=F["res"], levels=F["levels"], radiance_format=FORMAT_RGBA16F, quirks=QUIRK_ALL,
+                  raster_samples=1, downsample_region_size=10, extent_l0=16.0)
+    vp, vp_inv, eye = scenes.camera(F["seed"], W, H, scale=S)
+    ctx.update_clip_regions(eye)
+    light = scenes.light(F["seed"], scale=S)
+    ld, ls = light_structs(light)
+    n = F["shadow"]
+    shadow_depth = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    A.call("vct_raycast_shadow", ctx._h, _stream(), C.byref(ds.mesh), C.byref(ls),
+           C.c_void_p(shadow_depth.data_ptr()), n, n)
+    g = gbuffer(ctx._h, ds, scene, (vp, vp_inv, eye), W, H)
+    # per-pixel perceptual roughness U[0.1, 0.8] in the diffuse alpha byte
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(F["seed"])
+    rough = torch.rand((H, W), generator=gen, device="cuda") * 0.7 + 0.1
+    rough_b = torch.round(rough * 255.0).to(torch.int32)
+    g["diffuse"] = (g["diffuse"] & 0x00FFFFFF) | (rough_b << 24)
+    torch.cuda.synchronize()
+    sm = shadow_struct(shadow_depth)
+    gb = gbuffer_struct(g)
+    cam = camera_ubo(vp, vp_inv, eye)
+    dout = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    sout = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    mgr = passes.RenderPassManager()
+    mgr.add_render_pass("VoxelizationPass", passes.VoxelizationPass(ctx, ds))
+    inj = passes.RadianceInjectionPass(ctx, ds, sm, ld, ls)
+    mgr.add_render_pass("RadianceInjectionPass", inj)
+    vct_pass = passes.VoxelConeTracingPass(ctx, gb, cam, sm, ld, ls, dout, sout)
+    vct_pass.samples = torch.zeros(1, dtype=torch.int64, device="cuda")
+    mgr.add_render_pass("VoxelConeTracingPass", vct_pass)
+    return dict(scene=scene, ds=ds, ctx=ctx, vp=vp, vp_inv=vp_inv, eye=eye, light=light, shadow=shadow_depth, g=g,
+                gb=gb, cam=cam, sm=sm, ld=ld, ls=ls, mgr=mgr, inj=inj, vct=vct_pass, W=W, H=H, dout=dout, sout=sout)
+
+
+def clipmap_frame_run(st: dict, ev=None) -> None:
+    """One GI frame through the three passes; with `ev` (4 events) records
+    the pass boundaries."""
+    from . import Frame
+
+    f = Frame()
+    st["inj"].frame_index = 0  # every clip level is due in the measured frame
+    st["vct"].samples.zero_()
+    for k, name in enumerate(("VoxelizationPass", "RadianceInjectionPass", "VoxelConeTracingPass")):
+        if ev is not None:
+            ev[k].record()
+        st["mgr"].draw_single_render_pass(name, f)
+    if ev is not None:
+        ev[3].record()
+
+
+# --------------------------------------------------------------------------
+def plain_vxao_inputs(seed: int = 0, res: int = 128, W: int = 640, H: int = 360) -> dict:
+    """Inputs of the plain method: the room (about 21k triangles), a ray-cast
+    W x H G-buffer from the seeded camera, the res^3 grid box around the
+    room and its trace descriptor."""
+    scene = scenes.room(seed, scale=1.0)
+    ds = DeviceScene(scene)
+    vp, vp_inv, eye = scenes.camera(seed, W, H)
+    sc = scratch_ctx()
+    try:
+        g = gbuffer(sc, ds, scene, (vp, vp_inv, eye), W, H)
+    finally:
+        A.call("vct_destroy", sc)
+    gmin, ext = grid_box(scene)
+    return dict(scene=scene, ds=ds, g=g, gb=gbuffer_struct(g), vp=vp, vp_inv=vp_inv, eye=eye,
+                cam=camera_ubo(vp, vp_inv, eye), gmin=gmin, ext=ext, res=res, W=W, H=H,
+                td=dense_trace_desc(float(np.sqrt(3.0)) * ext))
+
+
 def plain_vxao(seed: int = 0, res: int = 128, W: int = 640, H: int = 360, reps: int = 5) -> dict:
     """The plain method: the room (about 21k triangles) voxelized into a
     res^3 RGBA8 grid, mipped to 1^3, VXAO with 16 cones over a ray-cast W x H
     G-buffer. One frame = clear + voxelize + mips + trace."""
-    scene = scenes.room(seed, scale=1.0)
-    ds = DeviceScene(scene)
-    sc = scratch_ctx()
-    try:
-        g = gbuffer(sc, ds, scene, scenes.camera(seed, W, H), W, H)
-    finally:
-        A.call("vct_destroy", sc)
-    gb = gbuffer_struct(g)
-    vp, vp_inv, eye = scenes.camera(seed, W, H)
-    cam = camera_ubo(vp, vp_inv, eye)
-    gmin, ext = grid_box(scene)
+    inp = plain_vxao_inputs(seed, res, W, H)
+    scene, ds, gb, cam, gmin, ext, td = (inp[k] for k in ("scene", "ds", "gb", "cam", "gmin", "ext", "td"))
     h = C.c_void_p()
     A.call("vct_dense_create", C.byref(h), C.byref(A.DenseDesc(A.f3(*gmin), ext, res, 0, 1, 0)))
     out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
     samples = torch.zeros(1, dtype=torch.int64, device="cuda")
-    td = dense_trace_desc(float(np.sqrt(3.0)) * ext)
     s = _stream()
 
     def frame():
@@ -128,10 +205,10 @@ def plain_vxao(seed: int = 0, res: int = 128, W: int = 640, H: int = 360, reps: 
 
 
 # --------------------------------------------------------------------------
-def svo_build(seed: int = 1, level: int = 9, total_tris: int = 1_000_000, reps: int = 3) -> dict:
-    """The 512^3 sparse voxel octree: the room plus thin triangles up to
-    `total_tris`, fragments (K9) then flag / allocate / init per level and the
-    leaf and mip writes (K10-K15)."""
+def svo_inputs(seed: int = 1, total_tris: int = 1_000_000, shadow: int = 2048) -> dict:
+    """Inputs of the octree build: the room plus thin triangles up to
+    `total_tris`, the seeded light and its ray-cast depth map, and a context
+    (whose clipmaps are minimal) that owns the fragment list and the pool."""
     from . import Context, light_structs, shadow_struct
 
     scene = scenes.room(seed, scale=1.0, thin_total=total_tris)
@@ -139,10 +216,20 @@ def svo_build(seed: int = 1, level: int = 9, total_tris: int = 1_000_000, reps: 
     ctx = Context(resolution=16, levels=1, raster_samples=1, downsample_region_size=1, extent_l0=1.0)
     light = scenes.light(seed, scale=1.0)
     ld, ls = light_structs(light)
-    depth = torch.empty((2048, 2048), dtype=torch.float32, device="cuda")
+    depth = torch.empty((shadow, shadow), dtype=torch.float32, device="cuda")
+    A.call("vct_raycast_shadow", ctx._h, _stream(), C.byref(ds.mesh), C.byref(ls), C.c_void_p(depth.data_ptr()),
+           shadow, shadow)
+    torch.cuda.synchronize()
+    return dict(scene=scene, ds=ds, ctx=ctx, light=light, ld=ld, ls=ls, depth=depth, sm=shadow_struct(depth))
+
+
+def svo_build(seed: int = 1, level: int = 9, total_tris: int = 1_000_000, reps: int = 3) -> dict:
+    """The 512^3 sparse voxel octree: the room plus thin triangles up to
+    `total_tris`, fragments (K9) then flag / allocate / init per level and the
+    leaf and mip writes (K10-K15)."""
+    inp = svo_inputs(seed, total_tris)
+    scene, ds, ctx, sm, ld, ls = (inp[k] for k in ("scene", "ds", "ctx", "sm", "ld", "ls"))
     s = _stream()
-    A.call("vct_raycast_shadow", ctx._h, s, C.byref(ds.mesh), C.byref(ls), C.c_void_p(depth.data_ptr()), 2048, 2048)
-    sm = shadow_struct(depth)
     nfrag = C.c_uint32()
 
     def build():
@@ -159,11 +246,10 @@ def svo_build(seed: int = 1, level: int = 9, total_tris: int = 1_000_000, reps: 
 
 
 # --------------------------------------------------------------------------
-def batched(seed: int = 3, count: int = 64, W: int = 512, H: int = 512, reps: int = 3) -> dict:
-    """64 scenes of uneven size (5k-500k triangles, 64^3-256^3 grids,
-    log-uniform) voxelized, mipped and VXAO-traced at W x H: the per-scene
-    loop of vct_dense_* calls against one replay of the batched launch
-    (after vct_batch_autotune)."""
+def batched_inputs(seed: int = 3, count: int = 64, W: int = 512, H: int = 512):
+    """The scenes of the batched workload (scenes.batch) with a ray-cast
+    W x H G-buffer each, their grid boxes and output images, and the shared
+    trace descriptor. Returns (items, trace descriptor)."""
     items = []
     sc = scratch_ctx()
     try:
@@ -176,7 +262,27 @@ def batched(seed: int = 3, count: int = 64, W: int = 512, H: int = 512, reps: in
                               gmin=gmin, ext=ext, out=torch.empty((H, W, 4), dtype=torch.float32, device="cuda")))
     finally:
         A.call("vct_destroy", sc)
-    td = dense_trace_desc(float(np.sqrt(3.0)))  # the scenes are unit rooms
+    return items, dense_trace_desc(float(np.sqrt(3.0)))  # the scenes are unit rooms
+
+
+def batch_scene_array(items):
+    """vct_batch_scene entries for batched_inputs' items."""
+    arr = (A.BatchScene * len(items))()
+    for k, it in enumerate(items):
+        arr[k].mesh = it["ds"].mesh
+        arr[k].grid = A.DenseDesc(A.f3(*it["gmin"]), it["ext"], it["res"], 0, 1, 0)
+        arr[k].gbuffer = it["gb"]
+        arr[k].camera = it["cam"]
+        arr[k].out = it["out"].data_ptr()
+    return arr
+
+
+def batched(seed: int = 3, count: int = 64, W: int = 512, H: int = 512, reps: int = 3) -> dict:
+    """64 scenes of uneven size (5k-500k triangles, 64^3-256^3 grids,
+    log-uniform) voxelized, mipped and VXAO-traced at W x H: the per-scene
+    loop of vct_dense_* calls against one replay of the batched launch
+    (after vct_batch_autotune)."""
+    items, td = batched_inputs(seed, count, W, H)
     s = _stream()
     # per-scene loop (grids created once, outside the timing)
     grids = []
@@ -198,13 +304,7 @@ def batched(seed: int = 3, count: int = 64, W: int = 512, H: int = 512, reps: in
     finally:
         for h in grids:
             A.call("vct_dense_destroy", h)
-    arr = (A.BatchScene * len(items))()
-    for k, it in enumerate(items):
-        arr[k].mesh = it["ds"].mesh
-        arr[k].grid = A.DenseDesc(A.f3(*it["gmin"]), it["ext"], it["res"], 0, 1, 0)
-        arr[k].gbuffer = it["gb"]
-        arr[k].camera = it["cam"]
-        arr[k].out = it["out"].data_ptr()
+    arr = batch_scene_array(items)
     b = C.c_void_p()
     A.call("vct_batch_create", C.byref(b), arr, len(items), C.byref(td))
     try:
@@ -225,10 +325,11 @@ def batched(seed: int = 3, count: int = 64, W: int = 512, H: int = 512, reps: in
 
 
 # --------------------------------------------------------------------------
-def adjoint(seed: int = 4, res: int = 256, views: int = 16, W: int = 512, H: int = 512, reps: int = 3) -> dict:
-    """fp32 res^3 albedo + opacity grid, `views` seeded views: forward render
-    and the L2 loss against targets rendered from a perturbed grid
-    (sigma 0.05), then the backward through cone marching and mips."""
+def adjoint_inputs(seed: int = 4, res: int = 256, views: int = 16, W: int = 512, H: int = 512) -> dict:
+    """Inputs of the adjoint workload: `views` ray-cast G-buffers of the room
+    (stacked), their camera tensor, a seeded fp32 albedo U[0.2, 0.8] and
+    opacity U[0.05, 0.55] grid of res^3 and the target images rendered from
+    the grid perturbed by N(0, 0.05) (clamped)."""
     from . import adjoint as adj
 
     scene = scenes.room(seed, scale=1.0)
@@ -251,6 +352,19 @@ def adjoint(seed: int = 4, res: int = 256, views: int = 16, W: int = 512, H: int
         ta = (albedo + 0.05 * torch.randn(albedo.shape, generator=gen, device="cuda")).clamp(0.0, 1.0)
         to = (opacity + 0.05 * torch.randn(opacity.shape, generator=gen, device="cuda")).clamp(0.0, 0.95)
         target = adj.render(ta, to, diffuse, normal, depth, cam_t, (0.0, 0.0, 0.0), 1.0, p)
+    return dict(scene=scene, ds=ds, gbuffers=gs, cams=cams, diffuse=diffuse, normal=normal, depth=depth,
+                cameras=cam_t, albedo=albedo, opacity=opacity, target=target, params=p, res=res, W=W, H=H)
+
+
+def adjoint(seed: int = 4, res: int = 256, views: int = 16, W: int = 512, H: int = 512, reps: int = 3) -> dict:
+    """fp32 res^3 albedo + opacity grid, `views` seeded views: forward render
+    and the L2 loss against targets rendered from a perturbed grid
+    (sigma 0.05), then the backward through cone marching and mips."""
+    from . import adjoint as adj
+
+    inp = adjoint_inputs(seed, res, views, W, H)
+    diffuse, normal, depth, cam_t, albedo, opacity, target, p = (
+        inp[k] for k in ("diffuse", "normal", "depth", "cameras", "albedo", "opacity", "target", "params"))
     a = albedo.clone().requires_grad_(True)
     o = opacity.clone().requires_grad_(True)
 
