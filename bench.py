@@ -199,6 +199,153 @@ def cpu_baseline(st, rows: int, runs: int = 5):
                 host=hc, frame_parity=parity)
 
 
+def _crop_rows(g: dict, vp_inv, r0: int, h: int):
+    """Rows [r0, r0 + h) of a host G-buffer as a G-buffer of their own, with
+    the inverse view-projection that gives its pixels the full image's rays
+    (NDC y of the crop maps to (h y + 2 r0 + h) / H - 1 of the full image)."""
+    H = g["depth"].shape[0]
+    sub = {k: np.ascontiguousarray(v[r0:r0 + h]) for k, v in g.items()}
+    S = np.eye(4)
+    S[1, 1] = h / H
+    S[1, 3] = (2 * r0 + h) / H - 1.0
+    return sub, np.asarray(vp_inv, np.float64) @ S
+
+
+def _host_g(g) -> dict:
+    out = {}
+    for k, v in g.items():
+        a = v.cpu().numpy()
+        out[k] = a.view(np.uint32) if a.dtype == np.int32 else (a.view(np.uint16) if a.dtype == np.int16 else a)
+    return out
+
+
+def workload_cpu_baselines() -> dict:
+    """The oracle on bounded samples of the other workloads' own inputs
+    (OpenMP where the oracle has it), each scaled to the workload's unit:
+    plain VXAO (voxelize + mips + the full 640x360 FP64 trace), the 512^3
+    octree (fragment list, pool and bricks of the 1M-triangle scene), 4 of the
+    64 batched scenes (voxelize + mips + 64 of 512 trace rows each, scaled to
+    all rows and scenes) and the adjoint (level 0 + mips + one view's forward
+    and backward over 32 of 512 rows, scaled to 16 views)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_api as O
+    from vk_voxel_cone_tracing_amd import workloads as wl
+
+    hc = host_cpu()
+    out = {}
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        return time.perf_counter() - t0, r
+
+    def dense_frame(scene, gmin, ext, res, g, vp_inv, td, rows=None):
+        pos, nrm, mat = scene.world_triangles()
+        d, _lv = O.new_dense(res, O.FMT_RGBA8, gmin, ext)
+        tv, _ = timed(lambda: O.lib().orc_dense_voxelize(C.byref(d), C.byref(O.tris_struct(pos, nrm, mat)),
+                                                         O.materials_struct(scene), 1))
+        tm, _ = timed(lambda: O.lib().orc_dense_mip(C.byref(d)))
+        H = g["depth"].shape[0]
+        gg, vpi = (g, vp_inv) if rows is None else _crop_rows(g, vp_inv, rows[0], rows[1])
+        p = O.dense_params(vpi, aperture=td.aperture, max_dist=td.max_distance, decay=td.occlusion_decay,
+                           ao_factor=td.ambient_occlusion_factor, diffuse_intensity=td.indirect_diffuse_intensity)
+        h = gg["depth"].shape[0]
+        r64 = np.zeros((h, gg["depth"].shape[1], 4), np.float64)
+        n = C.c_uint64(0)
+        tt, _ = timed(lambda: O.lib().orc_dense_trace(C.byref(d), C.byref(O.gbuf_struct(gg)), C.byref(p), 0, h,
+                                                      O.vptr(r64), C.byref(n)))
+        return tv + tm + tt * H / h, n.value * H / h
+
+    try:
+        inp = wl.plain_vxao_inputs()
+        g = _host_g(inp["g"])
+        t, n = dense_frame(inp["scene"], inp["gmin"], inp["ext"], inp["res"], g, inp["vp_inv"], inp["td"])
+        out["plain_vxao_640x360"] = dict(value=round(t * 1e3, 1), unit="ms/frame", cores=hc["omp_threads"],
+                                         kind="port", cone_samples=int(n),
+                                         sample="the whole frame: dense voxelize + mips (one thread), FP64 trace "
+                                                "of all 640x360 pixels (OpenMP over rows)")
+    except Exception as e:
+        out["plain_vxao_640x360"] = {"error": f"{type(e).__name__}: {e}"}
+    try:
+        inp = wl.svo_inputs()
+        scene = inp["scene"]
+        pos, nrm, mat = scene.world_triangles()
+        depth = inp["depth"].cpu().numpy()
+        osm = O.Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
+        level, cap = 9, 16_000_000
+        fr = np.zeros((cap, 2), np.uint32)
+        tf, nf = timed(lambda: O.lib().orc_svo_fragments(
+            C.byref(O.tris_struct(pos, nrm, mat)), O.materials_struct(scene), C.byref(osm),
+            C.byref(O.light_struct(inp["light"])), (C.c_float * 3)(*scene.bb_min.tolist()),
+            (C.c_float * 3)(*scene.bb_max.tolist()), 1 << level, 1, 0x3F, O.vptr(fr), cap))
+        pool = np.zeros((max(1_000_000, 8 * nf), 2), np.uint32)
+        cnt = C.c_uint32(0)
+        tb, _ = timed(lambda: O.lib().orc_svo_build(O.vptr(fr), nf, level, 0x3F, O.vptr(pool), len(pool),
+                                                    C.byref(cnt)))
+        O.lib().orc_svo_bricks.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
+        bricks = np.zeros((cnt.value, 27), np.uint32)
+        tk, _ = timed(lambda: O.lib().orc_svo_bricks(pool.ctypes.data, cnt.value, level, bricks.ctypes.data))
+        inp["ctx"].close()
+        out["svo_512"] = dict(value=round((tf + tb + tk) * 1e3, 1), unit="ms/build", cores=hc["omp_threads"],
+                              kind="port", fragments=int(nf), nodes=int(cnt.value),
+                              parts_s=dict(fragments=round(tf, 2), pool=round(tb, 2), bricks=round(tk, 2)),
+                              sample="the whole build: fragment list (one thread, fragment order), node pool "
+                                     "(one thread), 3^3 bricks (OpenMP over nodes)")
+    except Exception as e:
+        out["svo_512"] = {"error": f"{type(e).__name__}: {e}"}
+    try:
+        from vk_voxel_cone_tracing_amd import scenes as sc
+
+        items, td = wl.batched_inputs()
+        order = sorted(range(len(items)), key=lambda i: items[i]["scene"].triangle_count)
+        pick = [order[k] for k in (0, len(order) // 3, 2 * len(order) // 3, len(order) - 1)]
+        tot, ns = 0.0, 0.0
+        for i in pick:
+            it = items[i]
+            g = _host_g(it["g"])
+            vp_inv = sc.camera(3000 + i, 512, 512)[1]  # the camera batched_inputs gives scene i
+            t, n = dense_frame(it["scene"], it["gmin"], it["ext"], it["res"], g, vp_inv, td, rows=(224, 64))
+            tot += t
+            ns += n
+        out["batched_64"] = dict(value=round(tot * len(items) / len(pick) * 1e3, 1), unit="ms/batch",
+                                 cores=hc["omp_threads"], kind="port",
+                                 sample=f"scenes {pick} (smallest, thirds and largest by triangle count): "
+                                        "voxelize + mips + trace of rows 224-287 of 512 each (FP64, OpenMP over "
+                                        "rows), scaled to 512 rows and to 64 scenes")
+    except Exception as e:
+        out["batched_64"] = {"error": f"{type(e).__name__}: {e}"}
+    try:
+        inp = wl.adjoint_inputs()
+        res = inp["res"]
+        a = O.Adjoint(res, 1.0, (0.0, 0.0, 0.0))
+        rgba = np.concatenate([inp["albedo"].cpu().numpy(), inp["opacity"].cpu().numpy()[..., None]], -1)
+        tl, _ = timed(lambda: a.set_level0(rgba))
+        g = {k: _host_g({k: inp[k][0]})[k] for k in ("diffuse", "normal", "depth")}
+        g["specular"] = g["diffuse"]
+        g["emission"] = g["normal"]
+        vp_inv = inp["cams"][0][1]
+        gg, vpi = _crop_rows(g, vp_inv, 240, 32)
+        pp = inp["params"]
+        P = O.dense_params(vpi, aperture=pp.aperture, max_dist=pp.max_distance, step_factor=pp.min_trace_step_factor,
+                           start_offset=pp.trace_start_offset, decay=pp.occlusion_decay,
+                           ao_factor=pp.ambient_occlusion_factor, diffuse_intensity=pp.indirect_diffuse_intensity)
+        tr, img = timed(lambda: a.render(gg, P))
+        tg, _ = timed(lambda: a.backward(gg, P, np.ones_like(img)))
+        views, H = inp["depth"].shape[0], inp["depth"].shape[1]
+        total = tl + (tr + tg) * views * H / gg["depth"].shape[0]
+        out["adjoint_256"] = dict(value=round(total * 1e3, 1), unit="ms/(forward+backward)",
+                                  cores=hc["omp_threads"], kind="port",
+                                  parts_s=dict(level0_mips=round(tl, 2), render_rows=round(tr, 2),
+                                               backward_rows=round(tg, 2)),
+                                  sample="FP64 oracle adjoint: level 0 + mips once, then rows 240-271 of view 0 "
+                                         "(render and backward, one thread), scaled to 512 rows x 16 views")
+    except Exception as e:
+        out["adjoint_256"] = {"error": f"{type(e).__name__}: {e}"}
+    for k, v in out.items():
+        print(f"[bench] cpu {k}: {json.dumps(v)}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -311,6 +458,8 @@ def main():
         out["workloads"] = extras()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(st, args.cpu_rows)
+        if not args.no_extras:
+            out["cpu_baseline"]["workloads"] = workload_cpu_baselines()
     print(json.dumps(out), flush=True)
 
 

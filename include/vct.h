@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define VCT_ABI_VERSION 1u
+#define VCT_ABI_VERSION 2u
 
 typedef struct vct_ctx vct_ctx;
 typedef void* vct_stream; /* hipStream_t */
@@ -38,7 +38,7 @@ typedef enum vct_status {
     VCT_ERR_OUT_OF_MEMORY = 2,
     VCT_ERR_HIP = 3,
     VCT_ERR_CAPACITY = 4,     /* a fixed-size pool would overflow */
-    VCT_ERR_UNSUPPORTED = 5,  /* e.g. textured materials */
+    VCT_ERR_UNSUPPORTED = 5,  /* e.g. textured materials on the dense method */
     VCT_ERR_NOT_READY = 6     /* a prerequisite pass has not run */
 } vct_status;
 
@@ -199,6 +199,18 @@ typedef struct vct_gltf_material {
     int32_t padding;
 } vct_gltf_material;
 
+/* A scene texture (GLTFScene::uploadImage, GLTFScene.cpp:192-340): level 0 of
+ * an RGBA8 UNORM image, row-major, texel (x, y) at texels[y * width + x],
+ * texture coordinate (0, 0) at the corner of texel (0, 0). The voxel passes
+ * sample it as the reference's samplers do: REPEAT addressing, LINEAR
+ * filtering and maxLod 0 (GLTFScene.cpp:339, Sampler.cpp:29-50), so texture()
+ * and textureLod() both read level 0; no mip levels are needed. */
+typedef struct vct_texture {
+    const uint32_t* texels; /* device */
+    uint32_t width;
+    uint32_t height;
+} vct_texture;
+
 /* One drawIndexed call of GLTFScene::cmdDraw (GLTFScene.cpp:457-489) with its
  * push constants {instanceIndex, materialIndex} (msaaVoxelizer.vert:25-29). */
 typedef struct vct_primitive {
@@ -226,6 +238,21 @@ typedef struct vct_mesh {
     uint32_t primitive_count;
     float scene_bb_min[3]; /* SceneManager::getSceneBoundingBox, used by the SVO voxelizer */
     float scene_bb_max[3];
+    /* uTextures (msaaVoxelizer.frag:18): host array of texture_count entries
+     * that the materials' texture indices refer to; may be NULL when no drawn
+     * material references a texture. The opacity voxelizer, the radiance
+     * injector and the SVO voxelizer sample the occlusion, emissive and
+     * base-colour textures as their shaders do (texcoords must then be set);
+     * the dense method and the G-buffer producer refuse textured materials
+     * (VCT_ERR_UNSUPPORTED). Texture indices are also bounds-checked on the
+     * device: an index outside the table is never read. */
+    const vct_texture* textures;
+    uint32_t texture_count;
+    /* The validation above reads the material table's texture indices back
+     * to the host once per (materials, material_count, material_version);
+     * a caller that rewrites a material table in place, or frees one and may
+     * get a new one at the same address, gives it a new version. */
+    uint32_t material_version;
 } vct_mesh;
 
 /* A D32 shadow map (DirectionalLight::createShadowMap, DirectionalLight.cpp:59-73). */
@@ -313,6 +340,21 @@ vct_status vct_inject_radiance(vct_ctx* ctx, vct_stream stream, const vct_mesh* 
                                const vct_directional_light_shadow_desc* light_shadow,
                                const vct_voxelization_desc* desc, uint32_t level);
 
+/* VoxelizationPass::onUpdate's and RadianceInjectionPass::onUpdate's loops
+ * over the clip levels (VoxelizationPass.cpp:190-212,
+ * RadianceInjectionPass.cpp:134-159) as one call each (no counterpart as one
+ * call): descs[i] (i < count <= 8, distinct clip levels) are voxelized or
+ * injected exactly as `count` vct_voxelize_opacity / vct_inject_radiance
+ * calls in that order would, bit for bit, with the levels' work in shared
+ * launches; the injection waits on the host once for all levels (the record
+ * count) and sorts once. */
+vct_status vct_voxelize_opacity_levels(vct_ctx* ctx, vct_stream stream, const vct_mesh* mesh,
+                                       const vct_voxelization_desc* descs, uint32_t count);
+vct_status vct_inject_radiance_levels(vct_ctx* ctx, vct_stream stream, const vct_mesh* mesh,
+                                      const vct_shadow_map* shadow, const vct_directional_light_desc* light,
+                                      const vct_directional_light_shadow_desc* light_shadow,
+                                      const vct_voxelization_desc* descs, uint32_t count);
+
 /* K3: ClipmapCleaner::cmdClearOpacityClipRegion / cmdClearRadianceClipRegion
  * (ClipmapCleaner.h:27-37). */
 vct_status vct_clear_region(vct_ctx* ctx, vct_stream stream, vct_volume volume,
@@ -381,6 +423,21 @@ vct_status vct_svo_read_nodes(vct_ctx* ctx, vct_stream stream, uint32_t* out, si
 /* SparseVoxelizer::readVoxelFragments (SparseVoxelizer.cpp:328-353). */
 vct_status vct_svo_read_fragments(vct_ctx* ctx, vct_stream stream, uint32_t* out, size_t cap_fragments,
                                   size_t* n_fragments);
+/* 3^3 RGBA8 bricks of every node of the built octree (no reference shader
+ * fills bricks; EngineConfig.h:24-25 only declares their counts). Texel
+ * (i, j, k), i, j, k in {0, 1, 2}, of the brick of the node at depth d and
+ * cell c (resolution 2^d) samples the node's depth at lattice point
+ * 2c + (i, j, k) in half-cell units (corners, edge midpoints, face centres,
+ * centre): per channel the mean of the colours of the depth-d nodes whose
+ * cells touch the point (1, 2, 4 or 8 cells; a cell without a node, or
+ * outside the grid, counts as zero), truncated to an integer. A trilinear
+ * fetch anywhere in a node's cell then reads only its brick and is
+ * continuous across nodes. Brick n belongs to node n (27 uint32, x fastest);
+ * octree levels up to 10. */
+vct_status vct_svo_build_bricks(vct_ctx* ctx, vct_stream stream);
+/* Host copy of the bricks (27 uint32 each); with out == NULL only *n_bricks. */
+vct_status vct_svo_read_bricks(vct_ctx* ctx, vct_stream stream, uint32_t* out, size_t cap_bricks,
+                               size_t* n_bricks);
 /* OctreeBuilder::getOctreeBuffer: device pointer to the node pool. */
 vct_status vct_svo_pool(vct_ctx* ctx, const uint32_t** pool, uint32_t* node_count);
 /* K16: OctreeVoxelConeTracing::onUpdate (OctreeVoxelConeTracing.cpp:74-104)

@@ -264,7 +264,7 @@ static void cone_run(orc_adj* a, const orc_dense_params* P, const double start_i
         st_rw[N] = R[3];
         st_occ[N] = occ;
         ++N;
-        const double op = 1.0 - pow(1.0 - s[3], c);
+        const double op = 1.0 - pow(fmax(1.0 - s[3], 0.0), c);
         const double b = 1.0 - R[3];
         for (int k = 0; k < 3; ++k) R[k] += b * s[k] * c;
         R[3] += b * op;
@@ -285,7 +285,7 @@ static void cone_run(orc_adj* a, const orc_dense_params* P, const double start_i
         cone_sample(a, &G, st_step[k], st_diam[k], s, &lvl);
         const double vsl = G.vs * exp2(lvl), c = st_seg[k] / vsl;
         const double q = 1.0 + (st_step[k] + vsl) * (double)P->decay;
-        const double base = 1.0 - s[3];
+        const double base = fmax(1.0 - s[3], 0.0);
         const double op = 1.0 - pow(base, c);
         const double b = 1.0 - st_rw[k];
         const double g_op = gR[3] * b + gO * (1.0 - st_occ[k]) / q;

@@ -225,7 +225,7 @@ static void dense_cone(const orc_dense* g, const orc_dense_params* P, const doub
         for (int c = 0; c < 4; ++c) s[c] = a[c] * (1.0 - fl) + b[c] * fl;
         double vsl = vs * exp2(lvl);
         double corr = seg / vsl;
-        double op = fmin(fmax(1.0 - pow(1.0 - s[3], corr), 0.0), 1.0);
+        double op = fmin(fmax(1.0 - pow(fmax(1.0 - s[3], 0.0), corr), 0.0), 1.0);
         double blend = fmin(fmax(1.0 - result[3], 0.0), 1.0);
         for (int c = 0; c < 3; ++c) result[c] += blend * (s[c] * corr);
         result[3] += blend * op;

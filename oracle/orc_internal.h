@@ -7,6 +7,8 @@
 
 uint8_t orc_unorm8(float f);
 float orc_unorm8_to_float(uint32_t b);
+int orc_has_texture(const orc_tris* tris, int32_t idx);
+int orc_occlusion_discard(const orc_tris* tris, const orc_material* m, const float uv[2]);
 void orc_rgba8_to_float4(uint32_t t, float out[4]);
 void orc_load(const orc_volume* v, int x, int y, int z, float out[4]);
 void orc_store(orc_volume* v, int x, int y, int z, const float val[4]);

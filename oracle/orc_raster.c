@@ -168,19 +168,85 @@ static void extended_cells(const orc_vox_desc* d, int lo[3], int hi[3])
 
 /* ---------------- K1 opacity ---------------- */
 
+/* ---------------- scene textures ---------------- */
+
+/* REPEAT addressing of texel coordinate f (an integral float): f mod n in
+ * [0, n), with f first clamped to +-2^30 so the integer arithmetic stays in
+ * range (only texcoords beyond 2^30 / n texture sizes are affected). */
+static uint32_t repeat_texel(float f, uint32_t n)
+{
+    float c = fminf(fmaxf(f, -1073741824.0f), 1073741824.0f);
+    int m = (int)c % (int)n;
+    if (m < 0) m += (int)n;
+    return (uint32_t)m;
+}
+
+/* Vulkan LINEAR filtering of level 0 with REPEAT on both axes: the sampler
+ * the reference creates for every glTF texture (GLTFScene.cpp:339,
+ * Sampler.cpp:29-50: maxLod 0, so textureLod's lod clamps to level 0).
+ * Weights (1-a)(1-b), a(1-b), (1-a)b, ab applied in that order. */
+void orc_sample_texture(const orc_texture* t, const float uv[2], float out[4])
+{
+    const float x = uv[0] * (float)t->width - 0.5f;
+    const float y = uv[1] * (float)t->height - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const uint32_t xs[2] = {repeat_texel(fx, t->width), (repeat_texel(fx, t->width) + 1u) % t->width};
+    const uint32_t ys[2] = {repeat_texel(fy, t->height), (repeat_texel(fy, t->height) + 1u) % t->height};
+    const float w[4] = {(1.0f - a) * (1.0f - b), a * (1.0f - b), (1.0f - a) * b, a * b};
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t texel = t->texels[(size_t)ys[q >> 1] * t->width + xs[q & 1]];
+        for (int k = 0; k < 4; ++k) {
+            const float v = orc_unorm8_to_float((texel >> (8 * k)) & 0xffu);
+            acc[k] = q == 0 ? w[0] * v : acc[k] + w[q] * v;
+        }
+    }
+    for (int k = 0; k < 4; ++k) out[k] = acc[k];
+}
+
+/* fs_in.texCoord: the vertices' texcoords weighted by the fragment's
+ * barycentrics, summed in double and rounded once. */
+void orc_fragment_uv(const orc_tris* tris, uint32_t tri, const double lam[3], float uv[2])
+{
+    const float* t = tris->uv + 6u * tri;
+    for (int c = 0; c < 2; ++c)
+        uv[c] = (float)(lam[0] * (double)t[c] + lam[1] * (double)t[2 + c] + lam[2] * (double)t[4 + c]);
+}
+
+/* msaaVoxelizer.frag:64, msaaInjectRadiance.frag:73, voxelizer.frag:52:
+ * discard where the occlusion texture's red channel is below 0.1. */
+int orc_has_texture(const orc_tris* tris, int32_t idx)
+{
+    return tris->textures && idx >= 0 && (uint32_t)idx < tris->texture_count;
+}
+
+int orc_occlusion_discard(const orc_tris* tris, const orc_material* m, const float uv[2])
+{
+    if (!orc_has_texture(tris, m->occlusion_texture)) return 0;
+    float s[4];
+    orc_sample_texture(&tris->textures[m->occlusion_texture], uv, s);
+    return s[0] < 0.1f;
+}
+
 typedef struct {
     orc_volume* vol;
     const orc_vox_desc* d;
     uint32_t quirks;
+    const orc_tris* tris;
+    const orc_material* mats;
 } opa_ctx;
 
 static void opacity_frag(void* user, uint32_t tri, const float pos[3], const double lam[3])
 {
-    (void)tri;
-    (void)lam;
     opa_ctx* c = (opa_ctx*)user;
     const orc_vox_desc* d = c->d;
     if (!in_region(d, pos, c->quirks)) return;
+    if (c->mats && c->tris->textures) {
+        float uv[2];
+        orc_fragment_uv(c->tris, tri, lam, uv);
+        if (orc_occlusion_discard(c->tris, &c->mats[c->tris->mat[tri]], uv)) return;
+    }
     int ic[3];
     image_coords(d, pos, c->quirks, ic);
     const int rwb = d->clipmap_resolution + 2;
@@ -195,14 +261,20 @@ static void opacity_frag(void* user, uint32_t tri, const float pos[3], const dou
     }
 }
 
-void orc_voxelize_opacity(orc_volume* vol, const orc_tris* tris, const orc_vox_desc* d, int samples,
-                          uint32_t quirks)
+void orc_voxelize_opacity_textured(orc_volume* vol, const orc_tris* tris, const orc_material* mats,
+                                   const orc_vox_desc* d, int samples, uint32_t quirks)
 {
-    opa_ctx c = {vol, d, quirks};
+    opa_ctx c = {vol, d, quirks, tris, mats};
     int lo[3], hi[3];
     extended_cells(d, lo, hi);
     for (uint32_t t = 0; t < tris->count; ++t)
         orc_raster_triangle(tris->pos + 9u * t, d->voxel_size, lo, hi, samples, opacity_frag, &c, t);
+}
+
+void orc_voxelize_opacity(orc_volume* vol, const orc_tris* tris, const orc_vox_desc* d, int samples,
+                          uint32_t quirks)
+{
+    orc_voxelize_opacity_textured(vol, tris, NULL, d, samples, quirks);
 }
 
 /* ---------------- shadow.glsl ---------------- */
@@ -327,13 +399,23 @@ static void inject_frag(void* user, uint32_t tri, const float pos[3], const doub
     for (int k = 0; k < 3; ++k)
         if (pos[k] < c->box_lo[k] || pos[k] > c->box_hi[k]) return; /* depth clip of the extended region */
     const orc_material* m = &c->mats[c->tris->mat[tri]];
+    float uv[2] = {0.0f, 0.0f};
+    if (c->tris->textures) {
+        orc_fragment_uv(c->tris, tri, lam, uv);
+        if (orc_occlusion_discard(c->tris, m, uv)) return;
+    }
     int ic[3];
     image_coords(c->d, pos, c->quirks, ic);
     const int rwb = c->d->clipmap_resolution + 2;
     if (m->emissive[0] > 0.0f || m->emissive[1] > 0.0f || m->emissive[2] > 0.0f) {
-        /* msaaInjectRadiance.frag:76-86 */
-        float e[4];
-        for (int k = 0; k < 3; ++k) e[k] = fminf(fmaxf(m->emissive[k], 0.0f), 1.0f);
+        /* msaaInjectRadiance.frag:76-86: emissive factor + emissive texture */
+        float e[4], tex[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int textured = orc_has_texture(c->tris, m->emissive_texture);
+        if (textured) orc_sample_texture(&c->tris->textures[m->emissive_texture], uv, tex);
+        for (int k = 0; k < 3; ++k) {
+            const float v = textured ? m->emissive[k] + tex[k] : m->emissive[k];
+            e[k] = fminf(fmaxf(v, 0.0f), 1.0f);
+        }
         e[3] = 1.0f;
         for (int f = 0; f < 6; ++f) sink(c, ic[0] + rwb * f, ic[1], ic[2], e);
         return;
@@ -351,8 +433,16 @@ static void inject_frag(void* user, uint32_t tri, const float pos[3], const doub
     float lc[3];
     for (int k = 0; k < 3; ++k) lc[k] = ndl * vis * c->L->color[k] * c->L->intensity;
     if (lc[0] == 0.0f && lc[1] == 0.0f && lc[2] == 0.0f) return;
+    /* msaaInjectRadiance.frag:131-136: base colour factor x base colour texture */
+    float col[4];
+    for (int k = 0; k < 4; ++k) col[k] = m->base_color[k];
+    if (orc_has_texture(c->tris, m->base_color_texture)) {
+        float tex[4];
+        orc_sample_texture(&c->tris->textures[m->base_color_texture], uv, tex);
+        for (int k = 0; k < 4; ++k) col[k] = col[k] * tex[k];
+    }
     float rad[3];
-    for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(lc[k] * m->base_color[k] * m->base_color[3], 0.0f), 1.0f);
+    for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(lc[k] * col[k] * col[3], 0.0f), 1.0f);
     int face[3] = {(-n[0] > 0.0f) ? 0 : 1, (-n[1] > 0.0f) ? 2 : 3, (-n[2] > 0.0f) ? 4 : 5};
     for (int k = 0; k < 3; ++k) {
         float w = fabsf(n[k]);

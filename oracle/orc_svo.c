@@ -45,16 +45,34 @@ static void svo_frag(void* user, uint32_t tri, const float pos[3], const double 
     /* fs_in.position = biasAndScale(p) (voxelizer.geom:27-30, 51) */
     float q[3];
     for (int k = 0; k < 3; ++k) q[k] = (pos[k] + 1.0f) * 0.5f;
-    uint32_t cur = c->counter++; /* atomicAdd before any discard (voxelizer.frag:55) */
-    if (cur >= c->cap) return;
     const orc_material* m = &c->mats[c->tris->mat[tri]];
+    const int textured = c->tris->textures != NULL;
+    float uv[2] = {0.0f, 0.0f};
+    if (textured) {
+        orc_fragment_uv(c->tris, tri, lam, uv);
+        if (orc_occlusion_discard(c->tris, m, uv)) return; /* voxelizer.frag:52-53, before the slot */
+    }
+    uint32_t cur = c->counter++; /* atomicAdd before the light discard (voxelizer.frag:55) */
+    if (cur >= c->cap) return;
     float rad[4] = {1.0f, 1.0f, 1.0f, 1.0f};
     if (m->emissive[0] > 0.0f || m->emissive[1] > 0.0f || m->emissive[2] > 0.0f) {
-        for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(m->emissive[k], 0.0f), 1.0f);
+        /* voxelizer.frag:60-69 */
+        float tex[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int has = orc_has_texture(c->tris, m->emissive_texture);
+        if (has) orc_sample_texture(&c->tris->textures[m->emissive_texture], uv, tex);
+        for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(has ? m->emissive[k] + tex[k] : m->emissive[k], 0.0f), 1.0f);
     } else {
+        /* voxelizer.frag:72-77: textureLod(base colour) * factor; without a
+         * base colour texture the colour stays (0, 0, 0, 1) under the quirk */
         float color[4] = {0.0f, 0.0f, 0.0f, 1.0f};
-        if (!(c->quirks & ORC_QUIRK_SVO_UNTEXTURED_BLACK))
+        const int has = orc_has_texture(c->tris, m->base_color_texture);
+        if (has) {
+            float tex[4];
+            orc_sample_texture(&c->tris->textures[m->base_color_texture], uv, tex);
+            for (int k = 0; k < 4; ++k) color[k] = tex[k] * m->base_color[k];
+        } else if (!(c->quirks & ORC_QUIRK_SVO_UNTEXTURED_BLACK)) {
             for (int k = 0; k < 4; ++k) color[k] = m->base_color[k];
+        }
         const float* n0 = c->tris->nrm + 9u * tri;
         float ni[3];
         for (int k = 0; k < 3; ++k)
@@ -95,7 +113,8 @@ uint32_t orc_svo_fragments(const orc_tris* tris, const orc_material* mats, const
     float ev = fmaxf(ext[0], fmaxf(ext[1], ext[2])) * 0.5f;
     float* np = (float*)malloc(sizeof(float) * 9u * (tris->count ? tris->count : 1u));
     for (uint32_t i = 0; i < 9u * tris->count; ++i) np[i] = (tris->pos[i] - center[i % 3u]) / ev;
-    orc_tris nt = {np, tris->nrm, tris->mat, tris->count};
+    orc_tris nt = *tris;
+    nt.pos = np;
     svo_ctx c;
     c.tris = &nt;
     c.mats = mats;
@@ -225,4 +244,97 @@ void orc_svo_pack_fragments(const uint32_t* xyz, const uint32_t* rgba8, uint32_t
 void orc_svo_unpack_fragments(const uint32_t* frags, uint32_t n, uint32_t quirks, uint32_t* xyz)
 {
     for (uint32_t i = 0; i < n; ++i) frag_pos(frags + 2u * i, quirks, xyz + 3u * i);
+}
+
+/* ---------------- 3^3 RGBA8 bricks (build-defined; the reference only
+ * declares brick constants, EngineConfig.h:24-25) ----------------
+ * Dense-grid restatement in FP64: every node of the pool gets its depth d
+ * and cell c (at resolution 2^d) from a walk down the child pointers; V_d is
+ * the dense grid of the depth-d node colours (zero where no node); texel
+ * (i, j, k) of the node's brick is the lattice point 2c + (i, j, k) in
+ * half-cell units, whose value per channel is the mean over the depth-d cells
+ * touching it - per axis {c} for i = 1, {c-1, c} for i = 0, {c, c+1} for
+ * i = 2, cells outside the grid counting as zero - truncated to an integer.
+ * bricks[n * 27 + (k * 3 + j) * 3 + i]. */
+int orc_svo_bricks(const uint32_t* pool, uint32_t node_count, int level, uint32_t* bricks)
+{
+    if (level < 1 || level > 10 || node_count < 8 || node_count % 8u) return -1;
+    int32_t* depth = (int32_t*)calloc(node_count, sizeof(int32_t));
+    uint32_t* cell = (uint32_t*)calloc((size_t)node_count * 3u, sizeof(uint32_t));
+    size_t total = 0, off[12];
+    for (int d = 1; d <= level; ++d) {
+        off[d] = total;
+        total += (size_t)1 << (3 * d);
+    }
+    uint32_t* grid = (uint32_t*)calloc(total, sizeof(uint32_t));
+    if (!depth || !cell || !grid) {
+        free(depth);
+        free(cell);
+        free(grid);
+        return -1;
+    }
+    /* root children, then every child block in pool order (a block always
+     * follows its parent's block) */
+    for (uint32_t s = 0; s < 8; ++s) {
+        depth[s] = 1;
+        cell[3u * s + 0] = (s >> 1) & 1u;
+        cell[3u * s + 1] = (s >> 2) & 1u;
+        cell[3u * s + 2] = s & 1u;
+    }
+    for (uint32_t n = 0; n < node_count; ++n) {
+        if (!depth[n]) continue;
+        const uint32_t child = pool[2u * n] & 0x7fffffffu;
+        if (!child || depth[n] >= level || child + 8u > node_count) continue;
+        for (uint32_t s = 0; s < 8; ++s) {
+            depth[child + s] = depth[n] + 1;
+            cell[3u * (child + s) + 0] = 2u * cell[3u * n + 0] + ((s >> 1) & 1u);
+            cell[3u * (child + s) + 1] = 2u * cell[3u * n + 1] + ((s >> 2) & 1u);
+            cell[3u * (child + s) + 2] = 2u * cell[3u * n + 2] + (s & 1u);
+        }
+    }
+    for (uint32_t n = 0; n < node_count; ++n) {
+        const int d = depth[n];
+        if (!d) continue;
+        const size_t r = (size_t)1 << d;
+        grid[off[d] + ((size_t)cell[3u * n + 2] * r + cell[3u * n + 1]) * r + cell[3u * n + 0]] = pool[2u * n + 1];
+    }
+    /* every brick reads only the grids: independent per node */
+#pragma omp parallel for schedule(static)
+    for (uint32_t n = 0; n < node_count; ++n) {
+        const int d = depth[n];
+        uint32_t* b = bricks + 27u * (size_t)n;
+        if (!d) {
+            memset(b, 0, 27u * sizeof(uint32_t));
+            continue;
+        }
+        const long r = 1L << d;
+        const long c[3] = {(long)cell[3u * n + 0], (long)cell[3u * n + 1], (long)cell[3u * n + 2]};
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    const int t[3] = {i, j, k};
+                    long lo[3], hi[3];
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = t[a] == 2 ? c[a] : c[a] - (t[a] == 0);
+                        hi[a] = t[a] == 0 ? c[a] : c[a] + (t[a] == 2);
+                    }
+                    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+                    int count = 0;
+                    for (long z = lo[2]; z <= hi[2]; ++z)
+                        for (long y = lo[1]; y <= hi[1]; ++y)
+                            for (long x = lo[0]; x <= hi[0]; ++x) {
+                                ++count;
+                                if (x < 0 || y < 0 || z < 0 || x >= r || y >= r || z >= r) continue;
+                                const uint32_t v = grid[off[d] + ((size_t)z * (size_t)r + (size_t)y) * (size_t)r + (size_t)x];
+                                for (int ch = 0; ch < 4; ++ch) sum[ch] += (double)((v >> (8 * ch)) & 0xffu);
+                            }
+                    uint32_t out = 0;
+                    for (int ch = 0; ch < 4; ++ch) out |= ((uint32_t)floor(sum[ch] / (double)count) & 0xffu) << (8 * ch);
+                    b[(k * 3 + j) * 3 + i] = out;
+                }
+    }
+    free(depth);
+    free(cell);
+    free(grid);
+    return 0;
 }

@@ -192,7 +192,7 @@ static void svo_trace_cone(const uint32_t* pool, const orc_trace_params* p, cons
         for (int k = 0; k < 4; ++k) s[k] = lo[k] * (1.0f - fl) + hi[k] * fl;
         voxel_size = p->voxel_size * exp2f(cur_level);
         float correction = seg / voxel_size;
-        float opacity = fminf(fmaxf(1.0f - powf(1.0f - s[3], correction), 0.0f), 1.0f);
+        float opacity = fminf(fmaxf(1.0f - powf(fmaxf(1.0f - s[3], 0.0f), correction), 0.0f), 1.0f);
         float blend = fminf(fmaxf(1.0f - result[3], 0.0f), 1.0f);
         for (int k = 0; k < 3; ++k) result[k] += blend * (s[k] * correction);
         result[3] += blend * opacity;

@@ -132,7 +132,9 @@ static void FN(trace_cone)(const orc_volume* vol, const orc_trace_params* p, con
         voxel_size = (R)p->voxel_size * M_EXP2(cur_level);
         R correction = seg / voxel_size;
         R rad[3] = {s[0] * correction, s[1] * correction, s[2] * correction};
-        R opacity = (R)1 - M_POW((R)1 - s[3], correction);
+        /* 1 - a clamped at 0 (an FP filtered alpha can exceed 1 by an ulp;
+         * the reference's filter weights sum to 1 exactly) */
+        R opacity = (R)1 - M_POW(M_FMAX((R)1 - s[3], (R)0), correction);
         opacity = M_FMIN(M_FMAX(opacity, (R)0), (R)1);
         R blend = M_FMIN(M_FMAX((R)1 - result[3], (R)0), (R)1);
         result[0] += blend * rad[0];

@@ -50,17 +50,29 @@ typedef struct orc_volume {
     int levels;     /* clip levels (6) */
 } orc_volume;
 
+/* An RGBA8 texture (R in the low byte), rows of `width` texels. */
+typedef struct orc_texture {
+    const uint32_t* texels;
+    uint32_t width, height;
+} orc_texture;
+
 /* World-space triangle soup after the vertex stage (msaaVoxelizer.vert:31-37). */
 typedef struct orc_tris {
     const float* pos;      /* 9 floats per triangle */
     const float* nrm;      /* 9 floats per triangle (itModel * normal, not normalised) */
     const uint32_t* mat;   /* material index per triangle */
     uint32_t count;
+    const float* uv;       /* 6 floats per triangle (the vertices' texcoords) */
+    const orc_texture* textures; /* NULL: no texture lookups at all */
+    uint32_t texture_count;
 } orc_tris;
 
+/* Texture indices are -1 for none; they are only read when the soup has a
+ * texture table. */
 typedef struct orc_material {
     float base_color[4];
     float emissive[3];
+    int32_t base_color_texture, emissive_texture, occlusion_texture;
 } orc_material;
 
 typedef struct orc_light {
@@ -142,6 +154,13 @@ void orc_downsample_radiance(orc_volume* vol, const int prev_min_corner[3], int 
 int orc_dominant_axis(const float* p);                     /* msaaVoxelizer.geom:27-32 */
 void orc_voxelize_opacity(orc_volume* vol, const orc_tris* tris, const orc_vox_desc* d,
                           int samples, uint32_t quirks);
+/* The same with the materials' occlusion textures (msaaVoxelizer.frag:64). */
+void orc_voxelize_opacity_textured(orc_volume* vol, const orc_tris* tris, const orc_material* mats,
+                                   const orc_vox_desc* d, int samples, uint32_t quirks);
+/* Bilinear level-0 REPEAT sample of a texture (the reference's samplers:
+ * GLTFScene.cpp:339, Sampler.cpp:29-50) and the texcoord of a fragment. */
+void orc_sample_texture(const orc_texture* t, const float uv[2], float out[4]);
+void orc_fragment_uv(const orc_tris* tris, uint32_t tri, const double lam[3], float uv[2]);
 float orc_visibility(const orc_shadow* sm, const orc_light* light, const float pos[3],
                      uint32_t quirks);                     /* shadow.glsl:28-36 */
 void orc_inject_radiance(orc_volume* vol, const orc_tris* tris, const orc_material* mats,
@@ -193,6 +212,12 @@ uint32_t orc_svo_fragments(const orc_tris* tris, const orc_material* mats, const
                            int resolution, int samples, uint32_t quirks, uint32_t* frags, uint32_t cap);
 int orc_svo_build(const uint32_t* frags, uint32_t nfrag, int level, uint32_t quirks,
                   uint32_t* pool, uint32_t pool_cap, uint32_t* node_count);
+/* 3^3 RGBA8 bricks of every node of a built pool (build-defined): texel
+ * (i, j, k) of node n at depth d and cell c is the truncated mean of the
+ * depth-d node colours of the cells touching lattice point 2c + (i, j, k)
+ * (half-cell units; outside cells count as zero). bricks: 27 per node,
+ * x fastest. Returns 0, or -1 for a bad pool. */
+int orc_svo_bricks(const uint32_t* pool, uint32_t node_count, int level, uint32_t* bricks);
 /* test hooks: fragment words (voxelizer.frag:99-100) and their position decode
  * (octreeNodeFlag.comp:28-32, >> 1 with ORC_QUIRK_SVO_HALF_COORDS); one sampleSVO */
 void orc_svo_pack_fragments(const uint32_t* xyz, const uint32_t* rgba8, uint32_t n, uint32_t* out);

@@ -35,12 +35,18 @@ class Volume(C.Structure):
     _fields_ = [("texels", C.c_void_p), ("fmt", C.c_int), ("res", C.c_int), ("levels", C.c_int)]
 
 
+class Texture(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
 class Tris(C.Structure):
-    _fields_ = [("pos", C.c_void_p), ("nrm", C.c_void_p), ("mat", C.c_void_p), ("count", C.c_uint32)]
+    _fields_ = [("pos", C.c_void_p), ("nrm", C.c_void_p), ("mat", C.c_void_p), ("count", C.c_uint32),
+                ("uv", C.c_void_p), ("textures", C.c_void_p), ("texture_count", C.c_uint32)]
 
 
 class Material(C.Structure):
-    _fields_ = [("base_color", C.c_float * 4), ("emissive", C.c_float * 3)]
+    _fields_ = [("base_color", C.c_float * 4), ("emissive", C.c_float * 3), ("base_color_texture", C.c_int32),
+                ("emissive_texture", C.c_int32), ("occlusion_texture", C.c_int32)]
 
 
 class Light(C.Structure):
@@ -167,8 +173,17 @@ def vol_struct(arr: np.ndarray, res: int, levels: int) -> Volume:
     return Volume(arr.ctypes.data, fmt, res, levels)
 
 
-def tris_struct(pos, nrm, mat) -> Tris:
-    return Tris(pos.ctypes.data, nrm.ctypes.data, mat.ctypes.data, len(mat))
+def tris_struct(pos, nrm, mat, scene=None) -> Tris:
+    """The soup; with a textured `scene`, also its texcoords (in
+    world_triangles order) and texture table, kept alive by the structure."""
+    t = Tris(pos.ctypes.data, nrm.ctypes.data, mat.ctypes.data, len(mat))
+    if scene is not None and getattr(scene, "textures", None):
+        uv = scene.triangle_uvs()
+        texels = [np.ascontiguousarray(x, np.uint32) for x in scene.textures]
+        table = (Texture * len(texels))(*[Texture(x.ctypes.data, x.shape[1], x.shape[0]) for x in texels])
+        t.uv, t.textures, t.texture_count = uv.ctypes.data, C.addressof(table), len(texels)
+        t._keep = (uv, texels, table)
+    return t
 
 
 def materials_struct(scene):
@@ -176,6 +191,8 @@ def materials_struct(scene):
     for i, m in enumerate(scene.materials):
         arr[i].base_color = (C.c_float * 4)(*m["base_color_factor"].tolist())
         arr[i].emissive = (C.c_float * 3)(*m["emissive_factor"].tolist())
+        for f in ("base_color_texture", "emissive_texture", "occlusion_texture"):
+            setattr(arr[i], f, int(m[f]) if f in m.dtype.names else -1)
     return arr
 
 
@@ -412,7 +429,7 @@ def clipmap_voxel_passes(scene, regions, descs, shadow_depth, light, res, levels
 
     L = lib()
     pos, nrm, mat = scene.world_triangles()
-    tris = tris_struct(pos, nrm, mat)
+    tris = tris_struct(pos, nrm, mat, scene)
     mats = materials_struct(scene)
     depth = np.ascontiguousarray(shadow_depth, np.float32)
     osm = Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
@@ -422,7 +439,7 @@ def clipmap_voxel_passes(scene, regions, descs, shadow_depth, light, res, levels
     vo, vr = vol_struct(opa, res, levels), vol_struct(rad, res, levels)
 
     def level(lvl):
-        L.orc_voxelize_opacity(C.byref(vo), C.byref(tris), C.byref(descs[lvl]), 1, quirks)
+        L.orc_voxelize_opacity_textured(C.byref(vo), C.byref(tris), mats, C.byref(descs[lvl]), 1, quirks)
         L.orc_inject_radiance(C.byref(vr), C.byref(tris), mats, C.byref(osm), C.byref(ol), C.byref(descs[lvl]), 1,
                               quirks)
 
@@ -450,7 +467,8 @@ def trace_rows(rad, res, levels, g: dict, params, shadow_depth, light, rows, qui
     osm = Shadow(depth.ctypes.data, depth.shape[1], depth.shape[0])
     n = C.c_uint64(0)
     fn = lib().orc_trace_rowlist_f64 if fp64 else lib().orc_trace_rowlist
-    fn(C.addressof(vol_struct(rad, res, levels)), C.addressof(gbuf_struct(g)), C.addressof(params),
-       C.addressof(osm), C.addressof(light_struct(light)), quirks, rows.ctypes.data, len(rows), d.ctypes.data,
-       s.ctypes.data, C.addressof(n))
+    # the structures stay referenced until the call returns
+    vol, gb, ol = vol_struct(rad, res, levels), gbuf_struct(g), light_struct(light)
+    fn(C.byref(vol), C.byref(gb), C.byref(params), C.byref(osm), C.byref(ol), quirks, rows.ctypes.data, len(rows),
+       d.ctypes.data, s.ctypes.data, C.byref(n))
     return d, s, n.value

@@ -456,3 +456,49 @@ def test_voxelize_and_inject_at_128():
     assert occupied > 100000
     np.testing.assert_array_equal(ctx.read_volume(V.VOLUME_OPACITY), ref_o)
     np.testing.assert_array_equal(ctx.read_volume(V.VOLUME_RADIANCE), ref_r)
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_level_set_calls_equal_per_level_calls(fmt):
+    # vct_voxelize_opacity_levels / vct_inject_radiance_levels against one
+    # vct_voxelize_opacity / vct_inject_radiance call per level, bit for bit,
+    # in the order the passes use (RGBA16F: begin_radiance_level first, so
+    # the injection keeps the stored alpha), with levels given out of order
+    import vk_voxel_cone_tracing_amd as V
+
+    print(f"[clipmap] level-set calls fmt={fmt}", flush=True)
+    scene, ctx, ds = _setup_case(0x3F, fmt=fmt)
+    try:
+        pos, nrm, mat = scene.world_triangles()
+        light, depth, depth_d = _light_and_shadow(scene, pos, nrm, mat)
+        ld, ls = V.light_structs(light)
+        sm = V.shadow_struct(depth_d)
+        order = [2, 0, 1][:LEVELS]
+        out = []
+        for batched in (False, True):
+            ctx.clear_volume(V.VOLUME_OPACITY)
+            ctx.clear_volume(V.VOLUME_RADIANCE)
+            descs = [ctx.voxelization_desc(level) for level in order]
+            if batched:
+                ctx.voxelize_opacity_levels(ds, descs)
+            else:
+                for level, d in zip(order, descs):
+                    ctx.voxelize_opacity(ds, d, level)
+            if fmt == 1:
+                for level in order:
+                    ctx.begin_radiance_level(level)
+            if batched:
+                ctx.inject_radiance_levels(ds, sm, ld, ls, descs)
+            else:
+                for level, d in zip(order, descs):
+                    ctx.inject_radiance(ds, sm, ld, ls, d, level)
+            out.append((ctx.read_volume(V.VOLUME_OPACITY), ctx.read_volume(V.VOLUME_RADIANCE)))
+        np.testing.assert_array_equal(out[0][0], out[1][0])
+        np.testing.assert_array_equal(out[0][1], out[1][1])
+        assert (out[0][0] != 0).sum() > 500 and (out[0][1] != 0).sum() > 500
+        # a repeated level is refused
+        d0 = ctx.voxelization_desc(0)
+        with pytest.raises(V._abi.VctError):
+            ctx.voxelize_opacity_levels(ds, [d0, d0])
+    finally:
+        ctx.close()

@@ -9,9 +9,9 @@ the inputs bench.py times:
 - the plain method (128^3 RGBA8, 640x360 VXAO): voxelization and every mip
   level bit-exact, the trace within 1e-3 of the FP64 oracle, sample counts
   equal up to step-boundary rounding;
-- the 512^3 octree from 1M triangles: fragment list and node pool bit-exact
-  against the oracle, and the topology equal to the distinct ancestors of the
-  fragments;
+- the 512^3 octree from 1M triangles: fragment list, node pool and 3^3
+  bricks bit-exact against the oracle, and the topology equal to the distinct
+  ancestors of the fragments;
 - the 64-scene batch: every pyramid level of every scene bit-identical to the
   per-scene calls and every image equal, under the phased and the persistent
   schedules, with equal sample totals;
@@ -189,6 +189,9 @@ def test_svo_config_bit_exact():
         A.call("vct_svo_read_nodes", ctx._h, None, None, 0, C.byref(nn))
         pool = np.zeros((nn.value, 2), np.uint32)
         A.call("vct_svo_read_nodes", ctx._h, None, pool.ctypes.data_as(C.c_void_p), nn.value, C.byref(nn))
+        A.call("vct_svo_build_bricks", ctx._h, None)
+        bricks = np.zeros((nn.value, 27), np.uint32)
+        A.call("vct_svo_read_bricks", ctx._h, None, bricks.ctypes.data_as(C.c_void_p), nn.value, C.byref(nn))
     finally:
         ctx.close()
     scene = inp["scene"]
@@ -208,6 +211,13 @@ def test_svo_config_bit_exact():
     assert O.lib().orc_svo_build(O.vptr(ofr), onf, level, 0x3F, O.vptr(opool), len(opool), C.byref(ocount)) == 0
     assert ocount.value == len(pool)
     np.testing.assert_array_equal(pool, opool[:len(pool)])
+    # the 3^3 bricks against the dense-grid oracle
+    L = O.lib()
+    L.orc_svo_bricks.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
+    L.orc_svo_bricks.restype = C.c_int
+    obricks = np.zeros_like(bricks)
+    assert L.orc_svo_bricks(pool.ctypes.data, len(pool), level, obricks.ctypes.data) == 0
+    np.testing.assert_array_equal(bricks, obricks)
     # topology: with the half-coordinate quirk the tree is built over positions >> 1 at resolution 512;
     # the subdivided nodes of depth d are the distinct ancestors of the fragment positions at depth d
     p = np.stack([frags[:, 0] & 0xFFF, (frags[:, 0] >> 12) & 0xFFF,

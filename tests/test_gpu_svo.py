@@ -220,3 +220,33 @@ def test_empty_draw_list():
     ds.mesh.primitive_count = full_count
     ctx.voxelize_opacity(ds, ctx.voxelization_desc(0), 0)
     assert ctx.read_volume(V.VOLUME_OPACITY).any()
+
+
+@pytest.mark.parametrize("quirks,level", [(0x3F, LEVEL), (0x00, LEVEL), (0x3F, 1)])
+def test_bricks_match_dense_grid_oracle(quirks, level):
+    # the 3^3 bricks of every node of the built pool against the oracle's
+    # dense-grid restatement (orc_svo_bricks), bit for bit
+    import vk_voxel_cone_tracing_amd as V
+
+    print(f"[svo] bricks quirks={quirks:#x} level={level}", flush=True)
+    r = _run(quirks, level=level)
+    ctx = r["ctx"]
+    try:
+        V._abi.call("vct_svo_build_bricks", ctx._h, None)
+        n = C.c_size_t(0)
+        V._abi.call("vct_svo_read_bricks", ctx._h, None, None, 0, C.byref(n))
+        assert n.value == len(r["nodes"])
+        got = np.zeros((n.value, 27), np.uint32)
+        V._abi.call("vct_svo_read_bricks", ctx._h, None, got.ctypes.data_as(C.c_void_p), n.value, C.byref(n))
+    finally:
+        ctx.close()
+    L = O.lib()
+    L.orc_svo_bricks.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
+    L.orc_svo_bricks.restype = C.c_int
+    ref = np.zeros_like(got)
+    nodes = np.ascontiguousarray(r["nodes"])
+    assert L.orc_svo_bricks(nodes.ctypes.data, len(nodes), level, ref.ctypes.data) == 0
+    nonzero = int((ref != 0).any(axis=1).sum())
+    print(f"{len(nodes)} nodes, {nonzero} bricks with a nonzero texel", flush=True)
+    assert nonzero > 0
+    np.testing.assert_array_equal(got, ref)

@@ -196,3 +196,48 @@ def test_octree_trace_empty_pool_ambient_occlusion():
     S = float(np.sum(np.clip(dirs[:, 2], 0.0, None)))
     want = min(1.0, 0.5 * (1.0 + S) / S)
     assert np.allclose(d[..., :3], want, rtol=2e-6, atol=0)
+
+
+def _bricks(pool, count, level):
+    L = O.lib()
+    L.orc_svo_bricks.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
+    L.orc_svo_bricks.restype = C.c_int
+    out = np.zeros((count, 3, 3, 3), np.uint32)  # [node, k, j, i]
+    assert L.orc_svo_bricks(pool.ctypes.data, count, level, out.ctypes.data) == 0
+    return out
+
+
+def test_bricks_on_a_hand_built_tree():
+    # the 2x2x2 block of leaves at cells 0..1 (colour 240, 160, 80, alpha 15) of a 16^3 tree
+    L = 4
+    block = [(x, y, z) for z in (0, 1) for y in (0, 1) for x in (0, 1)]
+    pool, count = _build(_pack(block, [0xFF50A0F0] * 8), L, 0)
+    b = _bricks(pool, count, L)
+    leaf = _node(pool, (0, 0, 0), 4, L)
+    v = [240, 160, 80, 15]
+    assert _rgba(b[leaf, 1, 1, 1]) == v                       # its own cell
+    assert _rgba(b[leaf, 1, 1, 2]) == v                       # face shared with filled (1,0,0)
+    assert _rgba(b[leaf, 1, 1, 0]) == [120, 80, 40, 7]        # face shared with outside (-1,0,0)
+    assert _rgba(b[leaf, 0, 0, 0]) == [30, 20, 10, 1]         # corner: 1 filled cell of 8
+    assert _rgba(b[leaf, 2, 2, 2]) == v                       # corner inside the filled block
+    assert _rgba(b[leaf, 0, 1, 1]) == [120, 80, 40, 7]        # face z = 0 boundary
+    assert _rgba(b[leaf, 1, 0, 2]) == [120, 80, 40, 7]        # edge (z centre): 2 filled cells of 4
+    assert _rgba(b[leaf, 2, 0, 2]) == [120, 80, 40, 7]        # corner: 4 filled cells of 8 (y = -1 outside)
+    assert _rgba(b[leaf, 1, 0, 1]) == [120, 80, 40, 7]        # face y = 0: (0,-1,0) outside
+
+
+def test_bricks_of_a_uniform_level_are_uniform_inside():
+    # every cell of a 4^3 tree filled with one colour: the bricks of interior
+    # leaves are that colour everywhere; a boundary texel averages with the
+    # outside (one face: half, truncated)
+    L = 2
+    cells = [(x, y, z) for z in range(4) for y in range(4) for x in range(4)]
+    pool, count = _build(_pack(cells, [0xFF649632] * len(cells)), L, 0)  # (50, 150, 100, 15)
+    b = _bricks(pool, count, L)
+    v = [50, 150, 100, 15]
+    for c in [(1, 1, 1), (2, 1, 2), (1, 2, 2)]:
+        n = _node(pool, c, 2, L)
+        assert all(_rgba(w) == v for w in b[n].reshape(-1))
+    edge = _node(pool, (0, 1, 1), 2, L)
+    assert _rgba(b[edge, 1, 1, 0]) == [25, 75, 50, 7]
+    assert _rgba(b[edge, 1, 1, 1]) == v

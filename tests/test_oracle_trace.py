@@ -197,3 +197,70 @@ def test_whole_pixel_ambient_occlusion_empty_volume():
                           0x3F, 0, 1, O.vptr(rd), O.vptr(rs), None)
         assert abs(rd[0, 0, 0] - want) < 1e-5 and rd[0, 0, 0] == rd[0, 0, 1] == rd[0, 0, 2]
         assert n is not None
+
+
+def test_row_list_trace_equals_row_range_trace():
+    # trace_rows (orc_trace_rowlist / _f64, OpenMP over a row list) writes the
+    # listed rows exactly as orc_trace / orc_trace_f64 over each row, and
+    # leaves the other rows zero.
+    rng = np.random.default_rng(5)
+    H, W = 6, 5
+    vol = O.new_volume(RES, LEVELS)
+    vol[...] = rng.integers(0, 2**32, vol.shape, dtype=np.uint64).astype(np.uint32) & np.uint32(0x3F3F3F3F)
+    nrm = rng.normal(size=(H, W, 3))
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    h2f = np.vectorize(lambda x: O.lib().orc_float_to_half(float(x)))
+    g = dict(
+        diffuse=rng.integers(0, 2**32, (H, W), dtype=np.uint64).astype(np.uint32),
+        normal=np.ascontiguousarray(h2f(np.concatenate([nrm * 0.5 + 0.5, np.ones((H, W, 1))], -1)), np.uint16),
+        specular=rng.integers(0, 2**32, (H, W), dtype=np.uint64).astype(np.uint32),
+        emission=np.zeros((H, W, 4), np.uint16),
+        depth=rng.uniform(0.2, 0.8, (H, W)).astype(np.float32),
+    )
+    light = dict(direction=[0.0, -1.0, 0.0], intensity=1.0, color=[1.0, 1.0, 1.0], view=np.eye(4), proj=np.eye(4),
+                 z_near=0.1, z_far=10.0)
+    shadow = np.ones((4, 4), np.float32)
+    p = _params(mode=8)
+    rows = np.array([4, 1, 3], np.int32)
+    for fp64, fn in ((False, O.lib().orc_trace), (True, O.lib().orc_trace_f64)):
+        d, s, n = O.trace_rows(vol, RES, LEVELS, g, p, shadow, light, rows, fp64=fp64)
+        dt = np.float64 if fp64 else np.float32
+        osm = O.Shadow(shadow.ctypes.data, 4, 4)
+        ol = O.light_struct(light)
+        total = 0
+        for r in rows:
+            rd, rs = np.zeros((H, W, 4), dt), np.zeros((H, W, 4), dt)
+            cnt = C.c_uint64(0)
+            fn(C.byref(O.vol_struct(vol, RES, LEVELS)), C.byref(O.gbuf_struct(g)), C.byref(p), C.byref(osm),
+               C.byref(ol), 0x3F, int(r), int(r) + 1, O.vptr(rd), O.vptr(rs), C.byref(cnt))
+            np.testing.assert_array_equal(d[r], rd[r])
+            np.testing.assert_array_equal(s[r], rs[r])
+            total += cnt.value
+        assert n == total and n > 0
+        others = np.setdiff1d(np.arange(H), rows)
+        assert not d[others].any() and not s[others].any()
+        assert d[rows].any()
+
+
+def test_alpha_above_one_saturates_like_alpha_one():
+    # A filtered alpha can exceed 1 by rounding (FP32 trilinear weights; the
+    # reference's fixed-point filter weights sum to 1 exactly). 1 - a is
+    # clamped at 0, so such samples are opaque: a volume of half-float
+    # alpha 1 + 2^-10 traces exactly as one of alpha 1 (same colour, the same
+    # occlusion, the same early exit). Without the clamp pow(negative,
+    # non-integer) is NaN, the opacity clamps to 0 after the first step and
+    # the march runs on to the maximum distance.
+    h2f = lambda x: O.lib().orc_float_to_half(float(x))  # noqa: E731
+    vols = []
+    for alpha in (1.0, 1.0009765625):
+        vol = O.new_volume(RES, LEVELS, O.FMT_RGBA16F)
+        vol[...] = np.array([h2f(0.25), h2f(0.5), h2f(0.75), h2f(alpha)], np.uint16)
+        vols.append(vol)
+    assert O.lib().orc_half_to_float(int(vols[1].reshape(-1)[3])) > 1.0
+    p = _params(center=(100.0, 100.0, 100.0))
+    start, d = np.array([0.3, 0.2, 0.1]), np.array([0.0, 0.6, 0.8])
+    for sf in (1.0, VS):
+        outs = [_trace_cone(vol, p, start, d, 0.2, sf=sf) for vol in vols]
+        (o1, n1), (o2, n2) = outs
+        assert n1 == n2 and o1[3] < 0.05
+        np.testing.assert_array_equal(o1, o2)

@@ -16,6 +16,7 @@ missing every entry point raises.
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 from dataclasses import dataclass
 
 import numpy as np
@@ -64,6 +65,9 @@ class Frame:
     frame_time: float = 0.0
 
 
+_MATERIAL_VERSIONS = itertools.count(1)
+
+
 class DeviceScene:
     """A Scene uploaded to device memory plus the vct_mesh describing it."""
 
@@ -96,6 +100,16 @@ class DeviceScene:
         m.primitive_count = len(prims)
         m.scene_bb_min = _abi.f3(*scene.bb_min.tolist())
         m.scene_bb_max = _abi.f3(*scene.bb_max.tolist())
+        textures = getattr(scene, "textures", [])
+        self.textures = [torch.from_numpy(np.ascontiguousarray(t, np.uint32).view(np.int32)).to(device)
+                         for t in textures]
+        self._tex = (_abi.Texture * max(1, len(textures)))()
+        for i, (t, d) in enumerate(zip(textures, self.textures)):
+            self._tex[i] = _abi.Texture(d.data_ptr(), int(t.shape[1]), int(t.shape[0]))
+        m.textures = C.cast(self._tex, C.POINTER(_abi.Texture)) if textures else None
+        m.texture_count = len(textures)
+        # a new material table each time: its address may be a freed one's
+        m.material_version = next(_MATERIAL_VERSIONS)
         self.mesh = m
 
 
@@ -225,6 +239,17 @@ class Context:
     def inject_radiance(self, dscene: DeviceScene, shadow, light_desc, light_shadow, desc, level, stream=None):
         _abi.call("vct_inject_radiance", self._h, _stream_handle(stream), C.byref(dscene.mesh), C.byref(shadow),
                   C.byref(light_desc), C.byref(light_shadow), C.byref(desc), level)
+
+    def voxelize_opacity_levels(self, dscene: DeviceScene, descs, stream=None):
+        """vct_voxelize_opacity_levels: the descriptors' clip levels in one call."""
+        arr = (_abi.VoxelizationDesc * len(descs))(*descs)
+        _abi.call("vct_voxelize_opacity_levels", self._h, _stream_handle(stream), C.byref(dscene.mesh), arr, len(descs))
+
+    def inject_radiance_levels(self, dscene: DeviceScene, shadow, light_desc, light_shadow, descs, stream=None):
+        """vct_inject_radiance_levels: the descriptors' clip levels in one call."""
+        arr = (_abi.VoxelizationDesc * len(descs))(*descs)
+        _abi.call("vct_inject_radiance_levels", self._h, _stream_handle(stream), C.byref(dscene.mesh),
+                  C.byref(shadow), C.byref(light_desc), C.byref(light_shadow), arr, len(descs))
 
     def clear_region(self, volume, min_corner, extent, level, face_count=6, stream=None):
         d = _abi.ImageCleaningDesc(_abi.i3(*min_corner), level, _abi.u3(*extent), self.resolution, face_count)

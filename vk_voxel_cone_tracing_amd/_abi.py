@@ -176,6 +176,10 @@ class Primitive(C.Structure):
     ]
 
 
+class Texture(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
 class Mesh(C.Structure):
     _fields_ = [
         ("positions", C.c_void_p),
@@ -192,6 +196,9 @@ class Mesh(C.Structure):
         ("primitive_count", C.c_uint32),
         ("scene_bb_min", f3),
         ("scene_bb_max", f3),
+        ("textures", C.POINTER(Texture)),
+        ("texture_count", C.c_uint32),
+        ("material_version", C.c_uint32),
     ]
 
 
@@ -287,9 +294,10 @@ STRUCT_SIZES = {
     "vct_gltf_material": (GltfMaterial, 80),
     "vct_dense_desc": (DenseDesc, 32),
     "vct_dense_trace_desc": (DenseTraceDesc, 32),
-    "vct_mesh": (Mesh, 104),
+    "vct_mesh": (Mesh, 120),
+    "vct_texture": (Texture, 16),
     "vct_gbuffer": (GBuffer, 48),
-    "vct_batch_scene": (BatchScene, 336),
+    "vct_batch_scene": (BatchScene, 352),
     "vct_batch_tuning": (BatchTuning, 16),
     "vct_batch_scene_shape": (BatchSceneShape, 16),
     "vct_specular_filter_desc": (SpecularFilterDesc, 16),
@@ -312,6 +320,12 @@ _PROTOS = {
     "vct_clear_volume": (C.c_int, [P, P, C.c_int]),
     "vct_voxelize_opacity": (C.c_int, [P, P, C.POINTER(Mesh), C.POINTER(VoxelizationDesc), C.c_uint32]),
     "vct_inject_radiance": (
+        C.c_int,
+        [P, P, C.POINTER(Mesh), C.POINTER(ShadowMap), C.POINTER(DirectionalLightDesc),
+         C.POINTER(DirectionalLightShadowDesc), C.POINTER(VoxelizationDesc), C.c_uint32],
+    ),
+    "vct_voxelize_opacity_levels": (C.c_int, [P, P, C.POINTER(Mesh), C.POINTER(VoxelizationDesc), C.c_uint32]),
+    "vct_inject_radiance_levels": (
         C.c_int,
         [P, P, C.POINTER(Mesh), C.POINTER(ShadowMap), C.POINTER(DirectionalLightDesc),
          C.POINTER(DirectionalLightShadowDesc), C.POINTER(VoxelizationDesc), C.c_uint32],
@@ -339,6 +353,8 @@ _PROTOS = {
     "vct_svo_read_nodes": (C.c_int, [P, P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vct_svo_read_fragments": (C.c_int, [P, P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vct_svo_pool": (C.c_int, [P, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
+    "vct_svo_build_bricks": (C.c_int, [P, P]),
+    "vct_svo_read_bricks": (C.c_int, [P, P, P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "vct_svo_trace": (
         C.c_int,
         [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(ConeTracingDesc), C.POINTER(ShadowMap),

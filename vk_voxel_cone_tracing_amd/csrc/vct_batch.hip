@@ -41,7 +41,7 @@
 #include "vct_texel.hpp"
 
 static_assert(sizeof(vct_batch_tuning) == 16, "vct_batch_tuning layout");
-static_assert(sizeof(vct_batch_scene) == 336, "vct_batch_scene layout");
+static_assert(sizeof(vct_batch_scene) == 352, "vct_batch_scene layout");
 static_assert(sizeof(vct_batch_scene_shape) == 16, "vct_batch_scene_shape layout");
 
 namespace vct {
@@ -689,11 +689,13 @@ static vct_status batch_init(vct_batch* b, const vct_batch_scene* scenes, uint32
             !sc.out)
             return fail(VCT_ERR_INVALID_ARGUMENT, "scene %u: G-buffer and output are required", i);
         {
-            std::vector<uint8_t> textured(m.material_count ? m.material_count : 1);
-            VCT_TRY(read_textured_flags(nullptr, m.materials, m.material_count, textured.data()));
+            std::vector<int32_t> tex(3u * (m.material_count ? m.material_count : 1u));
+            VCT_TRY(read_texture_indices(nullptr, m.materials, m.material_count, tex.data()));
             for (uint32_t k = 0; k < m.primitive_count; ++k)
                 if (m.primitives[k].index_count && m.primitives[k].material_index < m.material_count &&
-                    textured[m.primitives[k].material_index])
+                    (tex[3u * m.primitives[k].material_index] >= 0 ||
+                     tex[3u * m.primitives[k].material_index + 1] >= 0 ||
+                     tex[3u * m.primitives[k].material_index + 2] >= 0))
                     return fail(VCT_ERR_UNSUPPORTED,
                                 "scene %u primitive %u: the material references a base-colour, emissive or "
                                 "occlusion texture; the voxel passes use material factors only",

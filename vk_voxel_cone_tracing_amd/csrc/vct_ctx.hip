@@ -9,6 +9,7 @@
 #include <new>
 
 #include "vct_internal.hpp"
+#include "vct_textures.hpp"
 
 static_assert(sizeof(vct_clipmap_region) == 28, "ClipmapRegion layout");
 static_assert(sizeof(vct_voxelization_desc) == 64, "VoxelizationDesc std140 layout");
@@ -21,6 +22,8 @@ static_assert(sizeof(vct_directional_light_desc) == 32, "DirectionalLightDesc st
 static_assert(sizeof(vct_directional_light_shadow_desc) == 144, "DirectionalLightShadowDesc std140 layout");
 static_assert(sizeof(vct_camera_ubo) == 144, "CameraUBO layout");
 static_assert(sizeof(vct_gltf_material) == 80, "GltfShadeMaterial std430 layout");
+static_assert(sizeof(vct_mesh) == 120, "vct_mesh layout");
+static_assert(sizeof(vct_texture) == 16, "vct_texture layout");
 static_assert(offsetof(vct_gltf_material, emissive_factor) == 48, "GltfShadeMaterial emissiveFactor offset");
 static_assert(offsetof(vct_voxelization_desc, clip_max_extent) == 28, "VoxelizationDesc clipMaxExtent offset");
 static_assert(offsetof(vct_voxelization_desc, clipmap_resolution) == 60, "VoxelizationDesc clipmapResolution offset");
@@ -107,9 +110,10 @@ struct DevPrim {
 };
 
 __global__ void k_vertex_stage(const float* __restrict__ positions, const float* __restrict__ normals,
-                               const uint32_t* __restrict__ indices, const float* __restrict__ nodes,
-                               const DevPrim* __restrict__ prims, uint32_t nprims, uint32_t ntris,
-                               float* __restrict__ out_pos, float* __restrict__ out_nrm, uint32_t* __restrict__ out_mat)
+                               const float* __restrict__ texcoords, const uint32_t* __restrict__ indices,
+                               const float* __restrict__ nodes, const DevPrim* __restrict__ prims, uint32_t nprims,
+                               uint32_t ntris, float* __restrict__ out_pos, float* __restrict__ out_nrm,
+                               uint32_t* __restrict__ out_mat, float* __restrict__ out_uv)
 {
 #pragma clang fp contract(off)
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -132,11 +136,13 @@ __global__ void k_vertex_stage(const float* __restrict__ positions, const float*
             out_pos[9u * t + 3 * k + r] = ((m[r] * px + m[4 + r] * py) + m[8 + r] * pz) + m[12 + r];
             out_nrm[9u * t + 3 * k + r] = (it[r] * nx + it[4 + r] * ny) + it[8 + r] * nz;
         }
+        out_uv[6u * t + 2 * k] = texcoords ? texcoords[2u * vi] : 0.0f;
+        out_uv[6u * t + 2 * k + 1] = texcoords ? texcoords[2u * vi + 1] : 0.0f;
     }
     out_mat[t] = pr.material_index;
 }
 
-vct_status read_textured_flags(hipStream_t s, const vct_gltf_material* mats, uint32_t count, uint8_t* flags)
+vct_status read_texture_indices(hipStream_t s, const vct_gltf_material* mats, uint32_t count, int32_t* idx)
 {
     if (!count) return VCT_OK;
     vct_gltf_material* host = new (std::nothrow) vct_gltf_material[count];
@@ -149,57 +155,86 @@ vct_status read_textured_flags(hipStream_t s, const vct_gltf_material* mats, uin
     }
     // msaaVoxelizer.frag:64, msaaInjectRadiance.frag:73-82,132-135 and
     // voxelizer.frag:52-76 sample these three textures
-    for (uint32_t i = 0; i < count; ++i)
-        flags[i] = host[i].base_color_texture > -1 || host[i].emissive_texture > -1 || host[i].occlusion_texture > -1;
+    for (uint32_t i = 0; i < count; ++i) {
+        idx[3 * i + 0] = host[i].base_color_texture;
+        idx[3 * i + 1] = host[i].emissive_texture;
+        idx[3 * i + 2] = host[i].occlusion_texture;
+    }
     delete[] host;
     return VCT_OK;
 }
 
-static vct_status check_untextured(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh)
+// The drawn materials' texture indices against `use` and the mesh's texture
+// table. The material table is read back when its pointer, count or the
+// mesh's material_version differs from the last one read.
+static vct_status check_textures(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, TextureUse use)
 {
-    if (ctx->mat_table != (const void*)mesh->materials || ctx->mat_count != mesh->material_count) {
-        delete[] ctx->mat_textured;
-        ctx->mat_textured = nullptr;
+    if (use == TextureUse::kIgnore) return VCT_OK;
+    if (ctx->mat_table != (const void*)mesh->materials || ctx->mat_count != mesh->material_count ||
+        ctx->mat_version != mesh->material_version) {
+        delete[] ctx->mat_textures;
+        ctx->mat_textures = nullptr;
         ctx->mat_table = nullptr;
         ctx->mat_count = 0;
-        uint8_t* flags = new (std::nothrow) uint8_t[mesh->material_count ? mesh->material_count : 1];
-        if (!flags) return fail(VCT_ERR_OUT_OF_MEMORY, "host allocation failed");
-        vct_status st = read_textured_flags(s, mesh->materials, mesh->material_count, flags);
+        int32_t* idx = new (std::nothrow) int32_t[3u * (mesh->material_count ? mesh->material_count : 1u)];
+        if (!idx) return fail(VCT_ERR_OUT_OF_MEMORY, "host allocation failed");
+        vct_status st = read_texture_indices(s, mesh->materials, mesh->material_count, idx);
         if (st != VCT_OK) {
-            delete[] flags;
+            delete[] idx;
             return st;
         }
-        ctx->mat_textured = flags;
+        ctx->mat_textures = idx;
         ctx->mat_table = mesh->materials;
         ctx->mat_count = mesh->material_count;
+        ctx->mat_version = mesh->material_version;
     }
     for (uint32_t i = 0; i < mesh->primitive_count; ++i) {
         const vct_primitive& p = mesh->primitives[i];
-        if (p.index_count && p.material_index < ctx->mat_count && ctx->mat_textured[p.material_index])
-            return fail(VCT_ERR_UNSUPPORTED,
-                        "primitive %u: material %u references a base-colour, emissive or occlusion texture; the "
-                        "voxel passes use material factors only",
-                        i, p.material_index);
+        if (!p.index_count) continue;
+        for (int k = 0; k < 3; ++k) {
+            const int32_t t = ctx->mat_textures[3u * p.material_index + (uint32_t)k];
+            if (t < 0) continue;
+            if (use == TextureUse::kRefuse)
+                return fail(VCT_ERR_UNSUPPORTED,
+                            "primitive %u: material %u references a base-colour, emissive or occlusion texture; this "
+                            "pass uses material factors only",
+                            i, p.material_index);
+            if ((uint32_t)t >= mesh->texture_count || !mesh->textures)
+                return fail(VCT_ERR_INVALID_ARGUMENT, "primitive %u: material %u references texture %d of %u", i,
+                            p.material_index, t, mesh->texture_count);
+            if (!mesh->texcoords)
+                return fail(VCT_ERR_INVALID_ARGUMENT, "primitive %u: a textured material needs texcoords", i);
+        }
     }
     return VCT_OK;
 }
 
-vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, TriSoup* out)
+vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, TriSoup* out, TextureUse use)
 {
     if (!mesh || !mesh->positions || !mesh->normals || !mesh->indices || !mesh->node_matrices || !mesh->materials)
         return fail(VCT_ERR_INVALID_ARGUMENT, "mesh buffers must be non-null");
     if (!mesh->primitives && mesh->primitive_count)
         return fail(VCT_ERR_INVALID_ARGUMENT, "mesh primitive list is null");
+    if (!mesh->textures && mesh->texture_count)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "mesh texture list is null");
     for (uint32_t i = 0; i < mesh->primitive_count; ++i)
         if (mesh->primitives[i].material_index >= mesh->material_count)
             return fail(VCT_ERR_INVALID_ARGUMENT, "primitive %u material out of range", i);
-    VCT_TRY(check_untextured(ctx, s, mesh));
-    // host side of GLTFScene::cmdDraw: flatten the draws
+    for (uint32_t i = 0; i < mesh->texture_count; ++i)
+        if (!mesh->textures[i].texels || !mesh->textures[i].width || !mesh->textures[i].height ||
+            mesh->textures[i].width > 65536u || mesh->textures[i].height > 65536u)
+            return fail(VCT_ERR_INVALID_ARGUMENT, "texture %u: texels and a size in [1, 65536] are required", i);
+    VCT_TRY(check_textures(ctx, s, mesh, use));
+    // the kernels take the texture table whenever the mesh has one
+    const bool sampled = use == TextureUse::kSample && mesh->texture_count > 0;
+    // host side of GLTFScene::cmdDraw: flatten the draws; the texture table
+    // follows the primitive table in the same upload
     static thread_local DevPrim* hp = nullptr;
     static thread_local size_t hp_cap = 0;
-    if (mesh->primitive_count + 1 > hp_cap) {
+    const size_t tex_slots = ((size_t)mesh->texture_count * sizeof(DevTexture) + sizeof(DevPrim) - 1) / sizeof(DevPrim);
+    if (mesh->primitive_count + 1 + tex_slots > hp_cap) {
         delete[] hp;
-        hp_cap = mesh->primitive_count + 1;
+        hp_cap = mesh->primitive_count + 1 + tex_slots;
         hp = new (std::nothrow) DevPrim[hp_cap];
         if (!hp) {
             hp_cap = 0;
@@ -215,8 +250,6 @@ vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, T
             return fail(VCT_ERR_INVALID_ARGUMENT, "primitive %u indices out of range", i);
         if (p.instance_index >= mesh->instance_count)
             return fail(VCT_ERR_INVALID_ARGUMENT, "primitive %u instance out of range", i);
-        if (p.material_index >= mesh->material_count)
-            return fail(VCT_ERR_INVALID_ARGUMENT, "primitive %u material out of range", i);
         if (p.index_count == 0) continue;
         hp[n].first_index = p.first_index;
         hp[n].tri_count = p.index_count / 3u;
@@ -228,20 +261,32 @@ vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, T
         ++n;
     }
     if (ntris > 0xffffffffull / 9u) return fail(VCT_ERR_CAPACITY, "too many triangles");
+    DevTexture* ht = (DevTexture*)(hp + n);
+    for (uint32_t i = 0; sampled && i < mesh->texture_count; ++i)
+        ht[i] = DevTexture{mesh->textures[i].texels, mesh->textures[i].width, mesh->textures[i].height};
     out->count = (uint32_t)ntris;
     VCT_TRY(ensure(ctx->soup_pos, 36u * (ntris ? ntris : 1)));
     VCT_TRY(ensure(ctx->soup_nrm, 36u * (ntris ? ntris : 1)));
     VCT_TRY(ensure(ctx->soup_mat, 4u * (ntris ? ntris : 1)));
+    VCT_TRY(ensure(ctx->soup_uv, 24u * (ntris ? ntris : 1)));
     out->pos = (const float*)ctx->soup_pos.ptr;
     out->nrm = (const float*)ctx->soup_nrm.ptr;
     out->mat = (const uint32_t*)ctx->soup_mat.ptr;
+    out->uv = (const float*)ctx->soup_uv.ptr;
+    out->tex = nullptr;
+    out->tex_count = 0;
     if (ntris == 0) return VCT_OK;
-    VCT_TRY(upload_small(ctx, s, ctx->prims, hp, sizeof(DevPrim) * n));
+    const uint32_t ntex = sampled ? mesh->texture_count : 0u;
+    VCT_TRY(upload_small(ctx, s, ctx->prims, hp, sizeof(DevPrim) * n + sizeof(DevTexture) * ntex));
+    // null without a texture table: the kernels skip the lookups
+    out->tex = ntex ? (const DevTexture*)((const DevPrim*)ctx->prims.ptr + n) : nullptr;
+    out->tex_count = ntex;
     uint32_t threads = 256;
     uint32_t blocks = (uint32_t)((ntris + threads - 1) / threads);
     hipLaunchKernelGGL(k_vertex_stage, dim3(blocks), dim3(threads), 0, s, mesh->positions, mesh->normals,
-                       mesh->indices, mesh->node_matrices, (const DevPrim*)ctx->prims.ptr, n, (uint32_t)ntris,
-                       (float*)ctx->soup_pos.ptr, (float*)ctx->soup_nrm.ptr, (uint32_t*)ctx->soup_mat.ptr);
+                       mesh->texcoords, mesh->indices, mesh->node_matrices, (const DevPrim*)ctx->prims.ptr, n,
+                       (uint32_t)ntris, (float*)ctx->soup_pos.ptr, (float*)ctx->soup_nrm.ptr,
+                       (uint32_t*)ctx->soup_mat.ptr, (float*)ctx->soup_uv.ptr);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
@@ -279,7 +324,7 @@ extern "C" {
 
 const char* vct_version(void)
 {
-    return "vct-mi355x 0.1.0 (abi 1, gfx950)";
+    return "vct-mi355x 0.2.0 (abi 2, gfx950)";
 }
 
 const char* vct_status_string(vct_status s)
@@ -369,11 +414,12 @@ vct_status vct_destroy(vct_ctx* c)
     DeviceBuffer* bufs[] = {&c->soup_pos, &c->soup_nrm, &c->soup_mat, &c->prims, &c->prim_first, &c->counts,
                             &c->offsets, &c->scan_tmp, &c->totals, &c->rec_key[0], &c->rec_key[1], &c->rec_idx[0],
                             &c->rec_idx[1], &c->rec_val, &c->sort_tmp, &c->staging, &c->materials_flat,
-                            &c->svo.frags, &c->svo.pool, &c->svo.counters, &c->spec_perm};
+                            &c->svo.frags, &c->svo.pool, &c->svo.counters, &c->spec_perm, &c->svo.cells,
+                            &c->svo.grids, &c->svo.bricks, &c->soup_uv};
     for (DeviceBuffer* b : bufs) release(*b);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->pinned_word) (void)hipHostFree(c->pinned_word);
-    delete[] c->mat_textured;
+    delete[] c->mat_textures;
     delete c;
     return VCT_OK;
 }

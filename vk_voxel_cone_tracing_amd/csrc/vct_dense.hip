@@ -226,7 +226,7 @@ vct_status vct_dense_voxelize(vct_dense* g, vct_stream stream, const vct_mesh* m
     hipStream_t s = (hipStream_t)stream;
     vct_ctx* c = g->scratch;
     TriSoup soup;
-    VCT_TRY(run_vertex_stage(c, s, mesh, &soup));
+    VCT_TRY(run_vertex_stage(c, s, mesh, &soup, TextureUse::kRefuse));
     if (!soup.count) return VCT_OK;
     const uint32_t n9 = soup.count * 9u;
     VCT_TRY(ensure(c->rec_val, 4u * (size_t)n9));

@@ -431,7 +431,7 @@ __device__ __forceinline__ void step_coefs(const Pyramid& P, const March& M, int
 // (1 - s.a)^corr for the reverse sweep.
 __device__ __forceinline__ float replay_step(f4& result, float& occlusion, const f4& s, float corr, float q)
 {
-    const float pc = powf(1.0f - s.w, corr);
+    const float pc = powf(fmaxf(1.0f - s.w, 0.0f), corr);
     const float op = fminf(fmaxf(1.0f - pc, 0.0f), 1.0f);
     const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
     result.x += blend * (s.x * corr);
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
                     s = level_sample<true>(data, P, pos, lvl, face, w);
                 }
                 const float c = s_corr[k], iq = s_iq[k];
-                const float base_a = 1.0f - s.w;
+                const float base_a = fmaxf(1.0f - s.w, 0.0f);
                 const float pc = st_pc[k];  // base_a^c, from the replay
                 const float op = 1.0f - pc;
                 const float b = 1.0f - st_rw[k];

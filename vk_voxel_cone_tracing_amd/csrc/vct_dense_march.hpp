@@ -198,7 +198,7 @@ __device__ __forceinline__ void accumulate(f4& result, float& occlusion, const f
 {
     const float vsl = vs * exp2f(lvl);
     const float corr = seg / vsl;
-    const float op = fminf(fmaxf(1.0f - powf(1.0f - s.w, corr), 0.0f), 1.0f);
+    const float op = fminf(fmaxf(1.0f - powf(fmaxf(1.0f - s.w, 0.0f), corr), 0.0f), 1.0f);  // 1 - a >= 0
     const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
     result.x += blend * (s.x * corr);
     result.y += blend * (s.y * corr);

@@ -22,6 +22,7 @@ namespace vct {
 
 constexpr int kFaces = 6;
 constexpr int kMaxLevels = 8;
+constexpr size_t kPinnedWordBytes = 512;  // vct_ctx::pinned_word
 
 struct DeviceBuffer {
     void* ptr = nullptr;
@@ -49,13 +50,21 @@ vct_status fail(vct_status s, const char* fmt, ...);
         if (_s != VCT_OK) return _s;            \
     } while (0)
 
+struct DevTexture;
+
 // World-space triangle soup produced by the vertex stage.
 struct TriSoup {
     const float* pos;     // 9 floats per triangle
     const float* nrm;     // 9 floats per triangle
     const uint32_t* mat;  // material index per triangle
+    const float* uv;      // 6 floats per triangle (zero without texcoords)
+    const DevTexture* tex;  // the mesh's textures on the device (null without a table)
+    uint32_t tex_count;
     uint32_t count;
 };
+
+// What a pass does with materials that reference textures.
+enum class TextureUse { kSample, kRefuse, kIgnore };
 
 struct SvoState {
     DeviceBuffer frags;      // uvec2 per fragment
@@ -66,6 +75,13 @@ struct SvoState {
     uint32_t pool_nodes = 0; // capacity
     uint32_t node_count = 0; // 8 + 8 * allocated blocks, known after build
     bool built = false;
+    // pool range [begin, begin + num) of the nodes of each depth 1..L
+    uint32_t level_begin[12] = {}, level_num[12] = {};
+    uint32_t depths = 0;     // depths with nodes
+    DeviceBuffer cells;      // packed cell of every node (brick builder)
+    DeviceBuffer grids;      // dense node-colour grid of every depth (brick builder)
+    DeviceBuffer bricks;     // 27 RGBA8 per node
+    bool bricks_built = false;
 };
 
 }  // namespace vct
@@ -96,7 +112,7 @@ struct vct_ctx {
     size_t pinned_bytes = 0;
     const vct::DeviceBuffer* pinned_dst = nullptr;  // buffer that holds the pinned block's first pinned_valid bytes
     size_t pinned_valid = 0;
-    void* pinned_word = nullptr;  // pinned landing slot of small device-to-host reads
+    void* pinned_word = nullptr;  // pinned landing slot of small device-to-host reads (kPinnedWordBytes)
 
     vct::SvoState svo;
 
@@ -110,11 +126,12 @@ struct vct_ctx {
     uint32_t occ_words = 0, occ_pad = 0;
     uint64_t radiance_version = 1, mask_version = 0;
 
-    // Which materials of the last material table read back reference a
-    // texture (read once per change of the table's pointer or count).
+    // The texture indices of the last material table read back, keyed by
+    // the table's pointer, count and the mesh's material_version.
     const void* mat_table = nullptr;
-    uint32_t mat_count = 0;
-    uint8_t* mat_textured = nullptr;
+    uint32_t mat_count = 0, mat_version = 0;
+    int32_t* mat_textures = nullptr;  // base-colour, emissive, occlusion texture index per material
+    vct::DeviceBuffer soup_uv;
 };
 
 namespace vct {
@@ -161,15 +178,17 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n)
     return xcd * per + (xcd < rem ? xcd : rem) + k;
 }
 
-// Mesh vertex stage (msaaVoxelizer.vert:31-37): world-space triangles.
-// Refuses (VCT_ERR_UNSUPPORTED) a draw list whose materials reference a
-// base-colour, emissive or occlusion texture: the kernels use the material
-// factors only.
-vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, TriSoup* out);
+// Mesh vertex stage (msaaVoxelizer.vert:31-37): world-space triangles, their
+// texture coordinates and the mesh's texture table on the device. A draw
+// list whose materials reference a base-colour, emissive or occlusion
+// texture is refused (VCT_ERR_UNSUPPORTED) with TextureUse::kRefuse; with
+// kSample every such index must name one of the mesh's textures.
+vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, TriSoup* out,
+                            TextureUse use = TextureUse::kSample);
 
-// Host copy of a device material table's "references a texture the voxel
-// passes sample" flags (base colour, emissive or occlusion), one per material.
-vct_status read_textured_flags(hipStream_t s, const vct_gltf_material* mats, uint32_t count, uint8_t* flags);
+// The base-colour, emissive and occlusion texture indices of every material
+// of a device material table (3 per material), read back to the host.
+vct_status read_texture_indices(hipStream_t s, const vct_gltf_material* mats, uint32_t count, int32_t* idx);
 
 // Pinned staging copy of a small host array to a device buffer (stream ordered).
 vct_status upload_small(vct_ctx* ctx, hipStream_t s, DeviceBuffer& dst, const void* src, size_t bytes);

@@ -3,10 +3,12 @@
 // The reference rasterizes these in GBufferPass (GBufferPass.cpp:170-206) and
 // ReflectiveShadowMapPass (ReflectiveShadowMapPass.cpp:38-88); they feed the
 // hot path (cone trace and radiance injection) but are not part of it. Here
-// they are ray cast: one lane per pixel, triangles streamed through LDS in
-// tiles of 256 so that a workgroup's 256 rays test each triangle from LDS.
+// they are ray cast: one lane per pixel and one 256-lane workgroup per 16x16
+// pixel tile, which tests only the triangles binned to its tile (their
+// projected boxes), streamed through LDS 256 at a time.
 // Outputs use the reference's G-buffer channel layout.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 
@@ -52,11 +54,149 @@ __device__ __forceinline__ void mat_vec(const float* m, const float v[4], float 
     for (int k = 0; k < 4; ++k) out[k] = m[k] * v[0] + m[4 + k] * v[1] + m[8 + k] * v[2] + m[12 + k] * v[3];
 }
 
-// Casts the ray through pixel (px, py) of a viewProjInv; returns the nearest hit.
-__device__ void cast(const float* __restrict__ soup, uint32_t ntris, const float* inv, int px, int py, int w, int h,
-                     bool valid, float o[3], float d[3], float& best_t, int& best_tri, float& bu, float& bv)
+// ---- screen-tile binning: a ray is only tested against the triangles whose
+// projected box covers its 16x16 pixel tile. The nearest hit is the least
+// (t, triangle index) pair, so it does not depend on the order in which a
+// tile's list was filled, and equals the hit of a test against every
+// triangle.
+constexpr int kBinSide = 16;  // pixels per tile side; a tile is one 256-lane workgroup
+
+struct BinGrid {
+    Mat16 vp;  // world -> clip, the matrix the rays are cast with the inverse of
+    int w, h, tiles_x, tiles_y;
+};
+
+// The tiles [tx0, tx1] x [ty0, ty1] that triangle p may cover (false: none).
+// Rays start on the near plane (clip z = 0) and run forward, so only the
+// part of the triangle with clip z >= 0 can be hit: the triangle is clipped
+// to it (at most 4 vertices, all with w > 0 there) and the clipped polygon's
+// projected box taken. The box is widened by a pixel each way so that
+// rounding between the projection and the rays' inverse cannot drop a pixel
+// whose ray hits the triangle.
+__device__ __forceinline__ bool tile_box(const float* __restrict__ p, const BinGrid& G, int& tx0, int& ty0, int& tx1,
+                                         int& ty1)
+{
+    float c[3][4];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float v[4] = {p[3 * k], p[3 * k + 1], p[3 * k + 2], 1.0f};
+        mat_vec(G.vp.m, v, c[k]);
+    }
+    float xmin = 3.0e38f, ymin = 3.0e38f, xmax = -3.0e38f, ymax = -3.0e38f;
+    int n = 0;
+    bool any_w = true;
+    auto add = [&](const float q[4]) {
+        ++n;
+        if (!(q[3] > 0.0f)) {
+            any_w = false;
+            return;
+        }
+        const float x = q[0] / q[3], y = q[1] / q[3];
+        xmin = fminf(xmin, x);
+        xmax = fmaxf(xmax, x);
+        ymin = fminf(ymin, y);
+        ymax = fmaxf(ymax, y);
+    };
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float* a = c[k];
+        const float* b = c[(k + 1) % 3];
+        const bool ina = a[2] >= 0.0f, inb = b[2] >= 0.0f;
+        if (ina) add(a);
+        if (ina != inb) {
+            const float sc = a[2] / (a[2] - b[2]);
+            const float q[4] = {a[0] + sc * (b[0] - a[0]), a[1] + sc * (b[1] - a[1]), 0.0f, a[3] + sc * (b[3] - a[3])};
+            add(q);
+        }
+    }
+    if (n == 0) return false;  // wholly behind the near plane
+    if (!any_w) {
+        // a clipped vertex without a projection (a degenerate projection):
+        // every tile, conservatively
+        tx0 = 0;
+        ty0 = 0;
+        tx1 = G.tiles_x - 1;
+        ty1 = G.tiles_y - 1;
+        return true;
+    }
+    // pixel px's ray is at NDC x = (px + 0.5) / w * 2 - 1
+    const float px0 = floorf((xmin + 1.0f) * 0.5f * (float)G.w - 0.5f) - 1.0f;
+    const float px1 = ceilf((xmax + 1.0f) * 0.5f * (float)G.w - 0.5f) + 1.0f;
+    const float py0 = floorf((ymin + 1.0f) * 0.5f * (float)G.h - 0.5f) - 1.0f;
+    const float py1 = ceilf((ymax + 1.0f) * 0.5f * (float)G.h - 0.5f) + 1.0f;
+    if (!(px1 >= 0.0f) || !(py1 >= 0.0f) || !(px0 <= (float)(G.w - 1)) || !(py0 <= (float)(G.h - 1))) return false;
+    tx0 = (int)fmaxf(px0, 0.0f) / kBinSide;
+    ty0 = (int)fmaxf(py0, 0.0f) / kBinSide;
+    tx1 = (int)fminf(px1, (float)(G.w - 1)) / kBinSide;
+    ty1 = (int)fminf(py1, (float)(G.h - 1)) / kBinSide;
+    return true;
+}
+
+// Per triangle: its tile box (int4: tx0, ty0, tx1, ty1) and tile count (0:
+// culled). The (triangle, tile) pairs are then expanded one per lane, so a
+// wall covering thousands of tiles does not hold one lane for all of them.
+__global__ __launch_bounds__(256) void k_bin_boxes(const float* __restrict__ soup, uint32_t ntris, BinGrid G,
+                                                   int4* __restrict__ boxes, uint32_t* __restrict__ ntiles)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntris) return;
+    int tx0, ty0, tx1, ty1;
+    if (!tile_box(soup + 9u * t, G, tx0, ty0, tx1, ty1)) {
+        ntiles[t] = 0u;
+        return;
+    }
+    boxes[t] = make_int4(tx0, ty0, tx1, ty1);
+    ntiles[t] = (uint32_t)(tx1 - tx0 + 1) * (uint32_t)(ty1 - ty0 + 1);
+}
+
+// Pair p: its triangle (the last with toff[t] <= p) and tile.
+__device__ __forceinline__ void pair_at(const uint32_t* __restrict__ toff, uint32_t ntris,
+                                        const int4* __restrict__ boxes, int tiles_x, uint32_t p, uint32_t& t,
+                                        uint32_t& tile)
+{
+    uint32_t lo = 0, hi = ntris;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (toff[mid] <= p) lo = mid; else hi = mid;
+    }
+    t = lo;
+    const int4 b = boxes[t];
+    const uint32_t i = p - toff[t], bw = (uint32_t)(b.z - b.x + 1);
+    tile = (uint32_t)(b.y + (int)(i / bw)) * (uint32_t)tiles_x + (uint32_t)(b.x + (int)(i % bw));
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(const uint32_t* __restrict__ toff, uint32_t ntris,
+                                                   const int4* __restrict__ boxes, int tiles_x, uint32_t npairs,
+                                                   uint32_t* __restrict__ counts)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    uint32_t t, tile;
+    pair_at(toff, ntris, boxes, tiles_x, p, t, tile);
+    atomicAdd(&counts[tile], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_bin_fill(const uint32_t* __restrict__ toff, uint32_t ntris,
+                                                  const int4* __restrict__ boxes, int tiles_x, uint32_t npairs,
+                                                  const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
+                                                  uint32_t* __restrict__ lists)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    uint32_t t, tile;
+    pair_at(toff, ntris, boxes, tiles_x, p, t, tile);
+    lists[offsets[tile] + atomicAdd(&cursor[tile], 1u)] = t;
+}
+
+// Casts the ray through pixel (px, py) of a viewProjInv against the n
+// triangles of this workgroup's tile list; returns the nearest hit (least t,
+// then least triangle index).
+__device__ void cast(const float* __restrict__ soup, const uint32_t* __restrict__ list, uint32_t n, const float* inv,
+                     int px, int py, int w, int h, bool valid, float o[3], float d[3], float& best_t, int& best_tri,
+                     float& bu, float& bv)
 {
     __shared__ RayTri tile[kTile];
+    __shared__ uint32_t ids[kTile];
     if (valid) {
         const float nx = (((float)px + 0.5f) / (float)w) * 2.0f - 1.0f;
         const float ny = (((float)py + 0.5f) / (float)h) * 2.0f - 1.0f;
@@ -71,9 +211,10 @@ __device__ void cast(const float* __restrict__ soup, uint32_t ntris, const float
     }
     best_t = 3.0e38f;
     best_tri = -1;
-    for (uint32_t base = 0; base < ntris; base += kTile) {
-        const uint32_t t = base + threadIdx.x;
-        if (t < ntris) {
+    for (uint32_t base = 0; base < n; base += kTile) {
+        const uint32_t j = base + threadIdx.x;
+        if (j < n) {
+            const uint32_t t = list[j];
             const float* p = soup + 9u * t;
             RayTri r;
             for (int k = 0; k < 3; ++k) {
@@ -82,16 +223,18 @@ __device__ void cast(const float* __restrict__ soup, uint32_t ntris, const float
                 r.e2[k] = p[6 + k] - p[k];
             }
             tile[threadIdx.x] = r;
+            ids[threadIdx.x] = t;
         }
         __syncthreads();
-        const uint32_t n = ntris - base < (uint32_t)kTile ? ntris - base : (uint32_t)kTile;
+        const uint32_t m = n - base < (uint32_t)kTile ? n - base : (uint32_t)kTile;
         if (valid) {
-            for (uint32_t i = 0; i < n; ++i) {
+            for (uint32_t i = 0; i < m; ++i) {
                 float u, v;
                 const float th = intersect(o, d, tile[i], u, v);
-                if (th > 0.0f && th <= 1.0f && th < best_t) {
+                const int id = (int)ids[i];
+                if (th > 0.0f && th <= 1.0f && (th < best_t || (th == best_t && id < best_tri))) {
                     best_t = th;
-                    best_tri = (int)(base + i);
+                    best_tri = id;
                     bu = u;
                     bv = v;
                 }
@@ -104,8 +247,11 @@ __device__ void cast(const float* __restrict__ soup, uint32_t ntris, const float
 __global__ __launch_bounds__(kTile) void k_gbuffer(const float* __restrict__ soup, const float* __restrict__ nrm,
                                                    const uint32_t* __restrict__ mat,
                                                    const vct_gltf_material* __restrict__ mats,
-                                                   const float* __restrict__ surface, uint32_t ntris, int w, int h,
-                                                   Mat16 inv_m, Mat16 vp_m, uint32_t* __restrict__ g_diffuse,
+                                                   const float* __restrict__ surface,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ lists, int w, int h, Mat16 inv_m,
+                                                   Mat16 vp_m, uint32_t* __restrict__ g_diffuse,
                                                    uint2* __restrict__ g_normal, uint32_t* __restrict__ g_specular,
                                                    uint2* __restrict__ g_emission, float* __restrict__ g_depth)
 {
@@ -114,9 +260,10 @@ __global__ __launch_bounds__(kTile) void k_gbuffer(const float* __restrict__ sou
     const int px = blockIdx.x * 16 + (threadIdx.x & 15);
     const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
     const bool valid = px < w && py < h;
+    const uint32_t tile = blockIdx.y * gridDim.x + blockIdx.x;
     float o[3] = {0, 0, 0}, d[3] = {0, 0, 1}, t, bu = 0.0f, bv = 0.0f;
     int tri;
-    cast(soup, ntris, inv, px, py, w, h, valid, o, d, t, tri, bu, bv);
+    cast(soup, lists + offsets[tile], counts[tile], inv, px, py, w, h, valid, o, d, t, tri, bu, bv);
     if (!valid) return;
     const size_t idx = (size_t)py * (size_t)w + (size_t)px;
     if (tri < 0) {
@@ -146,19 +293,97 @@ __global__ __launch_bounds__(kTile) void k_gbuffer(const float* __restrict__ sou
     g_emission[idx] = pack_half(f4{mt.emissive_factor[0], mt.emissive_factor[1], mt.emissive_factor[2], 1.0f});
 }
 
-__global__ __launch_bounds__(kTile) void k_shadow(const float* __restrict__ soup, uint32_t ntris, int w, int h,
-                                                  Mat16 inv_m, float* __restrict__ depth)
+__global__ __launch_bounds__(kTile) void k_shadow(const float* __restrict__ soup, const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ counts,
+                                                  const uint32_t* __restrict__ lists, int w, int h, Mat16 inv_m,
+                                                  float* __restrict__ depth)
 {
     const float* inv = inv_m.m;
     const int px = blockIdx.x * 16 + (threadIdx.x & 15);
     const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
     const bool valid = px < w && py < h;
+    const uint32_t tile = blockIdx.y * gridDim.x + blockIdx.x;
     float o[3] = {0, 0, 0}, d[3] = {0, 0, 1}, t, bu, bv;
     int tri;
-    cast(soup, ntris, inv, px, py, w, h, valid, o, d, t, tri, bu, bv);
+    cast(soup, lists + offsets[tile], counts[tile], inv, px, py, w, h, valid, o, d, t, tri, bu, bv);
     if (!valid) return;
     // orthographic light: NDC depth is linear along the near-to-far ray
     depth[(size_t)py * (size_t)w + (size_t)px] = tri < 0 ? 1.0f : t;
+}
+
+__global__ void k_scan_total32(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ counts, uint32_t n,
+                               unsigned long long* __restrict__ total)
+{
+    total[0] = (unsigned long long)offsets[n - 1] + (unsigned long long)counts[n - 1];
+}
+
+// The tile lists of the soup for a w x h image seen through vp: per tile its
+// triangle count, the offset of its list, and the lists (in context scratch;
+// one host read-back, of the number of (triangle, tile) pairs).
+struct Bins {
+    const uint32_t *offsets, *counts, *lists;
+};
+
+static vct_status bin_triangles(vct_ctx* c, hipStream_t s, const TriSoup& soup, const Mat16& vp, int w, int h,
+                                Bins* out)
+{
+    BinGrid G;
+    G.vp = vp;
+    G.w = w;
+    G.h = h;
+    G.tiles_x = (w + kBinSide - 1) / kBinSide;
+    G.tiles_y = (h + kBinSide - 1) / kBinSide;
+    const uint32_t ntiles = (uint32_t)G.tiles_x * (uint32_t)G.tiles_y;
+    const uint32_t nt = soup.count ? soup.count : 1u;
+    VCT_TRY(ensure(c->counts, 4u * (size_t)ntiles));
+    VCT_TRY(ensure(c->offsets, 4u * (size_t)ntiles));
+    VCT_TRY(ensure(c->rec_idx[1], 4u * (size_t)ntiles));
+    VCT_TRY(ensure(c->rec_val, 16u * (size_t)nt));    // triangle boxes
+    VCT_TRY(ensure(c->rec_key[0], 4u * (size_t)nt));  // triangle tile counts
+    VCT_TRY(ensure(c->rec_key[1], 4u * (size_t)nt));  // their exclusive scan
+    VCT_TRY(ensure(c->totals, 64));
+    uint32_t* counts = (uint32_t*)c->counts.ptr;
+    uint32_t* offsets = (uint32_t*)c->offsets.ptr;
+    uint32_t* cursor = (uint32_t*)c->rec_idx[1].ptr;
+    int4* boxes = (int4*)c->rec_val.ptr;
+    uint32_t* tcount = (uint32_t*)c->rec_key[0].ptr;
+    uint32_t* toff = (uint32_t*)c->rec_key[1].ptr;
+    VCT_HIP(hipMemsetAsync(counts, 0, 4u * (size_t)ntiles, s));
+    VCT_HIP(hipMemsetAsync(cursor, 0, 4u * (size_t)ntiles, s));
+    unsigned long long npairs = 0;
+    if (soup.count) {
+        hipLaunchKernelGGL(k_bin_boxes, dim3((soup.count + 255) / 256), dim3(256), 0, s, soup.pos, soup.count, G,
+                           boxes, tcount);
+        size_t tmp = 0;
+        VCT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, tcount, toff, (int)soup.count, s));
+        VCT_TRY(ensure(c->scan_tmp, tmp));
+        VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, tcount, toff, (int)soup.count, s));
+        hipLaunchKernelGGL(k_scan_total32, dim3(1), dim3(1), 0, s, (const uint32_t*)toff, (const uint32_t*)tcount,
+                           soup.count, (unsigned long long*)c->totals.ptr);
+        if (!c->pinned_word) VCT_HIP(hipHostMalloc(&c->pinned_word, kPinnedWordBytes, hipHostMallocDefault));
+        volatile unsigned long long* landed = (volatile unsigned long long*)c->pinned_word;
+        VCT_HIP(hipMemcpyAsync((void*)landed, c->totals.ptr, 8, hipMemcpyDeviceToHost, s));
+        VCT_HIP(hipStreamSynchronize(s));
+        npairs = landed[0];
+        // the per-triangle tile counts are exact, so the 32-bit scan is exact below 2^32
+        if (npairs >= 0xffffffffull) return fail(VCT_ERR_CAPACITY, "%llu (triangle, tile) pairs exceed 2^32", npairs);
+    }
+    VCT_TRY(ensure(c->rec_idx[0], 4u * (size_t)(npairs ? npairs : 1)));
+    uint32_t* lists = (uint32_t*)c->rec_idx[0].ptr;
+    const uint32_t np = (uint32_t)npairs;
+    if (np)
+        hipLaunchKernelGGL(k_bin_count, dim3((np + 255) / 256), dim3(256), 0, s, (const uint32_t*)toff, soup.count,
+                           (const int4*)boxes, G.tiles_x, np, counts);
+    size_t tmp = 0;
+    VCT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, counts, offsets, (int)ntiles, s));
+    VCT_TRY(ensure(c->scan_tmp, tmp));
+    VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, counts, offsets, (int)ntiles, s));
+    if (np)
+        hipLaunchKernelGGL(k_bin_fill, dim3((np + 255) / 256), dim3(256), 0, s, (const uint32_t*)toff, soup.count,
+                           (const int4*)boxes, G.tiles_x, np, (const uint32_t*)offsets, cursor, lists);
+    VCT_HIP(hipGetLastError());
+    *out = Bins{offsets, counts, lists};
+    return VCT_OK;
 }
 
 static void invert4(const double* m, double* o)
@@ -197,16 +422,18 @@ vct_status vct_raycast_gbuffer(vct_ctx* c, vct_stream stream, const vct_mesh* me
         return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
     hipStream_t s = (hipStream_t)stream;
     TriSoup soup;
-    VCT_TRY(run_vertex_stage(c, s, mesh, &soup));
+    VCT_TRY(run_vertex_stage(c, s, mesh, &soup, TextureUse::kRefuse));
     Mat16 inv, vp;
     for (int i = 0; i < 16; ++i) {
         inv.m[i] = cam->view_proj_inv[i];
         vp.m[i] = cam->view_proj[i];
     }
+    Bins B;
+    VCT_TRY(bin_triangles(c, s, soup, vp, (int)width, (int)height, &B));
     dim3 grid((width + 15) / 16, (height + 15) / 16);
     hipLaunchKernelGGL(k_gbuffer, grid, dim3(kTile), 0, s, soup.pos, soup.nrm, soup.mat, mesh->materials, surface,
-                       soup.count, (int)width, (int)height, inv, vp, diffuse, (uint2*)normal, specular,
-                       (uint2*)emission, depth);
+                       B.offsets, B.counts, B.lists, (int)width, (int)height, inv, vp, diffuse, (uint2*)normal,
+                       specular, (uint2*)emission, depth);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }
@@ -218,7 +445,7 @@ vct_status vct_raycast_shadow(vct_ctx* c, vct_stream stream, const vct_mesh* mes
     if (!c || !mesh || !ls || !depth || !width || !height) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
     hipStream_t s = (hipStream_t)stream;
     TriSoup soup;
-    VCT_TRY(run_vertex_stage(c, s, mesh, &soup));
+    VCT_TRY(run_vertex_stage(c, s, mesh, &soup, TextureUse::kIgnore));
     double vp[16], inv[16];
     for (int col = 0; col < 4; ++col)
         for (int r = 0; r < 4; ++r) {
@@ -227,10 +454,16 @@ vct_status vct_raycast_shadow(vct_ctx* c, vct_stream stream, const vct_mesh* mes
             vp[col * 4 + r] = acc;
         }
     invert4(vp, inv);
-    Mat16 invf;
-    for (int i = 0; i < 16; ++i) invf.m[i] = (float)inv[i];
+    Mat16 invf, vpf;
+    for (int i = 0; i < 16; ++i) {
+        invf.m[i] = (float)inv[i];
+        vpf.m[i] = (float)vp[i];
+    }
+    Bins B;
+    VCT_TRY(bin_triangles(c, s, soup, vpf, (int)width, (int)height, &B));
     dim3 grid((width + 15) / 16, (height + 15) / 16);
-    hipLaunchKernelGGL(k_shadow, grid, dim3(kTile), 0, s, soup.pos, soup.count, (int)width, (int)height, invf, depth);
+    hipLaunchKernelGGL(k_shadow, grid, dim3(kTile), 0, s, soup.pos, B.offsets, B.counts, B.lists, (int)width,
+                       (int)height, invf, depth);
     VCT_HIP(hipGetLastError());
     return VCT_OK;
 }

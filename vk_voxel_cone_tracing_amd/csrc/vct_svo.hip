@@ -23,6 +23,7 @@
 #include "vct_raster.hpp"
 #include "vct_shadow.hpp"
 #include "vct_texel.hpp"
+#include "vct_textures.hpp"
 
 namespace vct {
 
@@ -70,7 +71,8 @@ __global__ void k_scan_total(const T* __restrict__ scan, const T* __restrict__ i
 // and consumes one fragment-list slot (voxelizer.frag:55).
 __global__ __launch_bounds__(256) void k_svo_cover(const float* __restrict__ npos,
                                                    const unsigned long long* __restrict__ offsets, uint32_t ntris,
-                                                   uint32_t nwork, SvoRaster R, uint32_t* __restrict__ covered)
+                                                   uint32_t nwork, SvoRaster R, TexInputs T,
+                                                   uint32_t* __restrict__ covered)
 {
     const uint32_t base = blockIdx.x * blockDim.x;
     const TriRange tr = block_triangles(offsets, ntris, base, min(base + blockDim.x, nwork) - 1u);
@@ -85,6 +87,12 @@ __global__ __launch_bounds__(256) void k_svo_cover(const float* __restrict__ npo
         float fp[3];
         double lam[3];
         c = tri_sample(p, s, R.vs, R.samples, (uint32_t)(wi - offsets[t]), fp, lam) ? 1u : 0u;
+        if (c && T.tex) {
+            // voxelizer.frag:52-53: occlusion-texture discard, before the slot
+            float uv[2];
+            interp_uv(T.uv + 6u * t, lam, uv);
+            if (occluded(T, T.mats[T.mat[t]], uv)) c = 0u;
+        }
     }
     covered[wi] = c;
 }
@@ -100,9 +108,8 @@ __global__ void k_svo_compact(const uint32_t* __restrict__ covered, const uint32
 // voxelizer.frag:48-100, one thread per fragment (so every lane of a wave
 // has a fragment to light), in fragment order.
 __global__ __launch_bounds__(256) void k_svo_write(const float* __restrict__ npos, const float* __restrict__ nrm,
-                                                   const uint32_t* __restrict__ mat,
-                                                   const vct_gltf_material* __restrict__ mats,
-                                                   const unsigned long long* __restrict__ offsets, uint32_t ntris,
+                                                   TexInputs T, const unsigned long long* __restrict__ offsets,
+                                                   uint32_t ntris,
                                                    uint32_t nfrag, SvoRaster R, LightParams L,
                                                    const uint32_t* __restrict__ work_of, uint2* __restrict__ frags)
 {
@@ -122,14 +129,33 @@ __global__ __launch_bounds__(256) void k_svo_write(const float* __restrict__ npo
     if (!tri_sample(p, s, R.vs, R.samples, (uint32_t)(wi - offsets[t]), fp, lam)) return;
     float q[3];
     for (int k = 0; k < 3; ++k) q[k] = (fp[k] + 1.0f) * 0.5f;  // biasAndScale (voxelizer.geom:27-30)
-    const vct_gltf_material& m = mats[mat[t]];
+    const vct_gltf_material& m = T.mats[T.mat[t]];
+    float uv[2] = {0.0f, 0.0f};
+    if (T.tex) interp_uv(T.uv + 6u * t, lam, uv);
     float rad[4] = {1.0f, 1.0f, 1.0f, 1.0f};
     if (m.emissive_factor[0] > 0.0f || m.emissive_factor[1] > 0.0f || m.emissive_factor[2] > 0.0f) {
-        for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(m.emissive_factor[k], 0.0f), 1.0f);
+        // voxelizer.frag:60-69; the emissive texture adds to the factor
+        float em[3] = {m.emissive_factor[0], m.emissive_factor[1], m.emissive_factor[2]};
+        if (has_texture(T, m.emissive_texture)) {
+            const f4 te = sample_texture(T.tex[m.emissive_texture], uv);
+            em[0] = em[0] + te.x;
+            em[1] = em[1] + te.y;
+            em[2] = em[2] + te.z;
+        }
+        for (int k = 0; k < 3; ++k) rad[k] = fminf(fmaxf(em[k], 0.0f), 1.0f);
     } else {
+        // voxelizer.frag:72-77: texture x factor (black without a texture
+        // under VCT_QUIRK_SVO_UNTEXTURED_BLACK)
         float color[4] = {0.0f, 0.0f, 0.0f, 1.0f};
         if (!(R.quirks & VCT_QUIRK_SVO_UNTEXTURED_BLACK) || m.base_color_texture > -1)
             for (int k = 0; k < 4; ++k) color[k] = m.base_color_factor[k];
+        if (has_texture(T, m.base_color_texture)) {
+            const f4 tb = sample_texture(T.tex[m.base_color_texture], uv);
+            color[0] = tb.x * color[0];
+            color[1] = tb.y * color[1];
+            color[2] = tb.z * color[2];
+            color[3] = tb.w * color[3];
+        }
         const float* n0 = nrm + 9u * t;
         float ni[3];
         for (int k = 0; k < 3; ++k)
@@ -413,7 +439,7 @@ vct_status vct_svo_voxelize(vct_ctx* c, vct_stream stream, const vct_mesh* mesh,
     VCT_TRY(ensure(c->scan_tmp, tmp));
     VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, (unsigned long long*)c->counts.ptr,
                                              (unsigned long long*)c->offsets.ptr, (int)soup.count, s));
-    if (!c->pinned_word) VCT_HIP(hipHostMalloc(&c->pinned_word, 64, hipHostMallocDefault));
+    if (!c->pinned_word) VCT_HIP(hipHostMalloc(&c->pinned_word, kPinnedWordBytes, hipHostMallocDefault));
     volatile unsigned long long* landed = (volatile unsigned long long*)c->pinned_word;
     hipLaunchKernelGGL(k_scan_total<unsigned long long>, dim3(1), dim3(1), 0, s,
                        (const unsigned long long*)c->offsets.ptr, (const unsigned long long*)c->counts.ptr, soup.count,
@@ -428,8 +454,9 @@ vct_status vct_svo_voxelize(vct_ctx* c, vct_stream stream, const vct_mesh* mesh,
     VCT_TRY(ensure(c->rec_idx[1], 4u * (size_t)nwork));
     uint32_t* covered = (uint32_t*)c->rec_idx[0].ptr;
     uint32_t* slot = (uint32_t*)c->rec_idx[1].ptr;
+    const TexInputs T{soup.uv, soup.mat, mesh->materials, soup.tex, soup.tex_count};
     hipLaunchKernelGGL(k_svo_cover, dim3(blocks_for(nwork)), dim3(256), 0, s, npos,
-                       (const unsigned long long*)c->offsets.ptr, soup.count, nwork, Rr, covered);
+                       (const unsigned long long*)c->offsets.ptr, soup.count, nwork, Rr, T, covered);
     VCT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, covered, slot, (int)nwork, s));
     VCT_TRY(ensure(c->scan_tmp, tmp));
     VCT_HIP(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, tmp, covered, slot, (int)nwork, s));
@@ -447,14 +474,122 @@ vct_status vct_svo_voxelize(vct_ctx* c, vct_stream stream, const vct_mesh* mesh,
         VCT_TRY(ensure(c->rec_key[0], 4u * (size_t)nfrag));
         uint32_t* work_of = (uint32_t*)c->rec_key[0].ptr;
         hipLaunchKernelGGL(k_svo_compact, dim3(blocks_for(nwork)), dim3(256), 0, s, covered, slot, nwork, work_of);
-        hipLaunchKernelGGL(k_svo_write, dim3(blocks_for(nfrag)), dim3(256), 0, s, npos, soup.nrm, soup.mat,
-                           mesh->materials, (const unsigned long long*)c->offsets.ptr, soup.count, nfrag, Rr, L,
+        hipLaunchKernelGGL(k_svo_write, dim3(blocks_for(nfrag)), dim3(256), 0, s, npos, soup.nrm, T,
+                           (const unsigned long long*)c->offsets.ptr, soup.count, nfrag, Rr, L,
                            work_of, (uint2*)c->svo.frags.ptr);
         VCT_HIP(hipGetLastError());
     }
     c->svo.nfrag = nfrag;
     if (nfrag_out) *nfrag_out = nfrag;
     return VCT_OK;
+}
+
+// ---- 3^3 RGBA8 bricks (build-defined; EngineConfig.h:24-25 declares brick
+// counts, no reference shader fills bricks). Texel (i, j, k) of the brick of
+// node n at depth d, cell c (resolution 2^d) is the lattice point
+// 2c + (i, j, k) in half-cell units: per channel the mean over the depth-d
+// cells touching it ({c-1, c}, {c} or {c, c+1} per axis for i = 0, 1, 2;
+// cells outside the grid count as zero) of their node colours, truncated.
+// Cells are packed x | y << 10 | z << 20 (depth <= 10).
+
+__device__ __forceinline__ uint32_t pack_cell(uint32_t x, uint32_t y, uint32_t z) { return x | (y << 10) | (z << 20); }
+
+// Root children: slot s = z | x << 1 | y << 2 (octreeNodeFlag.comp:40).
+__global__ void k_brick_root_cells(uint32_t* __restrict__ cells)
+{
+    const uint32_t s = threadIdx.x;
+    if (s < 8u) cells[s] = pack_cell((s >> 1) & 1u, (s >> 2) & 1u, s & 1u);
+}
+
+// The 8 children of every subdivided node of one depth range.
+__global__ __launch_bounds__(256) void k_brick_child_cells(const uint2* __restrict__ pool, uint32_t begin,
+                                                           uint32_t num, uint32_t node_count,
+                                                           uint32_t* __restrict__ cells)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num) return;
+    const uint32_t n = begin + i;
+    const uint32_t child = pool[n].x & 0x7fffffffu;
+    if (!child || child + 8u > node_count) return;
+    const uint32_t cc = cells[n];
+    const uint32_t x = 2u * (cc & 1023u), y = 2u * ((cc >> 10) & 1023u), z = 2u * (cc >> 20);
+#pragma unroll
+    for (uint32_t s = 0; s < 8u; ++s) cells[child + s] = pack_cell(x + ((s >> 1) & 1u), y + ((s >> 2) & 1u), z + (s & 1u));
+}
+
+// The depth's node colours into its dense grid (2^d cells per axis).
+__global__ __launch_bounds__(256) void k_brick_scatter(const uint2* __restrict__ pool, const uint32_t* __restrict__ cells,
+                                                       uint32_t begin, uint32_t num, uint32_t depth,
+                                                       uint32_t* __restrict__ grid)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num) return;
+    const uint32_t n = begin + i, cc = cells[n], r = 1u << depth;
+    const size_t at = ((size_t)(cc >> 20) * r + ((cc >> 10) & 1023u)) * r + (cc & 1023u);
+    grid[at] = pool[n].y;
+}
+
+// RGBA8 -> four 16-bit lanes of a 64-bit word (sums of up to 8 bytes fit a lane).
+__device__ __forceinline__ unsigned long long widen(uint32_t v)
+{
+    return (unsigned long long)(v & 0xffu) | ((unsigned long long)((v >> 8) & 0xffu) << 16) |
+           ((unsigned long long)((v >> 16) & 0xffu) << 32) | ((unsigned long long)(v >> 24) << 48);
+}
+
+// Per-lane floor(lane / 2^sh), repacked to RGBA8. Every lane holds a sum of
+// at most 2^sh bytes, so the quotient fits 8 bits; the mask keeps the next
+// lane's low bits (shifted down into bits 16 - sh and up) out of it.
+__device__ __forceinline__ uint32_t narrow(unsigned long long v, uint32_t sh)
+{
+    const unsigned long long m = 0xffull;
+    return (uint32_t)(((v >> sh) & m) | (((v >> (16 + sh)) & m) << 8) | (((v >> (32 + sh)) & m) << 16) |
+                      (((v >> (48 + sh)) & m) << 24));
+}
+
+__global__ __launch_bounds__(256) void k_brick_fill(const uint32_t* __restrict__ cells, uint32_t begin, uint32_t num,
+                                                    uint32_t depth, const uint32_t* __restrict__ grid,
+                                                    uint32_t* __restrict__ bricks)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num) return;
+    const uint32_t n = begin + i, cc = cells[n];
+    const int r = 1 << depth;
+    const int c[3] = {(int)(cc & 1023u), (int)((cc >> 10) & 1023u), (int)(cc >> 20)};
+    // the 3x3x3 cells around c, widened, x fastest: v[dz][dy][dx] for d in {-1, 0, 1}
+    unsigned long long v[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        const int x = c[0] + t % 3 - 1, y = c[1] + (t / 3) % 3 - 1, z = c[2] + t / 9 - 1;
+        const bool in = x >= 0 && y >= 0 && z >= 0 && x < r && y < r && z < r;
+        v[t] = in ? widen(grid[((size_t)z * (size_t)r + (size_t)y) * (size_t)r + (size_t)x]) : 0ull;
+    }
+    // separable sums: per axis texel 0 takes cells {-1, 0}, 1 takes {0}, 2 takes {0, +1}
+    unsigned long long sx[27], sy[27];
+#pragma unroll
+    for (int zy = 0; zy < 9; ++zy) {
+        sx[zy * 3 + 0] = v[zy * 3 + 0] + v[zy * 3 + 1];
+        sx[zy * 3 + 1] = v[zy * 3 + 1];
+        sx[zy * 3 + 2] = v[zy * 3 + 1] + v[zy * 3 + 2];
+    }
+#pragma unroll
+    for (int z = 0; z < 3; ++z)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            sy[(z * 3 + 0) * 3 + x] = sx[(z * 3 + 0) * 3 + x] + sx[(z * 3 + 1) * 3 + x];
+            sy[(z * 3 + 1) * 3 + x] = sx[(z * 3 + 1) * 3 + x];
+            sy[(z * 3 + 2) * 3 + x] = sx[(z * 3 + 1) * 3 + x] + sx[(z * 3 + 2) * 3 + x];
+        }
+    uint32_t* out = bricks + 27u * (size_t)n;
+#pragma unroll
+    for (int y = 0; y < 3; ++y)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            const unsigned long long a = sy[(0 * 3 + y) * 3 + x], b = sy[(1 * 3 + y) * 3 + x], d = sy[(2 * 3 + y) * 3 + x];
+            const uint32_t shxy = (uint32_t)(x != 1) + (uint32_t)(y != 1);
+            out[(0 * 3 + y) * 3 + x] = narrow(a + b, shxy + 1u);
+            out[(1 * 3 + y) * 3 + x] = narrow(b, shxy);
+            out[(2 * 3 + y) * 3 + x] = narrow(b + d, shxy + 1u);
+        }
 }
 
 // OctreeBuilder::cmdBuild (OctreeBuilder.cpp:205-345).
@@ -482,7 +617,7 @@ vct_status vct_svo_build(vct_ctx* c, vct_stream stream)
     VCT_TRY(ensure(c->svo.counters, 16));
     uint32_t* dev_counter = (uint32_t*)c->svo.counters.ptr;
     VCT_HIP(hipMemsetAsync(dev_counter, 0, 4, s));
-    if (!c->pinned_word) VCT_HIP(hipHostMalloc(&c->pinned_word, 64, hipHostMallocDefault));
+    if (!c->pinned_word) VCT_HIP(hipHostMalloc(&c->pinned_word, kPinnedWordBytes, hipHostMallocDefault));
     volatile uint32_t* host_counter = (volatile uint32_t*)c->pinned_word;
     for (uint32_t i = 1; i <= L; ++i) {
         if (num == 0) break;  // no node was flagged: the tree ends here (the reference's zero-group dispatches)
@@ -561,7 +696,68 @@ vct_status vct_svo_build(vct_ctx* c, vct_stream stream)
                            level_begin[depth - 1], level_num[depth - 1]);
     }
     VCT_HIP(hipGetLastError());
+    c->svo.depths = (uint32_t)level_begin.size();
+    for (uint32_t d = 0; d < c->svo.depths; ++d) {
+        c->svo.level_begin[d] = level_begin[d];
+        c->svo.level_num[d] = level_num[d];
+    }
     c->svo.built = true;
+    c->svo.bricks_built = false;
+    return VCT_OK;
+}
+
+vct_status vct_svo_build_bricks(vct_ctx* c, vct_stream stream)
+{
+    if (!c) return fail(VCT_ERR_INVALID_ARGUMENT, "null context");
+    if (!c->svo.built) return fail(VCT_ERR_NOT_READY, "vct_svo_build has not run");
+    const uint32_t L = c->svo.level;
+    if (L > 10) return fail(VCT_ERR_UNSUPPORTED, "bricks are built for octree levels up to 10 (level %u)", L);
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t n = c->svo.node_count;
+    size_t grid_total = 0;
+    unsigned long long off[12] = {};
+    for (uint32_t d = 1; d <= c->svo.depths; ++d) {
+        off[d] = grid_total;
+        grid_total += (size_t)1 << (3 * d);
+    }
+    VCT_TRY(ensure(c->svo.cells, 4u * (size_t)n));
+    VCT_TRY(ensure(c->svo.grids, 4u * (grid_total ? grid_total : 1)));
+    VCT_TRY(ensure(c->svo.bricks, 4u * 27u * (size_t)n));
+    const uint2* pool = (const uint2*)c->svo.pool.ptr;
+    uint32_t* cells = (uint32_t*)c->svo.cells.ptr;
+    uint32_t* grids = (uint32_t*)c->svo.grids.ptr;
+    VCT_HIP(hipMemsetAsync(grids, 0, 4u * grid_total, s));
+    hipLaunchKernelGGL(k_brick_root_cells, dim3(1), dim3(64), 0, s, cells);
+    for (uint32_t d = 1; d <= c->svo.depths; ++d) {
+        const uint32_t b = c->svo.level_begin[d - 1], num = c->svo.level_num[d - 1];
+        if (!num) continue;
+        if (d < c->svo.depths)
+            hipLaunchKernelGGL(k_brick_child_cells, dim3(blocks_for(num)), dim3(256), 0, s, pool, b, num, n, cells);
+        hipLaunchKernelGGL(k_brick_scatter, dim3(blocks_for(num)), dim3(256), 0, s, pool, cells, b, num, d,
+                           grids + off[d]);
+    }
+    for (uint32_t d = 1; d <= c->svo.depths; ++d) {
+        const uint32_t b = c->svo.level_begin[d - 1], num = c->svo.level_num[d - 1];
+        if (!num) continue;
+        hipLaunchKernelGGL(k_brick_fill, dim3(blocks_for(num)), dim3(256), 0, s, (const uint32_t*)cells, b, num, d,
+                           (const uint32_t*)(grids + off[d]), (uint32_t*)c->svo.bricks.ptr);
+    }
+    VCT_HIP(hipGetLastError());
+    c->svo.bricks_built = true;
+    return VCT_OK;
+}
+
+vct_status vct_svo_read_bricks(vct_ctx* c, vct_stream stream, uint32_t* out, size_t cap_bricks, size_t* n_bricks)
+{
+    if (!c || !n_bricks) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->svo.bricks_built) return fail(VCT_ERR_NOT_READY, "vct_svo_build_bricks has not run");
+    *n_bricks = c->svo.node_count;
+    if (!out) return VCT_OK;
+    if (cap_bricks < c->svo.node_count)
+        return fail(VCT_ERR_CAPACITY, "output holds %zu bricks, the octree has %u", cap_bricks, c->svo.node_count);
+    hipStream_t s = (hipStream_t)stream;
+    VCT_HIP(hipMemcpyAsync(out, c->svo.bricks.ptr, 4u * 27u * (size_t)c->svo.node_count, hipMemcpyDeviceToHost, s));
+    VCT_HIP(hipStreamSynchronize(s));
     return VCT_OK;
 }
 

@@ -264,7 +264,10 @@ __device__ __forceinline__ void cone_accumulate(const TraceParams& P, f4& result
 {
     const float voxel_size = P.voxel_size * exp2f(cur);
     const float correction = seg * (P.inv_voxel_size * exp2f(-cur));
-    float opacity = 1.0f - __builtin_amdgcn_exp2f(correction * __builtin_amdgcn_logf(1.0f - s.w));
+    // 1 - a clamped at 0: a filtered alpha of texels <= 1 can exceed 1 by an
+    // ulp in FP32 (the reference's fixed-point filter weights sum to 1
+    // exactly), and pow of a negative base is undefined
+    float opacity = 1.0f - __builtin_amdgcn_exp2f(correction * __builtin_amdgcn_logf(fmaxf(1.0f - s.w, 0.0f)));
     opacity = fminf(fmaxf(opacity, 0.0f), 1.0f);
     const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
     result.x += blend * (s.x * correction);
@@ -770,7 +773,7 @@ __device__ f4 svo_trace_cone(const uint2* __restrict__ pool, const TraceParams& 
                       lo.w * (1.0f - fl) + hi.w * fl};
         voxel_size = P.voxel_size * exp2f(cur);
         const float correction = seg / voxel_size;
-        const float opacity = fminf(fmaxf(1.0f - powf(1.0f - s.w, correction), 0.0f), 1.0f);
+        const float opacity = fminf(fmaxf(1.0f - powf(fmaxf(1.0f - s.w, 0.0f), correction), 0.0f), 1.0f);
         const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
         result.x += blend * (s.x * correction);
         result.y += blend * (s.y * correction);

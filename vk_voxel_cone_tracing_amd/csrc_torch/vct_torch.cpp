@@ -9,17 +9,23 @@
 //
 // The forward builds the level-0 texels, the anisotropic mips and traces
 // every view (vct_dense_set_level0 / vct_dense_mipmap / vct_dense_trace).
-// The backward rebuilds the pyramid from the saved inputs (the grid object is
-// shared between calls), zeroes a gradient pyramid, runs
-// vct_dense_trace_backward per view, vct_dense_mipmap_backward, and returns
-// d loss / d albedo and d loss / d opacity. All work is enqueued on the
-// current HIP stream of PyTorch. `cameras` rows are view_proj (16),
+// The backward reuses that pyramid when the grid still holds it (an id per
+// build, recorded by the forward; else it builds it again from the saved
+// inputs; the grid object is shared between calls),
+// zeroes a gradient pyramid, runs vct_dense_trace_backward per view,
+// vct_dense_mipmap_backward, and returns d loss / d albedo and d loss /
+// d opacity. All work is enqueued on the current HIP stream of PyTorch, and
+// the backward does not wait for it: a trace status set on the device turns
+// the gradients' first element into NaN there, and the next backward (or
+// check_status()) raises on it. `cameras` rows are view_proj (16),
 // view_proj_inv (16), eye (3), padding (1), column-major as glm stores them.
 #include <torch/extension.h>
 #include <ATen/hip/HIPContext.h>
 
 #include <array>
+#include <cmath>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -47,10 +53,22 @@ struct GridSlot {
     GridKey key{};
     vct_dense* grid = nullptr;
     size_t texels = 0;
+    // id of the pyramid the grid holds (0: none); every build takes a new one
+    int64_t build = 0;
 };
 
 std::mutex g_mu;
-std::vector<GridSlot> g_grids;
+std::deque<GridSlot> g_grids;  // stable addresses: slots are handed out by reference
+int64_t g_builds = 0;          // pyramid ids handed out
+
+// The trace backward's status word of the previous backward, copied to pinned
+// memory without waiting; checked at the start of the next backward.
+struct PendingStatus {
+    int32_t* host = nullptr;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+PendingStatus g_status;
 
 GridSlot& grid_for(int device, int64_t res, const std::array<float, 4>& box)
 {
@@ -137,15 +155,53 @@ void check_inputs(const torch::Tensor& albedo, const torch::Tensor& opacity, con
                 "cameras: contiguous float32 host tensor [V, 36]");
 }
 
-// Level 0 and the mips of `albedo`/`opacity` in the shared grid.
+// Level 0 and the mips of `albedo`/`opacity` in the shared grid. A grid that
+// still holds pyramid `reuse` (the id the forward recorded; autograd refuses
+// a backward whose saved inputs changed in place) is used as it is: the
+// backward reuses the forward's pyramid unless another render ran in between.
 GridSlot& build_pyramid(const torch::Tensor& albedo, const torch::Tensor& opacity, const std::array<float, 4>& box,
-                        hipStream_t s)
+                        hipStream_t s, int64_t reuse)
 {
     GridSlot& slot = grid_for(albedo.get_device(), albedo.size(0), box);
+    if (reuse != 0 && slot.build == reuse) return slot;
     check(vct_dense_set_level0(slot.grid, s, albedo.data_ptr<float>(), opacity.data_ptr<float>()),
           "vct_dense_set_level0");
     check(vct_dense_mipmap(slot.grid, s), "vct_dense_mipmap");
+    std::lock_guard<std::mutex> lock(g_mu);
+    slot.build = ++g_builds;
     return slot;
+}
+
+// Raises if the previous backward's trace status was set (its gradient was
+// already poisoned with NaN on the device).
+void check_previous_status()
+{
+    if (!g_status.pending) return;
+    g_status.pending = false;
+    check(hipEventSynchronize(g_status.done) == hipSuccess ? VCT_OK : VCT_ERR_HIP, "status event");
+    const int32_t st = *g_status.host;
+    TORCH_CHECK(st == 0, "a previous vct_dense_trace_backward reported status ", st,
+                " (bit 0: a march outgrew its per-lane state; bit 1: the gradient records outgrew their buffer); "
+                "its gradient holds NaN");
+}
+
+// Copies the status word to pinned memory behind the backward's work and
+// poisons the gradients with NaN on the device when it is set (no host wait).
+void defer_status(const torch::Tensor& status, torch::Tensor& ga, torch::Tensor& go, hipStream_t s)
+{
+    if (!g_status.host) {
+        check(hipHostMalloc((void**)&g_status.host, sizeof(int32_t), hipHostMallocDefault) == hipSuccess
+                  ? VCT_OK : VCT_ERR_OUT_OF_MEMORY, "pinned status word");
+        check(hipEventCreateWithFlags(&g_status.done, hipEventDisableTiming) == hipSuccess ? VCT_OK : VCT_ERR_HIP,
+              "status event");
+    }
+    const auto bad = status.ne(0);
+    ga.view(-1).narrow(0, 0, 1).masked_fill_(bad, std::nanf(""));
+    go.view(-1).narrow(0, 0, 1).masked_fill_(bad, std::nanf(""));
+    check(hipMemcpyAsync(g_status.host, status.data_ptr<int32_t>(), sizeof(int32_t), hipMemcpyDeviceToHost, s) ==
+              hipSuccess ? VCT_OK : VCT_ERR_HIP, "status copy");
+    check(hipEventRecord(g_status.done, s) == hipSuccess ? VCT_OK : VCT_ERR_HIP, "status event");
+    g_status.pending = true;
 }
 
 std::array<float, 4> box_of(const std::vector<double>& grid)
@@ -167,7 +223,7 @@ public:
         const auto box = box_of(grid);
         const vct_dense_trace_desc td = trace_desc(trace);
         hipStream_t s = c10::hip::getCurrentHIPStream().stream();
-        GridSlot& slot = build_pyramid(albedo, opacity, box, s);
+        GridSlot& slot = build_pyramid(albedo, opacity, box, s, 0);
         const int64_t V = depth.size(0), H = depth.size(1), W = depth.size(2);
         auto out = torch::empty({V, H, W, 4}, albedo.options());
         for (int64_t v = 0; v < V; ++v) {
@@ -180,6 +236,7 @@ public:
         ctx->saved_data["grid"] = grid;
         ctx->saved_data["trace"] = trace;
         ctx->saved_data["flags"] = flags;
+        ctx->saved_data["build"] = slot.build;
         return out;
     }
 
@@ -193,7 +250,8 @@ public:
         const vct_dense_trace_desc td = trace_desc(ctx->saved_data["trace"].toDoubleVector());
         const uint32_t flags = (uint32_t)ctx->saved_data["flags"].toInt();
         hipStream_t s = c10::hip::getCurrentHIPStream().stream();
-        GridSlot& slot = build_pyramid(albedo, opacity, box, s);
+        check_previous_status();
+        GridSlot& slot = build_pyramid(albedo, opacity, box, s, ctx->saved_data["build"].toInt());
         torch::Tensor gout = grads[0].contiguous();
         auto gpyr = torch::zeros({(int64_t)slot.texels * 4}, albedo.options());
         auto status = torch::zeros({1}, albedo.options().dtype(torch::kInt32));
@@ -210,9 +268,7 @@ public:
         auto go = torch::empty_like(opacity);
         check(vct_dense_level0_grad(slot.grid, s, gpyr.data_ptr<float>(), ga.data_ptr<float>(), go.data_ptr<float>()),
               "vct_dense_level0_grad");
-        const int32_t st = status.item<int32_t>();
-        TORCH_CHECK(st == 0, "vct_dense_trace_backward status ", st,
-                    " (bit 0: a march outgrew its per-lane state; bit 1: the gradient records outgrew their buffer)");
+        defer_status(status, ga, go, s);
         return {ga, go, torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(),
                 torch::Tensor(), torch::Tensor()};
     }
@@ -232,6 +288,9 @@ void release_grids()
     g_grids.clear();
 }
 
+// Waits for the last backward and raises if its trace status was set.
+void check_status() { check_previous_status(); }
+
 }  // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
@@ -239,4 +298,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.doc() = "dense-method render with a hand-written adjoint (torch.autograd.Function over libvct)";
     m.def("render", &render, "differentiable dense VXAO / indirect diffuse render");
     m.def("release_grids", &release_grids, "frees the cached device grids");
+    m.def("check_status", &check_status, "raises if the last backward's trace status was set");
 }

@@ -23,7 +23,12 @@ GLTFScene::uploadMatrixBuffer / cmdDraw (VFS/GLTFScene.cpp:398-409, 457-489):
   tree the two disagree. Here the instance index is the mesh node's ordinal.)
 * materials with the glTF defaults tinygltf fills in;
 * the scene box from each primitive's accessor min/max transformed by its
-  node's world matrix, as calculateSceneDimension does (:517-540).
+  node's world matrix, as calculateSceneDimension does (:517-540);
+* textures: texture i is the image of glTF texture i's source (:118-123),
+  decoded to RGBA8 as tinygltf's stb_image loader delivers it (RGB gets
+  alpha 255, grey is replicated, 16-bit channels keep their high byte).
+  Images are decoded with Pillow (PNG decoding is lossless, so identical;
+  a JPEG decoder may differ from stb_image's by a count in some texels).
 
 Supports .gltf (external .bin files or base64 data URIs) and .glb.
 """
@@ -87,6 +92,44 @@ def _accessor(doc, buffers, idx) -> np.ndarray:
         rows = [np.frombuffer(buf, dt, nc, start + r * stride) for r in range(count)]
         a = np.stack(rows) if rows else np.zeros((0, nc), dt)
     return a if nc > 1 else a.reshape(count)
+
+
+def _decode_image(data: bytes) -> np.ndarray:
+    """RGBA8 texels (H, W) uint32, R in the low byte."""
+    import io
+
+    try:
+        from PIL import Image
+    except ImportError as e:  # pragma: no cover - Pillow ships with this image
+        raise RuntimeError("decoding glTF images needs Pillow") from e
+    im = Image.open(io.BytesIO(data))
+    if im.mode in ("I;16", "I;16B", "I;16L", "I"):
+        g = (np.asarray(im, np.uint32) >> 8).astype(np.uint8)  # stb_image keeps the high byte
+        rgba = np.stack([g, g, g, np.full_like(g, 255)], -1)
+    else:
+        rgba = np.asarray(im.convert("RGBA"), np.uint8)
+    r = rgba.astype(np.uint32)
+    return np.ascontiguousarray(r[..., 0] | (r[..., 1] << 8) | (r[..., 2] << 16) | (r[..., 3] << 24))
+
+
+def _image_bytes(path: Path, doc, buffers, img: dict) -> bytes:
+    if "uri" in img:
+        return _read_uri(path, img["uri"])
+    bv = doc["bufferViews"][img["bufferView"]]
+    start = bv.get("byteOffset", 0)
+    return bytes(buffers[bv["buffer"]][start: start + bv["byteLength"]])
+
+
+def _encode_png(texels: np.ndarray) -> bytes:
+    import io
+
+    from PIL import Image
+
+    t = np.ascontiguousarray(texels, np.uint32)
+    rgba = np.stack([(t >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.uint8)
+    out = io.BytesIO()
+    Image.fromarray(rgba, "RGBA").save(out, format="PNG")
+    return out.getvalue()
 
 
 def _quat_to_mat4(q):
@@ -248,11 +291,13 @@ def load(path) -> Scene:
             hi = np.maximum(hi, p)
     if np.array_equal(lo, hi) or not draws:
         lo, hi = np.full(3, -1.0), np.full(3, 1.0)
+    textures = [_decode_image(_image_bytes(path, doc, buffers, doc["images"][t["source"]]))
+                for t in doc.get("textures", [])]
     cat = lambda xs, shape, dt: np.concatenate(xs).astype(dt) if xs else np.zeros(shape, dt)  # noqa: E731
     return Scene(positions=cat(positions, (0, 3), np.float32), normals=cat(normals, (0, 3), np.float32),
                  texcoords=cat(texcoords, (0, 2), np.float32), indices=cat(indices, (0,), np.uint32), nodes=nodes,
                  materials=materials, primitives=prims, surface=surface, bb_min=lo.astype(np.float32),
-                 bb_max=hi.astype(np.float32))
+                 bb_max=hi.astype(np.float32), textures=textures)
 
 
 def save(scene: Scene, path, binary: bool = False) -> None:
@@ -293,13 +338,33 @@ def save(scene: Scene, path, binary: bool = False) -> None:
         nodes.append({"mesh": len(meshes) - 1, "matrix": [float(x) for x in scene.nodes[inst, :16]]})
     mats = []
     for m in scene.materials:
-        mats.append({"pbrMetallicRoughness": {"baseColorFactor": [float(x) for x in m["base_color_factor"]],
-                                              "metallicFactor": float(m["metallic_factor"]),
-                                              "roughnessFactor": float(m["roughness_factor"])},
-                     "emissiveFactor": [float(x) for x in m["emissive_factor"]]})
+        mat = {"pbrMetallicRoughness": {"baseColorFactor": [float(x) for x in m["base_color_factor"]],
+                                        "metallicFactor": float(m["metallic_factor"]),
+                                        "roughnessFactor": float(m["roughness_factor"])},
+               "emissiveFactor": [float(x) for x in m["emissive_factor"]]}
+        if m["base_color_texture"] >= 0:
+            mat["pbrMetallicRoughness"]["baseColorTexture"] = {"index": int(m["base_color_texture"])}
+        if m["emissive_texture"] >= 0:
+            mat["emissiveTexture"] = {"index": int(m["emissive_texture"])}
+        if m["occlusion_texture"] >= 0:
+            mat["occlusionTexture"] = {"index": int(m["occlusion_texture"])}
+        mats.append(mat)
+    images = []
+    for t in getattr(scene, "textures", []):
+        png = _encode_png(t)
+        while len(blob) % 4:
+            blob.append(0)
+        views.append({"buffer": 0, "byteOffset": len(blob), "byteLength": len(png)})
+        blob.extend(png)
+        images.append({"bufferView": len(views) - 1, "mimeType": "image/png"})
     doc = {"asset": {"version": "2.0"}, "scene": 0, "scenes": [{"nodes": list(range(len(nodes)))}], "nodes": nodes,
            "meshes": meshes, "materials": mats, "accessors": accessors, "bufferViews": views,
            "buffers": [{"byteLength": len(blob)}]}
+    if images:
+        # REPEAT / LINEAR, the sampler the reference creates for every texture
+        doc["samplers"] = [{"magFilter": 9729, "minFilter": 9729, "wrapS": 10497, "wrapT": 10497}]
+        doc["images"] = images
+        doc["textures"] = [{"source": i, "sampler": 0} for i in range(len(images))]
     if binary:
         js = json.dumps(doc).encode("utf-8")
         js += b" " * (-len(js) % 4)

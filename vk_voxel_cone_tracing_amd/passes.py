@@ -98,9 +98,10 @@ class VoxelizationPass(RenderPassBase):
                     desc = self.ctx.voxelization_desc(level, region)
                     self.ctx.voxelize_opacity(self.scene, desc, level, frame.stream)
             return
-        for level in range(self.ctx.levels):
-            desc = self.ctx.voxelization_desc(level)
-            self.ctx.voxelize_opacity(self.scene, desc, level, frame.stream)
+        # every level in one call (vct_voxelize_opacity_levels): the same
+        # texels as one vct_voxelize_opacity per level
+        descs = [self.ctx.voxelization_desc(level) for level in range(self.ctx.levels)]
+        self.ctx.voxelize_opacity_levels(self.scene, descs, frame.stream)
 
     def on_end_render_pass(self, frame):
         regions = self.ctx.regions
@@ -145,11 +146,13 @@ class RadianceInjectionPass(RenderPassBase):
                     self.ctx.clear_region(VOLUME_RADIANCE, (0, 0, 0), (res, res, res), level, 6, frame.stream)
 
     def on_update(self, frame):
-        for level in range(self.ctx.levels):
-            if self._due(level):
-                desc = self.ctx.voxelization_desc(level)
-                self.ctx.inject_radiance(self.scene, self.shadow, self.light_desc, self.light_shadow, desc, level,
-                                         frame.stream)
+        # the due levels in one call (vct_inject_radiance_levels): one host
+        # wait and one sort for all of them, the same texels as one
+        # vct_inject_radiance per level
+        descs = [self.ctx.voxelization_desc(level) for level in range(self.ctx.levels) if self._due(level)]
+        if descs:
+            self.ctx.inject_radiance_levels(self.scene, self.shadow, self.light_desc, self.light_shadow, descs,
+                                            frame.stream)
 
     def on_end_render_pass(self, frame):
         for level in range(self.ctx.levels):

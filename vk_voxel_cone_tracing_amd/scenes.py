@@ -15,7 +15,13 @@ configuration index):
 * thin: the room plus thin triangles (edges U[0.001, 0.01]) up to a total
   triangle count (1 000 000 for the 512^3 octree workload);
 * every coordinate is multiplied by `scale` (16 for the 6 x 128^3 clipmap
-  workload, so level 0 spans the room).
+  workload, so level 0 spans the room);
+* textured=True adds texture coordinates (walls tiled 3 x 3, box faces and
+  a spherical map on the spheres) and four RGBA8 textures: a 16 x 16 and a
+  5 x 3 base colour, an 8 x 8 emissive and a 32 x 32 occlusion texture whose
+  red channel is below 0.1 on about a quarter of its texels. Walls, boxes and
+  spheres take them in turn; the texture draws use their own generator, so
+  the geometry and factors equal the untextured room's.
 """
 from __future__ import annotations
 
@@ -57,10 +63,22 @@ class Scene:
     surface: np.ndarray  # (M, 5) float32: perceptual roughness, metallic, specular rgb
     bb_min: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
     bb_max: np.ndarray = field(default_factory=lambda: np.ones(3, np.float32))
+    textures: list = field(default_factory=list)  # (H, W) uint32 RGBA8 arrays, R in the low byte
 
     @property
     def triangle_count(self) -> int:
         return int(self.primitives[:, 1].sum() // 3)
+
+    def triangle_uvs(self) -> np.ndarray:
+        """The three vertices' texcoords of every triangle, (T, 6) float32, in
+        the order of world_triangles()."""
+        out = []
+        for first, count, voff, inst, mat in self.primitives:
+            if count == 0:
+                continue
+            idx = self.indices[first : first + count].astype(np.int64) + int(np.int32(voff))
+            out.append(self.texcoords[idx].reshape(-1, 6))
+        return np.ascontiguousarray(np.concatenate(out).astype(np.float32))
 
     def world_triangles(self):
         """The vertex stage (msaaVoxelizer.vert:31-37) on the host, in the
@@ -161,12 +179,14 @@ def unit_box():
 class _Builder:
     def __init__(self):
         self.P, self.N, self.I, self.nodes, self.mats, self.prims, self.surf = [], [], [], [], [], [], []
+        self.UV, self.textures = [], []
         self.nv = 0
         self.ni = 0
 
-    def mesh(self, P, N, I):
+    def mesh(self, P, N, I, UV=None):
         """Adds vertex/index data; returns (first_index, index_count, vertex_offset)."""
         first, voff = self.ni, self.nv
+        self.UV.append(np.zeros((len(P), 2), np.float32) if UV is None else np.asarray(UV, np.float32))
         self.P.append(P)
         self.N.append(N)
         self.I.append(I)
@@ -194,7 +214,7 @@ class _Builder:
         return Scene(
             positions=P,
             normals=np.concatenate(self.N).astype(np.float32),
-            texcoords=np.zeros((self.nv, 2), np.float32),
+            texcoords=np.concatenate(self.UV).astype(np.float32),
             indices=np.concatenate(self.I).astype(np.uint32),
             nodes=np.stack(self.nodes).astype(np.float32),
             materials=np.array(self.mats, MATERIAL_DTYPE),
@@ -202,6 +222,7 @@ class _Builder:
             surface=np.array(self.surf, np.float32),
             bb_min=lo.astype(np.float32),
             bb_max=hi.astype(np.float32),
+            textures=list(self.textures),
         )
 
 
@@ -210,11 +231,41 @@ def _surface(rng):
     return (rng.uniform(0.1, 0.8), rng.uniform(0.1, 0.9), *rng.uniform(0.2, 0.8, 3))
 
 
+def _texture(rng, h, w, lo=0, hi=256):
+    t = rng.integers(lo, hi, (h, w, 4)).astype(np.uint32)
+    return np.ascontiguousarray(t[..., 0] | (t[..., 1] << 8) | (t[..., 2] << 16) | (t[..., 3] << 24))
+
+
+def _room_textures(seed: int):
+    """Base colour 16 x 16 and 5 x 3 (opaque), emissive 8 x 8, occlusion
+    32 x 32 (red below 0.1 on about a quarter of the texels)."""
+    rng = np.random.Generator(np.random.PCG64(5000 + seed))
+    base0 = _texture(rng, 16, 16) | np.uint32(0xFF000000)
+    base1 = _texture(rng, 3, 5) | np.uint32(0xFF000000)
+    emis = _texture(rng, 8, 8)
+    red = np.where(rng.uniform(size=(32, 32)) < 0.25, rng.integers(0, 20, (32, 32)), rng.integers(40, 256, (32, 32)))
+    occ = (_texture(rng, 32, 32) & np.uint32(0xFFFFFF00)) | red.astype(np.uint32)
+    return [base0, base1, emis, occ]
+
+
+def _with_textures(m, base=-1, emissive=-1, occlusion=-1):
+    m = m.copy()
+    m["base_color_texture"] = base
+    m["emissive_texture"] = emissive
+    m["occlusion_texture"] = occlusion
+    return m
+
+
 def room(seed: int, scale: float = 1.0, n_boxes: int = 64, n_spheres: int = 16, thin_total: int = 0,
-         emissive_boxes: int = 0) -> Scene:
+         emissive_boxes: int = 0, textured: bool = False) -> Scene:
     rng = np.random.Generator(np.random.PCG64(seed))
     S = np.diag([scale, scale, scale, 1.0])
     b = _Builder()
+    if textured:
+        b.textures = _room_textures(seed)
+    # texture indices by object number: (base colour, emissive, occlusion)
+    tex_of = lambda k, emissive: (-1 if not textured else (
+        (-1, 2 if emissive else -1, 3) if k % 4 == 3 else (k % 2, 2 if emissive else -1, 3 if k % 3 == 0 else -1)))
     # 12 inward-facing wall triangles, one primitive per wall
     for axis in range(3):
         for side in (0.0, 1.0):
@@ -228,24 +279,37 @@ def room(seed: int, scale: float = 1.0, n_boxes: int = 64, n_spheres: int = 16, 
             quad = [c + u * cu + w * cw for cu, cw in ((-0.5, -0.5), (0.5, -0.5), (0.5, 0.5), (-0.5, 0.5))]
             P = np.array(quad, np.float32)
             N = np.tile(n.astype(np.float32), (4, 1))
-            ref = b.mesh(P, N, np.array([0, 1, 2, 0, 2, 3], np.uint32))
-            b.draw(ref, S, _material(rng.uniform(0.2, 0.8, 3)), _surface(rng))
+            UV = np.array([(0.0, 0.0), (3.0, 0.0), (3.0, 3.0), (0.0, 3.0)], np.float32)  # tiled 3 x 3 (REPEAT)
+            ref = b.mesh(P, N, np.array([0, 1, 2, 0, 2, 3], np.uint32), UV)
+            mat = _material(rng.uniform(0.2, 0.8, 3))
+            if textured:
+                mat = _with_textures(mat, *tex_of(2 * axis + int(side), False))
+            b.draw(ref, S, mat, _surface(rng))
     bp, bn, bi = unit_box()
-    box_ref = b.mesh(bp, bn, bi)
+    box_uv = np.tile(np.array([(0.0, 0.0), (1.0, 0.0), (1.0, 1.0), (0.0, 1.0)], np.float32), (6, 1))
+    box_ref = b.mesh(bp, bn, bi, box_uv)
     for k in range(n_boxes):
         size = rng.uniform(0.02, 0.2, 3)
         center = rng.uniform(size / 2, 1.0 - size / 2)
         model = S @ np.array([[size[0], 0, 0, center[0]], [0, size[1], 0, center[1]],
                               [0, 0, size[2], center[2]], [0, 0, 0, 1.0]])
         em = tuple(rng.uniform(0.5, 1.0, 3)) if k < emissive_boxes else (0.0, 0.0, 0.0)
-        b.draw(box_ref, model, _material(rng.uniform(0.2, 0.8, 3), em), _surface(rng))
+        mat = _material(rng.uniform(0.2, 0.8, 3), em)
+        if textured:
+            mat = _with_textures(mat, *tex_of(k, k < emissive_boxes))
+        b.draw(box_ref, model, mat, _surface(rng))
     sp, sn, si = icosphere(3)
-    sphere_ref = b.mesh(sp, sn, si)
+    sphere_uv = np.stack([np.arctan2(sp[:, 2], sp[:, 0]) / (2 * np.pi) + 0.5,
+                          np.arccos(np.clip(sp[:, 1], -1.0, 1.0)) / np.pi], axis=1).astype(np.float32)
+    sphere_ref = b.mesh(sp, sn, si, sphere_uv)
     for _ in range(n_spheres):
         r = rng.uniform(0.03, 0.1)
         center = rng.uniform(r, 1.0 - r, 3)
         model = S @ np.array([[r, 0, 0, center[0]], [0, r, 0, center[1]], [0, 0, r, center[2]], [0, 0, 0, 1.0]])
-        b.draw(sphere_ref, model, _material(rng.uniform(0.2, 0.8, 3)), _surface(rng))
+        mat = _material(rng.uniform(0.2, 0.8, 3))
+        if textured:
+            mat = _with_textures(mat, *tex_of(len(b.prims), False))
+        b.draw(sphere_ref, model, mat, _surface(rng))
     room_tris = 12 + n_boxes * 12 + n_spheres * 1280
     n_thin = max(0, thin_total - room_tris)
     if n_thin:

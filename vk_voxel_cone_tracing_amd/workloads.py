@@ -240,8 +240,11 @@ def svo_build(seed: int = 1, level: int = 9, total_tris: int = 1_000_000, reps: 
     ms = time_ms(build, reps)
     n = C.c_size_t()
     A.call("vct_svo_read_nodes", ctx._h, s, None, 0, C.byref(n))
-    return dict(workload=f"SVO {1 << level}^3: room seed {seed} + thin triangles, {scene.triangle_count} tris",
-                ms_per_build=round(ms, 3), fragments=int(nfrag.value), nodes=int(n.value),
+    brick_ms = time_ms(lambda: A.call("vct_svo_build_bricks", ctx._h, s), reps)
+    return dict(workload=f"SVO {1 << level}^3: room seed {seed} + thin triangles, {scene.triangle_count} tris; "
+                         f"node pool and 3^3 RGBA8 bricks",
+                ms_per_build=round(ms, 3), ms_bricks=round(brick_ms, 3), ms_build_with_bricks=round(ms + brick_ms, 3),
+                fragments=int(nfrag.value), nodes=int(n.value), bricks=int(n.value),
                 nodes_per_s=round(n.value / (ms * 1e-3), 1))
 
 
