@@ -394,6 +394,23 @@ vct_status vct_trace(vct_ctx* ctx, vct_stream stream, const vct_gbuffer* gbuffer
                      float* diffuse_out, float* specular_out,
                      unsigned long long* samples_out);
 
+/* No counterpart in the reference. vct_trace for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the same arguments, kernels and results, with the
+ * specular work list in a static device array of the library instead of
+ * context scratch, and the radiance occupancy mask rebuilt on every call
+ * (vct_trace rebuilds it only when the host saw the radiance volume change,
+ * which a replayed graph would not). It makes no allocation; an image needing
+ * more than 2048 specular tiles of 64x64 pixels (32 MiB; 3840x2160 needs 2040) returns
+ * VCT_ERR_CAPACITY. Every GPU holds one such array for the whole process and
+ * all calls on that GPU share it: calls that overlap on one GPU race, so the
+ * caller serializes them (one stream, or events between streams). */
+vct_status vct_trace_noalloc(vct_ctx* ctx, vct_stream stream, const vct_gbuffer* gbuffer,
+                             const vct_camera_ubo* camera, const vct_cone_tracing_desc* desc,
+                             const vct_shadow_map* shadow, const vct_directional_light_desc* light,
+                             const vct_directional_light_shadow_desc* light_shadow, float* diffuse_out,
+                             float* specular_out, unsigned long long* samples_out);
+
 /* Volume access in the reference atlas layout: a 3-D image of
  * ((res+2)*6) x ((res+2)*levels) x (res+2) texels, x fastest
  * (Voxelizer.h:40-52). RGBA8 texels are 4 bytes, RGBA16F 8 bytes.

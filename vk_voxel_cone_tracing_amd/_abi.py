@@ -340,6 +340,11 @@ _PROTOS = {
         [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(ConeTracingDesc), C.POINTER(ShadowMap),
          C.POINTER(DirectionalLightDesc), C.POINTER(DirectionalLightShadowDesc), P, P, P],
     ),
+    "vct_trace_noalloc": (
+        C.c_int,
+        [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(ConeTracingDesc), C.POINTER(ShadowMap),
+         C.POINTER(DirectionalLightDesc), C.POINTER(DirectionalLightShadowDesc), P, P, P],
+    ),
     "vct_raycast_gbuffer": (
         C.c_int, [P, P, C.POINTER(Mesh), P, C.POINTER(CameraUBO), P, P, P, P, P, C.c_uint32, C.c_uint32]
     ),

@@ -387,6 +387,8 @@ vct_status vct_create(vct_ctx** out_ctx, const vct_config* cfg)
         if (e == hipSuccess) e = hipDeviceSynchronize();
         if (e != hipSuccess) s = fail(VCT_ERR_HIP, "initial clear failed: %s", hipGetErrorString(e));
     }
+    // device address of the allocation-free trace's static work list on this device
+    if (s == VCT_OK) s = trace_scratch_init(dev);
     if (s != VCT_OK) {
         vct_destroy(c);
         return s;

@@ -190,6 +190,10 @@ vct_status run_vertex_stage(vct_ctx* ctx, hipStream_t s, const vct_mesh* mesh, T
 // of a device material table (3 per material), read back to the host.
 vct_status read_texture_indices(hipStream_t s, const vct_gltf_material* mats, uint32_t count, int32_t* idx);
 
+// Looks up, once per device, the device address of the allocation-free
+// trace's static work list (vct_trace_noalloc); called by vct_create.
+vct_status trace_scratch_init(int device);
+
 // Pinned staging copy of a small host array to a device buffer (stream ordered).
 vct_status upload_small(vct_ctx* ctx, hipStream_t s, DeviceBuffer& dst, const void* src, size_t bytes);
 

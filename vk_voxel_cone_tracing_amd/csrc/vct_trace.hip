@@ -10,6 +10,8 @@
 // share the footprint offsets, so one address computation serves 24 loads.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "vct_internal.hpp"
 #include "vct_texel.hpp"
 
@@ -1136,6 +1138,75 @@ extern "C" vct_status vct_trace(vct_ctx* c, vct_stream stream, const vct_gbuffer
         VCT_TRY(build_occupancy(c, (hipStream_t)stream));
         c->mask_version = c->radiance_version;
     }
+    if (c->cfg.radiance_format == VCT_FORMAT_RGBA16F)
+        return launch_trace<true>((hipStream_t)stream, (const uint2*)c->radiance.ptr, P, g, (float4*)diffuse_out,
+                                  (float4*)specular_out, samples_out, perm);
+    return launch_trace<false>((hipStream_t)stream, (const uint32_t*)c->radiance.ptr, P, g, (float4*)diffuse_out,
+                               (float4*)specular_out, samples_out, perm);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free trace. The specular work list lives in `scratch`, a static
+// device array of kNoAllocTiles 64x64-pixel tiles (32 MiB: up to 2048 tiles,
+// e.g. 3840x2160); every GPU the library runs on holds its own copy until the
+// process ends. Its device address is looked up once per device (at context
+// creation) and kept in g_scratch_ptr; the array's name is never used as a
+// device pointer on the host. Calls that overlap on one GPU share the array
+// and race; the caller serializes them (one stream, or events).
+namespace vct {
+
+constexpr size_t kNoAllocTiles = 2048;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) uint32_t scratch[kNoAllocTiles * kSpecTilePixels];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+vct_status trace_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(VCT_ERR_UNSUPPORTED, "device ordinal %d", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return VCT_OK;
+    int cur = 0;
+    VCT_HIP(hipGetDevice(&cur));
+    if (cur != device) VCT_HIP(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !ptr)
+        return fail(VCT_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return VCT_OK;
+}
+
+}  // namespace vct
+
+extern "C" vct_status vct_trace_noalloc(vct_ctx* c, vct_stream stream, const vct_gbuffer* g,
+                                        const vct_camera_ubo* cam, const vct_cone_tracing_desc* d,
+                                        const vct_shadow_map* shadow, const vct_directional_light_desc* light,
+                                        const vct_directional_light_shadow_desc* light_shadow, float* diffuse_out,
+                                        float* specular_out, unsigned long long* samples_out)
+{
+    TraceParams P;
+    VCT_TRY(make_trace_params(c, g, cam, d, shadow, light, light_shadow, diffuse_out, specular_out,
+                              c ? c->levels : 0, &P));
+    if (d->volume_dimension != (float)c->res)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "volume_dimension must equal the resolution");
+    uint32_t* perm = nullptr;
+    if (traces_specular(P)) {
+        const size_t tiles = spec_tiles(g);
+        if (tiles > kNoAllocTiles)
+            return fail(VCT_ERR_CAPACITY,
+                        "a %ux%u image needs %zu specular tiles of 64x64 pixels; the allocation-free trace holds %zu",
+                        g->width, g->height, tiles, kNoAllocTiles);
+        VCT_TRY(trace_scratch_init(c->device));  // set at vct_create; a no-op here
+        perm = (uint32_t*)g_scratch_ptr[c->device];
+    }
+    // rebuilt on every call: the host-side version check vct_trace uses would
+    // not run again when a captured call is replayed
+    VCT_TRY(build_occupancy(c, (hipStream_t)stream));
+    c->mask_version = c->radiance_version;
     if (c->cfg.radiance_format == VCT_FORMAT_RGBA16F)
         return launch_trace<true>((hipStream_t)stream, (const uint2*)c->radiance.ptr, P, g, (float4*)diffuse_out,
                                   (float4*)specular_out, samples_out, perm);
