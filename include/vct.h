@@ -536,6 +536,38 @@ vct_status vct_dense_set_level0(vct_dense* grid, vct_stream stream, const float*
 vct_status vct_dense_trace_backward(vct_dense* grid, vct_stream stream, const vct_gbuffer* gbuffer,
                                     const vct_camera_ubo* camera, const vct_dense_trace_desc* desc,
                                     const float* grad_out, float* grad_pyramid, uint32_t flags, uint32_t* status);
+/* The adjoint tape (samples stored by the forward instead of recomputed by
+ * the backward): vct_dense_trace_tape gives vct_dense_trace's output and
+ * also writes every accumulated sample of every cone, with each pixel's step
+ * count per cone, to `tape` (16-byte aligned device memory of at least
+ * vct_dense_tape_bytes bytes: 16 bytes x cones x (march steps + 2) plus one
+ * byte per cone for each pixel of the image rounded up to 16x16 tiles, then
+ * the backward's work area, 72 bytes per 64 pixels and cone plus 16 per
+ * pixel). vct_dense_trace_backward_tape then runs the reverse sweep on the
+ * stored samples instead of marching again, with vct_dense_trace_backward's
+ * result and status bits; it overwrites the tape with the sample gradients,
+ * so one tape serves one backward. Backward calls of different views, each
+ * with its own tape, touch nothing shared but the grid's pyramid (read) and
+ * grad_pyramid (atomic adds): they may run concurrently on several streams. */
+vct_status vct_dense_tape_bytes(vct_dense* grid, const vct_gbuffer* gbuffer, const vct_camera_ubo* camera,
+                                const vct_dense_trace_desc* desc, size_t* bytes);
+vct_status vct_dense_trace_tape(vct_dense* grid, vct_stream stream, const vct_gbuffer* gbuffer,
+                                const vct_camera_ubo* camera, const vct_dense_trace_desc* desc, float* out,
+                                unsigned long long* samples_out, void* tape, size_t tape_bytes);
+vct_status vct_dense_trace_backward_tape(vct_dense* grid, vct_stream stream, const vct_gbuffer* gbuffer,
+                                         const vct_camera_ubo* camera, const vct_dense_trace_desc* desc,
+                                         const float* grad_out, void* tape, size_t tape_bytes, float* grad_pyramid,
+                                         uint32_t* status);
+/* vct_dense_trace_backward_tape over `views` views of one image size in one
+ * call (host arrays of views G-buffers and cameras, and of device pointers to
+ * each view's output gradient and tape, each tape of at least tape_bytes):
+ * the views share the launches (one per 16 views), so no launch's tail
+ * leaves the chip idle between views. */
+vct_status vct_dense_trace_backward_tapes(vct_dense* grid, vct_stream stream, uint32_t views,
+                                          const vct_gbuffer* gbuffers, const vct_camera_ubo* cameras,
+                                          const vct_dense_trace_desc* desc, const float* const* grad_outs,
+                                          void* const* tapes, size_t tape_bytes, float* grad_pyramid,
+                                          uint32_t* status);
 vct_status vct_dense_mipmap_backward(vct_dense* grid, vct_stream stream, float* grad_pyramid);
 vct_status vct_dense_level0_grad(vct_dense* grid, vct_stream stream, const float* grad_pyramid, float* grad_albedo,
                                  float* grad_opacity);

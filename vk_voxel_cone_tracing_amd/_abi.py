@@ -382,6 +382,19 @@ _PROTOS = {
     "vct_dense_trace_backward": (
         C.c_int, [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(DenseTraceDesc), P, P, C.c_uint32, P]
     ),
+    "vct_dense_tape_bytes": (
+        C.c_int, [P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(DenseTraceDesc), C.POINTER(C.c_size_t)]
+    ),
+    "vct_dense_trace_tape": (
+        C.c_int, [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(DenseTraceDesc), P, P, P, C.c_size_t]
+    ),
+    "vct_dense_trace_backward_tape": (
+        C.c_int, [P, P, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(DenseTraceDesc), P, P, C.c_size_t, P, P]
+    ),
+    "vct_dense_trace_backward_tapes": (
+        C.c_int, [P, P, C.c_uint32, C.POINTER(GBuffer), C.POINTER(CameraUBO), C.POINTER(DenseTraceDesc),
+                  C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_size_t, P, P]
+    ),
     "vct_dense_mipmap_backward": (C.c_int, [P, P, P]),
     "vct_dense_level0_grad": (C.c_int, [P, P, P, P, P]),
     "vct_batch_create": (C.c_int, [C.POINTER(P), C.POINTER(BatchScene), C.c_uint32, C.POINTER(DenseTraceDesc)]),

@@ -57,16 +57,22 @@ def cameras_tensor(cams) -> torch.Tensor:
 
 def render(albedo: torch.Tensor, opacity: torch.Tensor, diffuse: torch.Tensor, normal: torch.Tensor,
            depth: torch.Tensor, cameras: torch.Tensor, grid_min, extent: float, params: TraceParams | None = None,
-           recompute: bool = False) -> torch.Tensor:
+           recompute: bool = False, tape: bool = True) -> torch.Tensor:
     """[V, H, W, 4] = (indirect diffuse rgb, ambient occlusion) per view.
 
     albedo [R,R,R,3] and opacity [R,R,R] (float32, device) are level 0 of a
     grid whose minimum corner is grid_min and whose edge is extent; the
     G-buffers (diffuse int32 RGBA8 bits, normal int16 half bits, depth float)
-    are stacked over V views; cameras is cameras_tensor(...). With recompute
-    the adjoint fetches the march's samples again in its reverse sweep
-    instead of keeping them per lane."""
+    are stacked over V views; cameras is cameras_tensor(...).
+
+    Where the adjoint gets the march's samples: by default, when a gradient
+    can follow (grad mode, an input requiring grad), the forward stores them
+    (the tape: 16 bytes x cones x (march steps + 2) per pixel) and the
+    backward reads them, unless the tape would take more than half of the
+    free device memory; with tape=False the backward marches again and keeps
+    them per lane; with recompute it fetches them twice (march and reverse
+    sweep) and stores nothing."""
     p = params or TraceParams()
     grid = [float(grid_min[0]), float(grid_min[1]), float(grid_min[2]), float(extent)]
-    return _vct_torch.render(albedo, opacity, diffuse, normal, depth, cameras, grid, p.as_list(),
-                             1 if recompute else 0)
+    flags = (1 if recompute else 0) | (0 if tape else 2)
+    return _vct_torch.render(albedo, opacity, diffuse, normal, depth, cameras, grid, p.as_list(), flags)

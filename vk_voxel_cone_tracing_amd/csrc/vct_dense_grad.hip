@@ -20,8 +20,6 @@
 namespace vct {
 namespace dense {
 
-constexpr int kAdjMaxSteps = 40;
-
 __global__ void k_set_level0(const float* __restrict__ albedo, const float* __restrict__ opacity, int res,
                              float4* __restrict__ level0)
 {
@@ -80,7 +78,7 @@ __device__ __forceinline__ Corner corner_of(const Pyramid& P, int m, const float
 // fixed_scale); the per-workgroup accumulators of the coarse levels, shared
 // by many cones, stay in double.
 constexpr int kPoolUnits = 1024;  // a wave's LDS pool for its two live level windows, in 16 B units (16 KB)
-constexpr int kMaxCopies = 16;    // lane-striped copies of a small window
+constexpr int kMaxCopies = 4;     // lane-striped copies of a small window
 constexpr int kCoarseN = 4;       // levels with at most 4^3 interior cells stay in LDS per tile
 constexpr int kCoarseFloats = 24 * (64 + 8 + 1);
 
@@ -130,12 +128,19 @@ __device__ __forceinline__ Box box_of(const Pyramid& P, int m, const float blo[3
     return B;
 }
 
+// Words one copy of a window part of `vol` texels takes: 4 channel planes
+// and one word of padding, so that copy c of a texel lies c words further
+// along the banks and the lanes of one instruction, which add to the same
+// texel in different copies, do not meet in one bank.
+__device__ __forceinline__ int copy_stride(int vol) { return vol ? 4 * vol + 1 : 0; }
+
 // A wave's LDS window for one level: one box, or two when the wave's start
 // points split across the middle of their longest axis (a depth edge) and the
 // two boxes fit where the one does not. Layout in LDS: per part, `copies`
 // blocks of 4 channel planes of `vol` 32-bit fixed-point texels from the
-// part's base (in elements); lds is false when nothing fits in `cap` 16-byte
-// units; `used` is the units it takes; q[c] is channel c's fixed-point scale.
+// part's base (in elements), copy_stride(vol) elements per copy; lds is
+// false when nothing fits in `cap` 16-byte units; `used` is the units it
+// takes; q[c] is channel c's fixed-point scale.
 struct Window {
     Box part[2];
     int parts, axis, copies, used;
@@ -189,15 +194,16 @@ __device__ __forceinline__ Window window_of(const Pyramid& P, int m, const float
         }
     }
     const int total = W.part[0].vol + W.part[1].vol;
-    W.lds = total > 0 && total <= cap;
+    const int words = copy_stride(W.part[0].vol) + copy_stride(W.part[1].vol);  // one copy of both parts
+    W.lds = total > 0 && words <= 4 * cap;
     int copies = kMaxCopies;
-    while (copies > 1 && copies * total > cap) copies >>= 1;
+    while (copies > 1 && copies * words > 4 * cap) copies >>= 1;
     W.copies = copies;
-    W.used = W.lds ? copies * total : 0;
+    W.used = W.lds ? (copies * words + 3) / 4 : 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) W.q[c] = 0.0f;
     W.part[0].base = base;
-    W.part[1].base = base + 4 * copies * W.part[0].vol;
+    W.part[1].base = base + copies * copy_stride(W.part[0].vol);
     return W;
 }
 
@@ -292,7 +298,7 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, int* __res
     if (W.lds && rx >= 0 && ry >= 0 && rz >= 0 && rx + 1 < B.d[0] && ry + 1 < B.d[1] && rz + 1 < B.d[2]) {
         // the whole footprint lies in the window (whose cells are all interior)
         const int d0 = B.d[0], d01 = B.d[0] * B.d[1];
-        const int cp = 4 * (lane & (W.copies - 1)) * B.vol;
+        const int cp = (lane & (W.copies - 1)) * copy_stride(B.vol);
         const int bx = r & 1, by = (r >> 1) & 1, bz = r >> 2;
         const int t0 = B.base + cp + (rz + bz) * d01 + (ry + by) * d0 + rx + bx;
         const int sx = bx ? -1 : 1, sy = by ? -d0 : d0, sz = bz ? -d01 : d01;
@@ -309,7 +315,7 @@ __device__ __forceinline__ void splat_level(float* __restrict__ grad, int* __res
         return;
     }
     const int n = P.n[m];
-    const int cp = W.lds ? 4 * (lane & (W.copies - 1)) * B.vol : 0;
+    const int cp = W.lds ? (lane & (W.copies - 1)) * copy_stride(B.vol) : 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
@@ -356,7 +362,7 @@ __device__ __forceinline__ void flush_window(float* __restrict__ grad, int* __re
             int acc = 0;
             const int t = e >> 2, ch = e & 3;
             for (int cc = 0; cc < W.copies; ++cc) {
-                const int li = B.base + (4 * cc + ch) * B.vol + t;
+                const int li = B.base + cc * copy_stride(B.vol) + ch * B.vol + t;
                 acc += win[li];
                 win[li] = 0;
             }
@@ -455,7 +461,7 @@ struct SweepHeader {
     float blo[3], bhi[3];
     float gmax[4];
 };
-static_assert(sizeof(SweepHeader) == 72, "SweepHeader layout");
+static_assert(sizeof(SweepHeader) == kSweepHeaderBytes, "SweepHeader layout");
 
 // Sweep: one 16 x 16 pixel tile and a group of cones per workgroup (small
 // groups keep the grid many rounds deep, so waves with long marches do not
@@ -470,9 +476,40 @@ static_assert(sizeof(SweepHeader) == 72, "SweepHeader layout");
 //        ds.a = d op c (1 - s.a)^(c-1),
 //        gR.a <- gR.a (1 - op) - c gR.rgb . s.rgb,  gO <- gO (1 - op / q).
 // The sample gradients ds (zero where a lane's march has ended) go to one
-// record block per (wave, cone). Without kStore the samples are fetched
-// again in the reverse sweep instead of kept.
-template <bool kStore>
+// record block per (wave, cone). Where the samples come from:
+//   kSweepRecompute  the replay fetches them, the reverse sweep again;
+//   kSweepKeep       the replay fetches them and keeps them per lane;
+//   kSweepTape       the forward wrote them to the tape (vct_dense_trace_tape)
+//                    with each lane's step count: the replay only reads them,
+//                    and the record block is the (wave, cone)'s tape slot,
+//                    overwritten in place (rec is the tape's sample array).
+//                    Marches of at most kTapeRegs steps skip this kernel: the
+//                    fused splat runs their sweep (sweep_slot).
+enum { kSweepRecompute = 0, kSweepKeep = 1, kSweepTape = 2 };
+constexpr int kTapeRegs = 16;
+
+// One step of the reverse recurrence above: returns the sample gradient ds
+// and carries gR.a and gO back over the step. pc = (1 - s.a)^c from the
+// replay, rw and occ the accumulated alpha and occlusion before the step.
+__device__ __forceinline__ float4 reverse_step(f4& gR, float& gO, const f4& s, float c, float iq, float pc, float rw,
+                                               float occ)
+{
+    const float base_a = fmaxf(1.0f - s.w, 0.0f);
+    const float op = 1.0f - pc;
+    const float b = 1.0f - rw;
+    const float g_op = gR.w * b + gO * (1.0f - occ) * iq;
+    float4 out;
+    out.x = gR.x * b * c;
+    out.y = gR.y * b * c;
+    out.z = gR.z * b * c;
+    out.w = base_a > 0.0f ? g_op * c * (pc / base_a) : 0.0f;  // d/da of 1 - base_a^c
+    const float dot = gR.x * s.x + gR.y * s.y + gR.z * s.z;
+    gR.w = gR.w * (1.0f - op) - c * dot;
+    gO = gO * (1.0f - op * iq);
+    return out;
+}
+
+template <int kMode>
 __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict__ data, Pyramid P, March M,
                                                          const uint32_t* __restrict__ g_diffuse,
                                                          const uint16_t* __restrict__ g_normal,
@@ -481,8 +518,10 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
                                                          uint32_t rec_cap, float4* __restrict__ starts,
                                                          float4* __restrict__ rec,
                                                          SweepHeader* __restrict__ hdr, uint32_t* __restrict__ ctr,
-                                                         uint32_t* __restrict__ status)
+                                                         uint32_t* __restrict__ status,
+                                                         const uint8_t* __restrict__ tape_n, uint32_t tape_cap)
 {
+    constexpr bool kStore = kMode != kSweepRecompute;
     __shared__ float s_step[kAdjMaxSteps], s_seg[kAdjMaxSteps], s_lvl[kAdjMaxSteps];
     __shared__ float s_corr[kAdjMaxSteps], s_q[kAdjMaxSteps], s_iq[kAdjMaxSteps];
     __shared__ int s_nsteps;
@@ -536,7 +575,27 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
         float st_rw[kAdjMaxSteps], st_occ[kAdjMaxSteps], st_pc[kAdjMaxSteps];
         f4 smp[kStore ? kAdjMaxSteps : 1];
         int n = 0;
-        if (lane_on) {
+        const size_t slot = ((size_t)tile * 4u + (size_t)wave) * (size_t)M.ncones + (size_t)i;
+        if (kMode == kSweepTape) {
+            int nt = lane_on ? (int)tape_n[slot * 64u + (size_t)lane] : 0;
+            if (nt > nsteps) {  // kTapeOverflow, or a march longer than the step table
+                atomicOr(status, 1u);
+                nt = 0;
+            }
+            const float4* col = rec + slot * (size_t)tape_cap * 64u + (size_t)lane;
+            for (int k = 0; k < nt; ++k) {
+                const float4 t = col[(size_t)k * 64u];
+                smp[kStore ? k : 0] = {t.x, t.y, t.z, t.w};
+            }
+            f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
+            float occlusion = 0.0f;
+            for (int k = 0; k < nt; ++k) {
+                st_rw[k] = result.w;
+                st_occ[k] = occlusion;
+                st_pc[k] = replay_step(result, occlusion, smp[kStore ? k : 0], s_corr[k], s_q[k]);
+            }
+            n = nt;
+        } else if (lane_on) {
             f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
             float occlusion = 0.0f;
             for (int k = 0; k < nsteps && occlusion < 1.0f; k += 2) {
@@ -572,11 +631,15 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
         if (kc == 0) continue;
         const unsigned long long mask = __ballot(lane_on);
         uint32_t base = 0, h = 0;
-        if (lane == 0) base = atomicAdd(&ctr[0], (uint32_t)kc * 64u);
-        base = __shfl(base, 0, 64);
-        if (base + (uint32_t)kc * 64u > rec_cap) {  // never with the host's sizing; checked all the same
-            if (lane == 0) atomicOr(status, 2u);
-            continue;
+        if (kMode == kSweepTape) {
+            base = (uint32_t)(slot * (size_t)tape_cap * 64u);  // the host keeps the tape below 2^32 float4
+        } else {
+            if (lane == 0) base = atomicAdd(&ctr[0], (uint32_t)kc * 64u);
+            base = __shfl(base, 0, 64);
+            if (base + (uint32_t)kc * 64u > rec_cap) {  // never with the host's sizing; checked all the same
+                if (lane == 0) atomicOr(status, 2u);
+                continue;
+            }
         }
         if (lane == 0) {
             h = atomicAdd(&ctr[1], 1u);
@@ -602,29 +665,16 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
         for (int k = kc - 1; k >= 0; --k) {
             float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (k < n) {
-                const float stp = s_step[k], lvl = s_lvl[k];
                 f4 s;
                 if (kStore) {
                     s = smp[kStore ? k : 0];
                 } else {
                     float pos[3];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) pos[a] = cstart[a] + dir[a] * stp;
-                    s = level_sample<true>(data, P, pos, lvl, face, w);
+                    for (int a = 0; a < 3; ++a) pos[a] = cstart[a] + dir[a] * s_step[k];
+                    s = level_sample<true>(data, P, pos, s_lvl[k], face, w);
                 }
-                const float c = s_corr[k], iq = s_iq[k];
-                const float base_a = fmaxf(1.0f - s.w, 0.0f);
-                const float pc = st_pc[k];  // base_a^c, from the replay
-                const float op = 1.0f - pc;
-                const float b = 1.0f - st_rw[k];
-                const float g_op = gR.w * b + gO * (1.0f - st_occ[k]) * iq;
-                out.x = gR.x * b * c;
-                out.y = gR.y * b * c;
-                out.z = gR.z * b * c;
-                out.w = base_a > 0.0f ? g_op * c * (pc / base_a) : 0.0f;  // d/da of 1 - base_a^c
-                const float dot = gR.x * s.x + gR.y * s.y + gR.z * s.z;
-                gR.w = gR.w * (1.0f - op) - c * dot;
-                gO = gO * (1.0f - op * iq);
+                out = reverse_step(gR, gO, s, s_corr[k], s_iq[k], st_pc[k], st_rw[k], st_occ[k]);
             }
             amax[0] = fmaxf(amax[0], fabsf(out.x));
             amax[1] = fmaxf(amax[1], fabsf(out.y));
@@ -641,6 +691,86 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
     }
 }
 
+// The reverse sweep of one (wave, cone) tape slot inside the fused backward
+// (kSweepTapeRegs' arithmetic): replays the recurrence from the stored
+// samples, overwrites them with the sample gradients and fills the header
+// fields the splat reads. Returns false when the slot holds no step.
+__device__ __forceinline__ bool sweep_slot(const Pyramid& P, const March& M, float4* __restrict__ tape,
+                                           const uint8_t* __restrict__ counts, uint32_t cap, size_t slot, int cone,
+                                           int nsteps, const float* s_corr, const float* s_q, const float* s_iq,
+                                           bool on, const float start[3], const float normal[3],
+                                           const float albedo[3], const float4& go, int lane,
+                                           uint32_t* __restrict__ status, SweepHeader& H)
+{
+    const float* d = cone_dir(M, cone);
+    const float dir[3] = {d[0], d[1], d[2]};
+    const float ct = normal[0] * dir[0] + normal[1] * dir[1] + normal[2] * dir[2];
+    const bool lane_on = on && ct >= 0.0f;
+    int nt = lane_on ? (int)counts[slot * 64u + (size_t)lane] : 0;
+    if (nt > nsteps || nt > kTapeRegs) {  // kTapeOverflow, or a march longer than the step table
+        atomicOr(status, 1u);
+        nt = 0;
+    }
+    const int kc = (int)wave_max((float)nt);
+    if (kc == 0) return false;
+    float4* col = tape + slot * (size_t)cap * 64u + (size_t)lane;
+    float r_rw[kTapeRegs], r_occ[kTapeRegs], r_pc[kTapeRegs];
+    f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
+    float occlusion = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kTapeRegs; ++k) {
+        if (k >= kc) break;
+        const float4 t = col[(size_t)k * 64u];
+        r_rw[k] = result.w;
+        r_occ[k] = occlusion;
+        r_pc[k] = replay_step(result, occlusion, {t.x, t.y, t.z, t.w}, s_corr[k], s_q[k]);
+    }
+    const float inv = 1.0f / (float)M.ncones;
+    f4 gR = {go.x * inv * albedo[0] * M.diffuse_intensity * ct, go.y * inv * albedo[1] * M.diffuse_intensity * ct,
+             go.z * inv * albedo[2] * M.diffuse_intensity * ct, 0.0f};
+    float gO = -(go.w * inv * M.ao_factor * ct);
+    float amax[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = kTapeRegs - 1; k >= 0; --k) {
+        if (k >= kc) continue;
+        float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 t = col[(size_t)k * 64u];
+        if (k < nt) out = reverse_step(gR, gO, {t.x, t.y, t.z, t.w}, s_corr[k], s_iq[k], r_pc[k], r_rw[k], r_occ[k]);
+        amax[0] = fmaxf(amax[0], fabsf(out.x));
+        amax[1] = fmaxf(amax[1], fabsf(out.y));
+        amax[2] = fmaxf(amax[2], fabsf(out.z));
+        amax[3] = fmaxf(amax[3], fabsf(out.w));
+        col[(size_t)k * 64u] = out;
+    }
+    H.cone = (uint32_t)cone;
+    H.kc = (uint32_t)kc;
+    H.base = (uint32_t)(slot * (size_t)cap * 64u);  // the host keeps the tape below 2^32 float4
+    H.mask = __ballot(lane_on);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        H.blo[k] = wave_min(lane_on ? start[k] : INFINITY);
+        H.bhi[k] = wave_max(lane_on ? start[k] : -INFINITY);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) H.gmax[c] = wave_max(amax[c]);
+    return true;
+}
+
+// The views of one fused backward launch (same image size): each view's
+// camera, G-buffer, output gradient and tape (samples, then step counts).
+constexpr int kBwdViews = 16;
+struct BwdViews {
+    uint32_t n;      // views in this launch
+    uint32_t slots;  // tape slots per view
+    float vp_inv[kBwdViews][16];
+    const uint32_t* diffuse[kBwdViews];
+    const uint16_t* normal[kBwdViews];
+    const float* depth[kBwdViews];
+    const float4* grad_out[kBwdViews];
+    float4* tape[kBwdViews];
+    const uint8_t* counts[kBwdViews];
+};
+
 // Splat: resident workgroups whose waves pull sweep headers from a counter
 // until none is left (the LDS set-up and the coarse flush are paid once per
 // workgroup, not per header). Each step's sample
@@ -650,36 +780,85 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
 // texel, face and channel: into the workgroup's LDS accumulators for the
 // coarsest levels (written to global memory once, at the end), into global
 // memory with atomics for the others.
-__global__ __launch_bounds__(256) void k_dense_bwd_splat(Pyramid P, March M, const SweepHeader* __restrict__ hdr,
+// kFused (the backward from tapes, marches of at most kTapeRegs steps): the
+// waves pull (view, tape slot) items of up to kBwdViews views instead of
+// headers and run each slot's reverse sweep first (sweep_slot), so the
+// sweep's arithmetic interleaves with other waves' splats, no header, start
+// point or record leaves the kernel, and one launch serves all the views.
+template <bool kFused>
+__global__ __launch_bounds__(256, 2) void k_dense_bwd_splat(Pyramid P, March M, const SweepHeader* __restrict__ hdr,
                                                          const uint32_t* __restrict__ ctr,
-                                                         const float4* __restrict__ rec,
+                                                         float4* __restrict__ rec,
                                                          const float4* __restrict__ starts, int width, int height,
-                                                         float* __restrict__ grad, uint32_t* __restrict__ next)
+                                                         float* __restrict__ grad, uint32_t* __restrict__ next,
+                                                         const BwdViews VW, uint32_t cap,
+                                                         uint32_t* __restrict__ status)
 {
     __shared__ double s_win[4][2 * kPoolUnits];
     __shared__ double s_coarse[kCoarseFloats];
     __shared__ float s_step[kAdjMaxSteps], s_seg[kAdjMaxSteps], s_lvl[kAdjMaxSteps];
+    __shared__ float s_corr[kAdjMaxSteps], s_q[kAdjMaxSteps], s_iq[kAdjMaxSteps];
+    __shared__ int s_nsteps;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Coarse C = coarse_layout(P);
     int* win = (int*)s_win[wave];
     for (int i = lane; i < 4 * kPoolUnits; i += 64) win[i] = 0;
     for (int i = tid; i < kCoarseFloats; i += 256) s_coarse[i] = 0.0;
-    if (tid == 0) step_table(P, M, s_step, s_seg, s_lvl);
+    if (tid == 0) {
+        const int cnt = step_table(P, M, s_step, s_seg, s_lvl);
+        s_nsteps = cnt;
+        if (kFused && cnt > 0) step_coefs(P, M, cnt, s_step, s_seg, s_lvl, s_corr, s_q, s_iq);
+    }
     __syncthreads();
-    const uint32_t nh = ctr[1];
+    if (kFused && s_nsteps < 0) {
+        if (tid == 0) atomicOr(status, 1u);
+        return;
+    }
+    const uint32_t nh = kFused ? VW.n * VW.slots : ctr[1];
+    const unsigned tiles_x = ((unsigned)width + 15u) / 16u;
     for (;;) {
         uint32_t h = 0;
         if (lane == 0) h = atomicAdd(next, 1u);
         h = __shfl(h, 0, 64);
         if (h >= nh) break;
-        const SweepHeader H = hdr[h];
-        const int px = H.px0 + (lane & 7), py = H.py0 + (lane >> 3);
+        SweepHeader H;
         float start[3] = {0.0f, 0.0f, 0.0f};
-        if (px < width && py < height) {
-            const float4 st = starts[(size_t)py * (size_t)width + (size_t)px];
-            start[0] = st.x;
-            start[1] = st.y;
-            start[2] = st.z;
+        const float4* vrec = rec;  // the records: the item's view's tape when fused
+        if (kFused) {
+            // item h: view v, slot (wave of 8x8 pixels, cone), wave-major
+            const uint32_t v = h / VW.slots, slot = h - v * VW.slots;
+            vrec = VW.tape[v];
+            March Mv = M;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) Mv.vp_inv[k] = VW.vp_inv[v][k];
+            const uint32_t wave_id = slot / (uint32_t)M.ncones, tile = wave_id >> 2, wv = wave_id & 3u;
+            const int px = (int)(tile % tiles_x) * 16 + (int)(wv & 1u) * 8 + (lane & 7);
+            const int py = (int)(tile / tiles_x) * 16 + (int)(wv >> 1) * 8 + (lane >> 3);
+            bool on = px < width && py < height;
+            float normal[3] = {0.0f, 0.0f, 1.0f}, albedo[3] = {0.0f, 0.0f, 0.0f};
+            float4 go = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (on) {
+                const size_t idx = (size_t)py * (size_t)width + (size_t)px;
+                const float depth = VW.depth[v][idx];
+                on = depth != 1.0f;
+                if (on) {
+                    pixel_setup(P, Mv, px, py, width, height, depth, VW.normal[v] + 4 * idx, VW.diffuse[v][idx], start,
+                                normal, albedo);
+                    go = VW.grad_out[v][idx];
+                }
+            }
+            if (!sweep_slot(P, M, VW.tape[v], VW.counts[v], cap, slot, (int)(slot % (uint32_t)M.ncones), s_nsteps,
+                            s_corr, s_q, s_iq, on, start, normal, albedo, go, lane, status, H))
+                continue;
+        } else {
+            H = hdr[h];
+            const int px = H.px0 + (lane & 7), py = H.py0 + (lane >> 3);
+            if (px < width && py < height) {
+                const float4 st = starts[(size_t)py * (size_t)width + (size_t)px];
+                start[0] = st.x;
+                start[1] = st.y;
+                start[2] = st.z;
+            }
         }
         const float* d = cone_dir(M, (int)H.cone);
         const float dir[3] = {d[0], d[1], d[2]};
@@ -698,10 +877,10 @@ __global__ __launch_bounds__(256) void k_dense_bwd_splat(Pyramid P, March M, con
         Window w0, w1;
         const bool lane_in = ((H.mask >> lane) & 1ull) != 0ull;
         const int kc = (int)H.kc;
-        float4 gnext = rec[H.base + (uint32_t)(kc - 1) * 64u + (uint32_t)lane];
+        float4 gnext = vrec[H.base + (uint32_t)(kc - 1) * 64u + (uint32_t)lane];
         for (int k = kc - 1; k >= 0; --k) {
             const float4 g4 = gnext;
-            if (k > 0) gnext = rec[H.base + (uint32_t)(k - 1) * 64u + (uint32_t)lane];
+            if (k > 0) gnext = vrec[H.base + (uint32_t)(k - 1) * 64u + (uint32_t)lane];
             const f4 gs = {g4.x, g4.y, g4.z, g4.w};
             const bool has = g4.x != 0.0f || g4.y != 0.0f || g4.z != 0.0f || g4.w != 0.0f;
             const float stp = s_step[k], lvl = s_lvl[k];
@@ -852,20 +1031,6 @@ static vct_status need_f32(const vct_dense* g)
     return VCT_OK;
 }
 
-// Step count of the march for these parameters: the step sequence does not
-// depend on the data, only the exit on full occlusion does.
-static int march_steps(const dense::Pyramid& P, const dense::March& M)
-{
-    float step = 0.0f, diameter = P.vs;
-    int n = 0;
-    while (step < M.max_dist && n <= dense::kAdjMaxSteps + 1) {
-        ++n;
-        step += std::fmax(diameter, P.vs) * M.step_factor;
-        diameter = step * M.coef;
-    }
-    return n;
-}
-
 extern "C" {
 
 vct_status vct_dense_texels(vct_dense* g, size_t* texels)
@@ -898,7 +1063,7 @@ vct_status vct_dense_trace_backward(vct_dense* g, vct_stream stream, const vct_g
     dense::Pyramid P;
     dense::March M;
     VCT_TRY(dense::make_params(g, cam, d, &P, &M));
-    const int steps = march_steps(P, M);
+    const int steps = dense::march_step_count(P, M);
     if (steps + 1 > dense::kAdjMaxSteps)
         return fail(VCT_ERR_UNSUPPORTED, "the march takes more than %d steps", dense::kAdjMaxSteps - 1);
     int coarse = 0;
@@ -930,22 +1095,117 @@ vct_status vct_dense_trace_backward(vct_dense* g, vct_stream stream, const vct_g
         const uint32_t nt = std::min(chunk, tiles - t0);
         hipLaunchKernelGGL(dense::k_bwd_reset, dim3(1), dim3(64), 0, s, ctr);
         if (flags & VCT_ADJOINT_RECOMPUTE)
-            hipLaunchKernelGGL(dense::k_dense_bwd_sweep<false>, dim3(nt, cone_groups), dim3(256), 0, s, g->data.ptr, P, M,
-                               gb->diffuse, gb->normal, gb->depth, (int)gb->width, (int)gb->height,
+            hipLaunchKernelGGL(dense::k_dense_bwd_sweep<dense::kSweepRecompute>, dim3(nt, cone_groups), dim3(256), 0, s,
+                               g->data.ptr, P, M, gb->diffuse, gb->normal, gb->depth, (int)gb->width, (int)gb->height,
                                (const float4*)grad_out, t0, rec_cap, (float4*)g->bwd_start.ptr, (float4*)g->bwd_rec.ptr,
-                               (dense::SweepHeader*)g->bwd_hdr.ptr, ctr, status);
+                               (dense::SweepHeader*)g->bwd_hdr.ptr, ctr, status, nullptr, 0u);
         else
-            hipLaunchKernelGGL(dense::k_dense_bwd_sweep<true>, dim3(nt, cone_groups), dim3(256), 0, s, g->data.ptr, P, M,
-                               gb->diffuse, gb->normal, gb->depth, (int)gb->width, (int)gb->height,
+            hipLaunchKernelGGL(dense::k_dense_bwd_sweep<dense::kSweepKeep>, dim3(nt, cone_groups), dim3(256), 0, s,
+                               g->data.ptr, P, M, gb->diffuse, gb->normal, gb->depth, (int)gb->width, (int)gb->height,
                                (const float4*)grad_out, t0, rec_cap, (float4*)g->bwd_start.ptr, (float4*)g->bwd_rec.ptr,
-                               (dense::SweepHeader*)g->bwd_hdr.ptr, ctr, status);
-        hipLaunchKernelGGL(dense::k_dense_bwd_splat, dim3(splat_grid), dim3(256), 0, s, P, M,
-                           (const dense::SweepHeader*)g->bwd_hdr.ptr, (const uint32_t*)ctr,
-                           (const float4*)g->bwd_rec.ptr, (const float4*)g->bwd_start.ptr, (int)gb->width,
-                           (int)gb->height, grad_pyramid, ctr + 2);
+                               (dense::SweepHeader*)g->bwd_hdr.ptr, ctr, status, nullptr, 0u);
+        hipLaunchKernelGGL(dense::k_dense_bwd_splat<false>, dim3(splat_grid), dim3(256), 0, s, P, M,
+                           (const dense::SweepHeader*)g->bwd_hdr.ptr, (const uint32_t*)ctr, (float4*)g->bwd_rec.ptr,
+                           (const float4*)g->bwd_start.ptr, (int)gb->width, (int)gb->height, grad_pyramid, ctr + 2,
+                           dense::BwdViews{}, 0u, nullptr);
     }
     VCT_HIP(hipGetLastError());
     return VCT_OK;
+}
+
+vct_status vct_dense_trace_backward_tapes(vct_dense* g, vct_stream stream, uint32_t views, const vct_gbuffer* gbs,
+                                          const vct_camera_ubo* cams, const vct_dense_trace_desc* d,
+                                          const float* const* grad_outs, void* const* tapes, size_t tape_bytes,
+                                          float* grad_pyramid, uint32_t* status)
+{
+    VCT_TRY(need_f32(g));
+    if (!gbs || !cams || !d || !grad_outs || !tapes || !grad_pyramid || !status)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    if (views == 0) return VCT_OK;
+    const uint32_t W = gbs[0].width, H = gbs[0].height;
+    for (uint32_t v = 0; v < views; ++v) {
+        const vct_gbuffer& gb = gbs[v];
+        if (!gb.diffuse || !gb.normal || !gb.depth || !gb.width || !gb.height)
+            return fail(VCT_ERR_INVALID_ARGUMENT, "view %u: G-buffer needs diffuse, normal and depth", v);
+        if (gb.width != W || gb.height != H) return fail(VCT_ERR_INVALID_ARGUMENT, "views differ in image size");
+        if (!grad_outs[v] || !tapes[v]) return fail(VCT_ERR_INVALID_ARGUMENT, "view %u: null gradient or tape", v);
+        if ((uintptr_t)tapes[v] % 16u) return fail(VCT_ERR_INVALID_ARGUMENT, "the tapes must be 16-byte aligned");
+    }
+    dense::Pyramid P;
+    dense::March M;
+    VCT_TRY(dense::make_params(g, &cams[0], d, &P, &M));
+    const dense::TapeLayout T = dense::tape_layout(P, M, W, H, sizeof(dense::SweepHeader));
+    if (T.cap > (uint32_t)dense::kAdjMaxSteps)
+        return fail(VCT_ERR_UNSUPPORTED, "the march takes more than %d steps", dense::kAdjMaxSteps - 2);
+    if (tape_bytes < T.bytes)
+        return fail(VCT_ERR_INVALID_ARGUMENT, "the tape holds %zu bytes; a view's takes %zu", tape_bytes, T.bytes);
+    if ((T.sample_bytes / sizeof(float4)) >> 32) return fail(VCT_ERR_UNSUPPORTED, "tape above 2^32 samples");
+    int coarse = 0;
+    for (int m = 1; m < P.nlevels; ++m)
+        if (P.n[m] <= dense::kCoarseN) coarse += 24 * P.n[m] * P.n[m] * P.n[m];
+    if (coarse > dense::kCoarseFloats) return fail(VCT_ERR_UNSUPPORTED, "coarse levels exceed the tile accumulator");
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0, cus = 0;
+    VCT_HIP(hipGetDevice(&dev));
+    VCT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t splat_grid = (uint32_t)std::max(1, 2 * cus);
+    if (T.cap <= (uint32_t)dense::kTapeRegs) {
+        // one kernel per kBwdViews views: each slot's reverse sweep, then its splat
+        for (uint32_t v0 = 0; v0 < views; v0 += dense::kBwdViews) {
+            dense::BwdViews VW{};
+            VW.n = std::min<uint32_t>(views - v0, dense::kBwdViews);
+            VW.slots = (uint32_t)T.slots;
+            if ((uint64_t)VW.n * VW.slots >> 32) return fail(VCT_ERR_UNSUPPORTED, "too many tape slots per launch");
+            for (uint32_t k = 0; k < VW.n; ++k) {
+                const uint32_t v = v0 + k;
+                for (int e = 0; e < 16; ++e) VW.vp_inv[k][e] = cams[v].view_proj_inv[e];
+                VW.diffuse[k] = gbs[v].diffuse;
+                VW.normal[k] = gbs[v].normal;
+                VW.depth[k] = gbs[v].depth;
+                VW.grad_out[k] = (const float4*)grad_outs[v];
+                VW.tape[k] = (float4*)tapes[v];
+                VW.counts[k] = (const uint8_t*)tapes[v] + T.sample_bytes;
+            }
+            auto* ctr = (uint32_t*)((uint8_t*)tapes[v0] + T.ctr_off);
+            hipLaunchKernelGGL(dense::k_bwd_reset, dim3(1), dim3(64), 0, s, ctr);
+            hipLaunchKernelGGL(dense::k_dense_bwd_splat<true>, dim3(splat_grid), dim3(256), 0, s, P, M, nullptr, nullptr,
+                               nullptr, nullptr, (int)W, (int)H, grad_pyramid, ctr + 2, VW, T.cap, status);
+        }
+        VCT_HIP(hipGetLastError());
+        return VCT_OK;
+    }
+    // longer marches: per view, the sweep kernel writes headers into the
+    // tape's work area, then the splat
+    const uint32_t tiles = ((W + 15) / 16) * ((H + 15) / 16);
+    const uint32_t cone_groups = (uint32_t)((M.ncones + 1) / 2);
+    for (uint32_t v = 0; v < views; ++v) {
+        dense::March Mv = M;
+        for (int e = 0; e < 16; ++e) Mv.vp_inv[e] = cams[v].view_proj_inv[e];
+        auto* base = (uint8_t*)tapes[v];
+        auto* hdr = (dense::SweepHeader*)(base + T.header_off);
+        auto* starts = (float4*)(base + T.start_off);
+        auto* ctr = (uint32_t*)(base + T.ctr_off);
+        hipLaunchKernelGGL(dense::k_bwd_reset, dim3(1), dim3(64), 0, s, ctr);
+        hipLaunchKernelGGL(dense::k_dense_bwd_sweep<dense::kSweepTape>, dim3(tiles, cone_groups), dim3(256), 0, s,
+                           g->data.ptr, P, Mv, gbs[v].diffuse, gbs[v].normal, gbs[v].depth, (int)W, (int)H,
+                           (const float4*)grad_outs[v], 0u, 0u, starts, (float4*)base, hdr, ctr, status,
+                           (const uint8_t*)base + T.sample_bytes, T.cap);
+        hipLaunchKernelGGL(dense::k_dense_bwd_splat<false>, dim3(splat_grid), dim3(256), 0, s, P, Mv,
+                           (const dense::SweepHeader*)hdr, (const uint32_t*)ctr, (float4*)base, (const float4*)starts,
+                           (int)W, (int)H, grad_pyramid, ctr + 2, dense::BwdViews{}, 0u, nullptr);
+    }
+    VCT_HIP(hipGetLastError());
+    return VCT_OK;
+}
+
+vct_status vct_dense_trace_backward_tape(vct_dense* g, vct_stream stream, const vct_gbuffer* gb,
+                                         const vct_camera_ubo* cam, const vct_dense_trace_desc* d,
+                                         const float* grad_out, void* tape, size_t tape_bytes, float* grad_pyramid,
+                                         uint32_t* status)
+{
+    if (!gb || !cam || !grad_out || !tape) return fail(VCT_ERR_INVALID_ARGUMENT, "null argument");
+    return vct_dense_trace_backward_tapes(g, stream, 1, gb, cam, d, &grad_out, &tape, tape_bytes, grad_pyramid,
+                                          status);
 }
 
 vct_status vct_dense_mipmap_backward(vct_dense* g, vct_stream stream, float* grad_pyramid)

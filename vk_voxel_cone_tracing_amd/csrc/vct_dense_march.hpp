@@ -207,15 +207,31 @@ __device__ __forceinline__ void accumulate(f4& result, float& occlusion, const f
     occlusion += (1.0f - occlusion) * op / (1.0f + (step + vsl) * decay);
 }
 
+// A cone's slot of the adjoint tape: the sample of accumulated step k goes
+// to at[k * 64] (the lane's column of the wave's records), for k < cap;
+// `count` ends as the number of accumulated steps.
+struct TapeSlot {
+    float4* at;
+    uint32_t cap;
+    uint32_t count;
+};
+
+__device__ __forceinline__ void tape_put(TapeSlot* t, const f4& s)
+{
+    if (t->count < t->cap) t->at[(size_t)t->count * 64u] = make_float4(s.x, s.y, s.z, s.w);
+    ++t->count;
+}
+
 // traceCone (voxelConeTracing.frag:341-392) with the dense level rule.
 // The step sequence t_k does not depend on the data (only the loop exit on
 // full occlusion does), so with kAhead the samples of steps k and k+1 are
 // fetched together and step k+1 is accumulated only if the loop would have
 // reached it: the same arithmetic in the same order, with twice the loads in
-// flight per lane.
+// flight per lane. With a tape, every accumulated sample is also written to
+// the cone's tape slot.
 template <bool kF32, bool kAhead = false>
 __device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March& M, const float start_in[3],
-                    const float dir[3], uint32_t& steps)
+                    const float dir[3], uint32_t& steps, TapeSlot* tape = nullptr)
 {
     const float vs = P.vs;
     float start[3];
@@ -235,6 +251,7 @@ __device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March
         if (!kAhead) {
             const f4 s = level_sample<kF32>(data, P, pos, lvl, face, w);
             accumulate(result, occlusion, s, lvl, step, seg, vs, M.decay);
+            if (tape) tape_put(tape, s);
             const float prev = step;
             step += fmaxf(diameter, vs) * M.step_factor;
             seg = step - prev;
@@ -251,12 +268,14 @@ __device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March
             const f4 sa = level_sample<kF32>(data, P, pos, lvl, face, w);
             const f4 sb = level_sample<kF32>(data, P, pos_b, lvl_b, face, w);
             accumulate(result, occlusion, sa, lvl, step, seg, vs, M.decay);
+            if (tape) tape_put(tape, sa);
             ++steps;
             step = step_b;
             seg = seg_b;
             diameter = diam_b;
             if (!(step < M.max_dist && occlusion < 1.0f)) break;
             accumulate(result, occlusion, sb, lvl_b, step, seg, vs, M.decay);
+            if (tape) tape_put(tape, sb);
             ++steps;
             const float prev = step;
             step += fmaxf(diameter, vs) * M.step_factor;
@@ -291,13 +310,27 @@ __device__ __forceinline__ void pixel_setup(const Pyramid& P, const March& M, in
     albedo[2] = u8_to_f((diffuse >> 16) & 0xffu);
 }
 
+// A lane's part of the adjoint tape: cone i's samples from col + i * stride
+// (float4 units, one column of 64), its accumulated step count at
+// counts[i * 64] (kTapeOverflow when the march took more than cap steps).
+struct LaneTape {
+    float4* col;
+    size_t stride;
+    uint8_t* counts;
+    uint32_t cap;
+};
+
+constexpr uint8_t kTapeOverflow = 255;
+
 // One pixel of the dense trace: 16 (or 32) cones weighted by cos, output
-// (indirect diffuse x albedo x intensity, ambient occlusion).
+// (indirect diffuse x albedo x intensity, ambient occlusion). With a tape,
+// the counts of the cones it marches are written (the caller zeroes them).
 template <bool kF32, bool kAhead>
 __device__ __forceinline__ float4 dense_pixel(const void* __restrict__ data, const Pyramid& P, const March& M, int px,
                                               int py, int width, int height, const uint32_t* __restrict__ g_diffuse,
                                               const uint16_t* __restrict__ g_normal,
-                                              const float* __restrict__ g_depth, uint32_t& steps)
+                                              const float* __restrict__ g_depth, uint32_t& steps,
+                                              const LaneTape* lt = nullptr)
 {
     const size_t idx = (size_t)py * (size_t)width + (size_t)px;
     const float depth = g_depth[idx];
@@ -309,7 +342,14 @@ __device__ __forceinline__ float4 dense_pixel(const void* __restrict__ data, con
         const float* d = cone_dir(M, i);
         const float ct = normal[0] * d[0] + normal[1] * d[1] + normal[2] * d[2];
         if (ct < 0.0f) continue;
-        const f4 c = march<kF32, kAhead>(data, P, M, start, d, steps);
+        f4 c;
+        if (lt) {
+            TapeSlot t{lt->col + (size_t)i * lt->stride, lt->cap, 0u};
+            c = march<kF32, kAhead>(data, P, M, start, d, steps, &t);
+            lt->counts[i * 64] = t.count > lt->cap ? kTapeOverflow : (uint8_t)t.count;
+        } else {
+            c = march<kF32, kAhead>(data, P, M, start, d, steps);
+        }
         ind.x += c.x * ct;
         ind.y += c.y * ct;
         ind.z += c.z * ct;
@@ -318,6 +358,59 @@ __device__ __forceinline__ float4 dense_pixel(const void* __restrict__ data, con
     const float inv = 1.0f / (float)M.ncones;
     return make_float4(ind.x * inv * albedo[0] * M.diffuse_intensity, ind.y * inv * albedo[1] * M.diffuse_intensity,
                        ind.z * inv * albedo[2] * M.diffuse_intensity, ind.w * inv * M.ao_factor);
+}
+
+// Longest march the adjoint keeps per-step state for (per-lane arrays).
+constexpr int kAdjMaxSteps = 40;
+
+// Bytes of the adjoint sweep's per-(wave, cone) header.
+constexpr size_t kSweepHeaderBytes = 72;
+
+// Step count of the march for these parameters, on the host: the step
+// sequence does not depend on the data, only the exit on full occlusion does.
+inline int march_step_count(const Pyramid& P, const March& M)
+{
+    float step = 0.0f, diameter = P.vs;
+    int n = 0;
+    while (step < M.max_dist && n <= kAdjMaxSteps + 1) {
+        ++n;
+        step += std::fmax(diameter, P.vs) * M.step_factor;
+        diameter = step * M.coef;
+    }
+    return n;
+}
+
+// The adjoint tape of one view (vct_dense_trace_tape): per slot (a wave of
+// 8x8 pixels of a 16x16 tile, wave-major within the tile, and a cone), `cap`
+// steps x 64 lanes of float4 samples, slot after slot; then one step-count
+// byte per (slot, lane). cap is the host's step count plus two steps of
+// margin (device rounding may differ in the last bit). Behind them, the
+// backward's own work area for the view: the sweep's headers (one per slot
+// at most), the pixels' cone start points and the sweep / splat counters, so
+// that backward calls of different views share nothing but the grid.
+struct TapeLayout {
+    size_t slots;
+    uint32_t cap;
+    size_t sample_bytes, count_bytes;
+    size_t header_off, start_off, ctr_off;
+    size_t bytes;
+};
+
+inline TapeLayout tape_layout(const Pyramid& P, const March& M, uint32_t width, uint32_t height,
+                              size_t header_bytes)
+{
+    auto up = [](size_t v) { return (v + 255u) / 256u * 256u; };
+    TapeLayout T;
+    const size_t tiles = (size_t)((width + 15u) / 16u) * (size_t)((height + 15u) / 16u);
+    T.slots = tiles * 4u * (size_t)M.ncones;
+    T.cap = (uint32_t)march_step_count(P, M) + 2u;
+    T.sample_bytes = T.slots * T.cap * 64u * sizeof(float4);
+    T.count_bytes = T.slots * 64u;
+    T.header_off = up(T.sample_bytes + T.count_bytes);
+    T.start_off = up(T.header_off + T.slots * header_bytes);
+    T.ctr_off = up(T.start_off + (size_t)width * height * sizeof(float4));
+    T.bytes = T.ctr_off + 16u;
+    return T;
 }
 
 inline vct_status make_params(const vct_dense* g, const vct_camera_ubo* cam, const vct_dense_trace_desc* d,

@@ -12,8 +12,9 @@
 // The backward reuses that pyramid when the grid still holds it (an id per
 // build, recorded by the forward; else it builds it again from the saved
 // inputs; the grid object is shared between calls),
-// zeroes a gradient pyramid, runs vct_dense_trace_backward per view,
-// vct_dense_mipmap_backward, and returns d loss / d albedo and d loss /
+// zeroes a gradient pyramid, runs the trace adjoint (from the forward's tapes
+// in one vct_dense_trace_backward_tapes call, or vct_dense_trace_backward
+// per view), vct_dense_mipmap_backward, and returns d loss / d albedo and d loss /
 // d opacity. All work is enqueued on the current HIP stream of PyTorch, and
 // the backward does not wait for it: a trace status set on the device turns
 // the gradients' first element into NaN there, and the next backward (or
@@ -210,6 +211,13 @@ std::array<float, 4> box_of(const std::vector<double>& grid)
     return {(float)grid[0], (float)grid[1], (float)grid[2], (float)grid[3]};
 }
 
+// Flag bits of render() beside VCT_ADJOINT_RECOMPUTE: kNoTape (the caller's)
+// makes the backward march again with the samples kept per lane instead of
+// reading the forward's tape; kTapeWanted (set by render) says a backward
+// can follow, so the forward may keep the tape.
+constexpr int64_t kNoTape = int64_t(1) << 1;
+constexpr int64_t kTapeWanted = int64_t(1) << 32;
+
 class DenseRender : public torch::autograd::Function<DenseRender> {
 public:
     static torch::Tensor forward(torch::autograd::AutogradContext* ctx, torch::Tensor albedo, torch::Tensor opacity,
@@ -226,11 +234,38 @@ public:
         GridSlot& slot = build_pyramid(albedo, opacity, box, s, 0);
         const int64_t V = depth.size(0), H = depth.size(1), W = depth.size(2);
         auto out = torch::empty({V, H, W, 4}, albedo.options());
+        // When a gradient will be asked for, the forward keeps the march's
+        // samples (the adjoint tape) unless told to recompute them or they
+        // would take more than half of the free device memory.
+        const bool grad = (flags & kTapeWanted) != 0;
+        torch::Tensor tape;
+        int64_t per_view = 0;
+        if (grad && !(flags & (VCT_ADJOINT_RECOMPUTE | kNoTape)) && V > 0) {
+            const vct_gbuffer gb = gbuffer(diffuse, normal, depth, 0);
+            const vct_camera_ubo cam = camera(cameras, 0);
+            size_t bytes = 0;
+            size_t free_b = 0, total_b = 0;
+            if (vct_dense_tape_bytes(slot.grid, &gb, &cam, &td, &bytes) == VCT_OK &&
+                hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+                per_view = (int64_t)((bytes + 255) / 256 * 256);
+                if ((size_t)(per_view * V) <= free_b / 2)
+                    tape = torch::empty({per_view * V}, albedo.options().dtype(torch::kUInt8));
+            }
+        }
         for (int64_t v = 0; v < V; ++v) {
             const vct_gbuffer gb = gbuffer(diffuse, normal, depth, v);
             const vct_camera_ubo cam = camera(cameras, v);
-            check(vct_dense_trace(slot.grid, s, &gb, &cam, &td, out.data_ptr<float>() + v * H * W * 4, nullptr),
-                  "vct_dense_trace");
+            float* o = out.data_ptr<float>() + v * H * W * 4;
+            if (tape.defined())
+                check(vct_dense_trace_tape(slot.grid, s, &gb, &cam, &td, o, nullptr,
+                                           tape.data_ptr<uint8_t>() + v * per_view, (size_t)per_view),
+                      "vct_dense_trace_tape");
+            else
+                check(vct_dense_trace(slot.grid, s, &gb, &cam, &td, o, nullptr), "vct_dense_trace");
+        }
+        if (tape.defined()) {
+            ctx->saved_data["tape"] = tape;
+            ctx->saved_data["tape_view_bytes"] = per_view;
         }
         ctx->save_for_backward({albedo, opacity, diffuse, normal, depth, cameras});
         ctx->saved_data["grid"] = grid;
@@ -248,7 +283,7 @@ public:
                             &depth = saved[4], &cameras = saved[5];
         const auto box = box_of(ctx->saved_data["grid"].toDoubleVector());
         const vct_dense_trace_desc td = trace_desc(ctx->saved_data["trace"].toDoubleVector());
-        const uint32_t flags = (uint32_t)ctx->saved_data["flags"].toInt();
+        const uint32_t flags = (uint32_t)(ctx->saved_data["flags"].toInt() & VCT_ADJOINT_RECOMPUTE);
         hipStream_t s = c10::hip::getCurrentHIPStream().stream();
         check_previous_status();
         GridSlot& slot = build_pyramid(albedo, opacity, box, s, ctx->saved_data["build"].toInt());
@@ -256,12 +291,37 @@ public:
         auto gpyr = torch::zeros({(int64_t)slot.texels * 4}, albedo.options());
         auto status = torch::zeros({1}, albedo.options().dtype(torch::kInt32));
         const int64_t V = depth.size(0), H = depth.size(1), W = depth.size(2);
-        for (int64_t v = 0; v < V; ++v) {
-            const vct_gbuffer gb = gbuffer(diffuse, normal, depth, v);
-            const vct_camera_ubo cam = camera(cameras, v);
-            check(vct_dense_trace_backward(slot.grid, s, &gb, &cam, &td, gout.data_ptr<float>() + v * H * W * 4,
-                                           gpyr.data_ptr<float>(), flags, (uint32_t*)status.data_ptr<int32_t>()),
-                  "vct_dense_trace_backward");
+        // The tape serves one backward (it is overwritten with the sample
+        // gradients); a second backward of a retained graph marches again.
+        torch::Tensor tape;
+        if (ctx->saved_data.count("tape")) {
+            tape = ctx->saved_data["tape"].toTensor();
+            ctx->saved_data.erase("tape");
+        }
+        const int64_t per_view = tape.defined() ? ctx->saved_data["tape_view_bytes"].toInt() : 0;
+        auto* st = (uint32_t*)status.data_ptr<int32_t>();
+        if (tape.defined()) {  // all views in one call: they share the launches
+            std::vector<vct_gbuffer> gbs;
+            std::vector<vct_camera_ubo> cams;
+            std::vector<const float*> gouts;
+            std::vector<void*> tapes;
+            for (int64_t v = 0; v < V; ++v) {
+                gbs.push_back(gbuffer(diffuse, normal, depth, v));
+                cams.push_back(camera(cameras, v));
+                gouts.push_back(gout.data_ptr<float>() + v * H * W * 4);
+                tapes.push_back(tape.data_ptr<uint8_t>() + v * per_view);
+            }
+            check(vct_dense_trace_backward_tapes(slot.grid, s, (uint32_t)V, gbs.data(), cams.data(), &td, gouts.data(),
+                                                 tapes.data(), (size_t)per_view, gpyr.data_ptr<float>(), st),
+                  "vct_dense_trace_backward_tapes");
+        } else {
+            for (int64_t v = 0; v < V; ++v) {
+                const vct_gbuffer gb = gbuffer(diffuse, normal, depth, v);
+                const vct_camera_ubo cam = camera(cameras, v);
+                check(vct_dense_trace_backward(slot.grid, s, &gb, &cam, &td, gout.data_ptr<float>() + v * H * W * 4,
+                                               gpyr.data_ptr<float>(), flags, st),
+                      "vct_dense_trace_backward");
+            }
         }
         check(vct_dense_mipmap_backward(slot.grid, s, gpyr.data_ptr<float>()), "vct_dense_mipmap_backward");
         auto ga = torch::empty_like(albedo);
@@ -278,6 +338,9 @@ torch::Tensor render(torch::Tensor albedo, torch::Tensor opacity, torch::Tensor 
                      torch::Tensor depth, torch::Tensor cameras, std::vector<double> grid, std::vector<double> trace,
                      int64_t flags)
 {
+    // what apply() will find: a backward can follow only in grad mode with an
+    // input that requires a gradient
+    if (torch::GradMode::is_enabled() && (albedo.requires_grad() || opacity.requires_grad())) flags |= kTapeWanted;
     return DenseRender::apply(albedo, opacity, diffuse, normal, depth, cameras, grid, trace, flags);
 }
 
