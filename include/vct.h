@@ -576,14 +576,16 @@ vct_status vct_dense_level0_grad(vct_dense* grid, vct_stream stream, const float
  * No reference counterpart (the reference renders one scene): many scenes of
  * uneven size, each voxelized into its own dense grid (vct_dense_voxelize's
  * rule), mipped (vct_dense_mipmap) and traced (vct_dense_trace), captured
- * once in a hipGraph and replayed by vct_batch_run. Two schedules over one
- * task table (vertex chunks, raster chunks, mip bricks, pixel tiles), chosen
- * by vct_batch_tuning.workgroups_per_cu: 0 (the default) launches each phase
- * over every scene as one grid; 1, 2 or 4 run ONE persistent launch whose
- * workgroups pull tasks from a queue ordered by phase, a task waiting on its
- * scene's completion counter of the previous phase, so small scenes trace
- * while large ones still voxelize. Results are bit-identical to the
- * per-scene vct_dense_* calls under every schedule. */
+ * once in a hipGraph and replayed by vct_batch_run. One task table (vertex
+ * chunks, raster chunks, mip bricks, pixel tiles); both schedules run the
+ * vertex chunks, then the raster of every scene's triangles as one
+ * sample-parallel pass; then, chosen by vct_batch_tuning.workgroups_per_cu,
+ * 0 (the default) launches each mip level and the pixel tiles over every
+ * scene as one grid each, and 1, 2 or 4 run ONE persistent launch whose
+ * workgroups pull the mip bricks and pixel tiles from a queue ordered by
+ * phase, a task waiting on its scene's completion counter of the previous
+ * mip level, so small scenes trace while large ones still mip. Results are
+ * bit-identical to the per-scene vct_dense_* calls under every schedule. */
 typedef struct vct_batch vct_batch;
 
 typedef struct vct_batch_scene {
@@ -596,15 +598,18 @@ typedef struct vct_batch_scene {
 
 /* Schedule knobs picked by vct_batch_autotune. */
 typedef struct vct_batch_tuning {
-    uint32_t raster_chunk;      /* triangles per raster task: 64, 128 or 256 */
+    uint32_t raster_chunk;      /* triangles per raster task of the task table: 64, 128 or
+                                   256 (vct_batch_plan); the launches raster sample-parallel
+                                   whatever it is */
     uint32_t mip_brick;         /* destination brick edge of a mip task: 4 or 8 */
     uint32_t trace_schedule;    /* 0: one cone step per loop trip; 1: two-step lookahead
                                    (the samples of steps k and k+1 fetched together,
                                    step k+1 accumulated only if the march reaches it) */
     uint32_t workgroups_per_cu; /* 0: one flat launch per phase (vertex, raster, each mip
                                    level, trace), kernel boundaries as dependencies;
-                                   1, 2 or 4: one persistent launch of CUs x this
-                                   workgroups pulling the tasks from a queue */
+                                   1, 2 or 4: vertex and raster launches, then one
+                                   persistent launch of CUs x this workgroups pulling the
+                                   mip bricks and pixel tiles from a queue */
 } vct_batch_tuning;
 
 vct_status vct_batch_create(vct_batch** out, const vct_batch_scene* scenes, uint32_t count,

@@ -97,6 +97,7 @@ struct SceneDev {
 struct Args {
     const Task* tasks;
     uint32_t ntasks;
+    uint32_t first;  // the persistent launch's first task (the first mip brick)
     const SceneDev* scenes;
     uint32_t* head;
     uint32_t* done;  // [scene * kPhases + phase]
@@ -159,68 +160,14 @@ __device__ void vertex_task(const SceneDev& S, uint32_t a, uint32_t b)
     S.soup_mat[t] = pr.material_index;
 }
 
-struct RasterLds {
-    TriSetup setup[256];
-    float pos[256 * 9];
-    uint32_t off[257];
-};
-
-// The dense raster (k_dense_raster) for triangles [a, b) of one scene: the
-// chunk's samples are enumerated through an LDS prefix sum so every lane of
-// the workgroup has work however uneven the triangles are.
-__device__ void raster_task(const SceneDev& S, uint32_t a, uint32_t b, RasterLds& L)
-{
-#pragma clang fp contract(off)
-    const int tid = threadIdx.x;
-    const uint32_t n = b - a;
-    uint32_t cnt = 0;
-    if ((uint32_t)tid < n) {
-        float p[9];
-        for (int k = 0; k < 9; ++k) p[k] = S.soup_pos[9u * (a + tid) + k] - S.grid_min[k % 3];
-        for (int k = 0; k < 9; ++k) L.pos[9 * tid + k] = p[k];
-        const int lo[3] = {0, 0, 0}, hi[3] = {S.res - 1, S.res - 1, S.res - 1};
-        TriSetup s;
-        if (tri_setup(p, S.vs, lo, hi, s)) {
-            cnt = (uint32_t)s.nu * (uint32_t)s.nv * (uint32_t)S.samples;
-            L.setup[tid] = s;
-        }
-    }
-    // inclusive scan of the counts (Hillis-Steele over 256 entries)
-    L.off[tid + 1] = cnt;
-    if (tid == 0) L.off[0] = 0;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        const uint32_t v = tid >= d ? L.off[tid + 1 - d] : 0u;
-        __syncthreads();
-        L.off[tid + 1] += v;
-        __syncthreads();
-    }
-    const uint32_t total = L.off[n];
-    for (uint32_t w = tid; w < total; w += 256) {
-        uint32_t lo = 0, hi = n;  // largest j with off[j] <= w
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (L.off[mid] <= w) lo = mid; else hi = mid;
-        }
-        const uint32_t j = lo;
-        float fp[3];
-        double lam[3];
-        if (!tri_sample(&L.pos[9 * j], L.setup[j], S.vs, S.samples, w - L.off[j], fp, lam)) continue;
-        int ic[3];
-        bool inside = true;
-        for (int k = 0; k < 3; ++k) {
-            const float f = floorf(fp[k] / S.vs);
-            if (!(f >= 0.0f) || f >= (float)S.res) inside = false;
-            ic[k] = (int)f;
-        }
-        if (!inside) continue;
-        atomicMin(&S.winner[((size_t)ic[2] * S.res + ic[1]) * S.res + ic[0]], a + j);
-    }
-}
-
 // One brick of mip level m. Level 1 resolves its 8 level-0 children from the
 // winners (k_dense_resolve), writes them to level 0 and returns the winners
-// to empty for the next launch.
+// to empty for the next launch. kLean (the persistent launch) keeps few
+// registers live: one face at a time, and level 1 reads its children back
+// from level 0 after writing them, so that the pixel tiles sharing the
+// launch keep four waves per SIMD; the flat mip launch keeps every load of a
+// texel in flight at once instead.
+template <bool kLean>
 __device__ void mip_task(const SceneDev& S, uint32_t brick, int m, uint32_t B)
 {
     const int n = S.res >> m;
@@ -235,13 +182,12 @@ __device__ void mip_task(const SceneDev& S, uint32_t brick, int m, uint32_t B)
         const int x = (int)(bx * be + i % be), y = (int)(by * be + (i / be) % be), z = (int)(bz * be + i / (be * be));
         if (x >= n || y >= n || z >= n) continue;
         if (m > 1) {
-            dense_mip_texel<false>(src, dst, n, x, y, z, false);
+            if (kLean) dense_mip_texel_lean<false>(src, dst, n, x, y, z, false);
+            else dense_mip_texel<false>(src, dst, n, x, y, z, false);
             continue;
         }
-        const size_t D0 = (size_t)S.res + 2, Dd = (size_t)n + 2, dblock = Dd * Dd * Dd;
-        f4 v[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
+        const size_t D0 = (size_t)S.res + 2;
+        auto resolve = [&](int c) -> uint32_t {
             const int cx = 2 * x + (c & 1), cy = 2 * y + ((c >> 1) & 1), cz = 2 * z + (c >> 2);
             uint32_t* wp = &S.winner[((size_t)cz * S.res + cy) * S.res + cx];
             const uint32_t t = *wp;
@@ -252,8 +198,18 @@ __device__ void mip_task(const SceneDev& S, uint32_t brick, int m, uint32_t B)
                 *wp = 0xffffffffu;
             }
             ((uint32_t*)src)[((size_t)(cz + 1) * D0 + (size_t)(cy + 1)) * D0 + (size_t)(cx + 1)] = packed;
-            v[c] = unpack_unorm(packed);
+            return packed;
+        };
+        if (kLean) {
+#pragma unroll 1
+            for (int c = 0; c < 8; ++c) (void)resolve(c);
+            dense_mip_texel_lean<false>(src, dst, n, x, y, z, true);  // the 8 children just written
+            continue;
         }
+        const size_t Dd = (size_t)n + 2, dblock = Dd * Dd * Dd;
+        f4 v[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = unpack_unorm(resolve(c));
 #pragma unroll
         for (int f = 0; f < kFaces; ++f) {
             const size_t di = (size_t)f * dblock + ((size_t)(z + 1) * Dd + (size_t)(y + 1)) * Dd + (size_t)(x + 1);
@@ -278,14 +234,21 @@ __device__ void trace_task(const SceneDev& S, uint32_t tile, uint32_t schedule, 
     S.out[(size_t)py * (size_t)S.width + (size_t)px] = o;
 }
 
+// The persistent launch: the mip bricks and pixel tiles of every scene, from
+// task A.first on (the vertex and raster phases ran before it as flat
+// launches). A task waits on its scene's previous phase: mip level m >= 2 on
+// level m - 1, a pixel tile on the last level; level 1 reads what the raster
+// launch left. Every task a wait depends on sits earlier in the queue, so it
+// was taken by a workgroup that is running (a workgroup takes a task only
+// while resident, and its own tasks wait on nothing later): waits end.
+template <bool kAhead>
 __global__ __launch_bounds__(256) void k_batch(Args A)
 {
-    __shared__ RasterLds lds;
     __shared__ uint32_t s_task;
     __shared__ uint32_t s_go;
     uint32_t steps = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_task = atomicAdd(A.head, 1u);
+        if (threadIdx.x == 0) s_task = A.first + atomicAdd(A.head, 1u);
         __syncthreads();
         const uint32_t ti = s_task;
         if (ti >= A.ntasks) break;
@@ -293,9 +256,8 @@ __global__ __launch_bounds__(256) void k_batch(Args A)
         const uint32_t type = T.type_scene >> 24, scene = T.type_scene & 0xffffffu;
         const SceneDev& S = A.scenes[scene];
         int dep = -1;
-        if (type == kRaster) dep = 0;
-        else if (type == kMip) dep = (int)T.level;  // level 1 waits on raster (phase 1), level m on level m-1
-        else if (type == kTrace) dep = S.pyr.nlevels;  // the last level, phase 1 + (nlevels - 1)
+        if (type == kMip && T.level > 1) dep = (int)T.level;  // phase 1 + (level - 1)
+        else if (type == kTrace) dep = S.pyr.nlevels;          // the last level, phase 1 + (nlevels - 1)
         if (threadIdx.x == 0) {
             bool ok = !__hip_atomic_load(A.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (ok && dep >= 0)
@@ -304,18 +266,20 @@ __global__ __launch_bounds__(256) void k_batch(Args A)
         }
         __syncthreads();
         if (!s_go) break;
-        __threadfence();
-        switch (type) {
-        case kVertex: vertex_task(S, T.a, T.b); break;
-        case kRaster: raster_task(S, T.a, T.b, lds); break;
-        case kMip: mip_task(S, T.a, (int)T.level, A.mip_brick); break;
-        default: trace_task(S, T.a, A.schedule, steps); break;
-        }
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0 && type != kTrace) {
-            const int phase = type == kVertex ? 0 : (type == kRaster ? 1 : 1 + (int)T.level);
-            __hip_atomic_fetch_add(&A.done[scene * kPhases + phase], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // Thread 0's acquire (its wait) and the barrier order this workgroup
+        // after the producers; a mip brick's writes are ordered before its
+        // completion count by the barrier and thread 0's release. No fence
+        // beyond those: on this chip an agent-scope fence writes back and
+        // invalidates the XCD's L2, which the pixel tiles depend on.
+        if (type == kMip) {
+            mip_task<true>(S, T.a, (int)T.level, A.mip_brick);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_fetch_add(&A.done[scene * kPhases + 1 + (int)T.level], 1u, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            trace_task(S, T.a, kAhead ? 1u : 0u, steps);
+            __syncthreads();  // s_task / s_go are rewritten by the next trip
         }
     }
     if (A.samples) {
@@ -415,19 +379,11 @@ __global__ __launch_bounds__(256) void k_phase_vertex(const Task* __restrict__ t
     vertex_task(scenes[T.type_scene & 0xffffffu], T.a, T.b);
 }
 
-__global__ __launch_bounds__(256) void k_phase_raster(const Task* __restrict__ tasks, uint32_t first,
-                                                      const SceneDev* __restrict__ scenes)
-{
-    __shared__ RasterLds lds;
-    const Task T = tasks[first + blockIdx.x];
-    raster_task(scenes[T.type_scene & 0xffffffu], T.a, T.b, lds);
-}
-
 __global__ __launch_bounds__(256) void k_phase_mip(const Task* __restrict__ tasks, uint32_t first,
                                                    const SceneDev* __restrict__ scenes, uint32_t brick)
 {
     const Task T = tasks[first + blockIdx.x];
-    mip_task(scenes[T.type_scene & 0xffffffu], T.a, (int)T.level, brick);
+    mip_task<false>(scenes[T.type_scene & 0xffffffu], T.a, (int)T.level, brick);
 }
 
 template <bool kAhead>
@@ -580,47 +536,53 @@ static vct_status capture_graph(vct_batch* b, unsigned long long* samples)
     A.schedule = b->tuning.trace_schedule;
     A.wait_ticks = b->wait_ticks;
     const unsigned grid = (unsigned)(b->cus * (int)b->tuning.workgroups_per_cu);
+    A.first = b->ntasks;
+    for (const vct_batch::Phase& p : b->phases)
+        if (p.type == kMip || p.type == kTrace) {
+            A.first = p.first;
+            break;
+        }
     VCT_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_batch_reset, dim3((b->ndone + 255) / 256), dim3(256), 0, b->capture, A.head, A.done, b->ndone,
                        A.abort_flag);
-    if (grid) {
-        hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), 0, b->capture, A);
-    } else {
-        for (const vct_batch::Phase& p : b->phases) {
-            const dim3 g(p.count);
-            if (p.type == kVertex) {
-                hipLaunchKernelGGL(k_phase_vertex, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes);
-            } else if (p.type == kRaster) {
-                // the raster chunks as one sample-parallel pass over the batch
-                const uint32_t ns = (uint32_t)b->scenes.size(), nt = b->ntris;
-                auto* counts = (unsigned long long*)b->counts.ptr;
-                auto* offsets = (unsigned long long*)b->offsets.ptr;
-                auto* total = (unsigned long long*)b->total.ptr;
-                hipLaunchKernelGGL(k_phase_count, dim3((nt + 255) / 256), dim3(256), 0, b->capture, A.scenes,
-                                   (const uint32_t*)b->tri_start.ptr, ns, nt, counts);
-                size_t tmp = b->scan_tmp.bytes;
-                hipError_t e = hipcub::DeviceScan::ExclusiveSum(b->scan_tmp.ptr, tmp, counts, offsets, (int)nt,
-                                                                b->capture);
-                if (e != hipSuccess) {
-                    hipGraph_t g0 = nullptr;
-                    (void)hipStreamEndCapture(b->capture, &g0);
-                    if (g0) (void)hipGraphDestroy(g0);
-                    return fail(VCT_ERR_HIP, "batched raster scan: %s", hipGetErrorString(e));
-                }
-                hipLaunchKernelGGL(k_phase_total, dim3(1), dim3(1), 0, b->capture, counts, offsets, nt, total);
-                hipLaunchKernelGGL(k_phase_raster_flat, dim3((unsigned)(b->cus * 32)), dim3(256), 0, b->capture,
-                                   A.scenes, (const uint32_t*)b->tri_start.ptr, ns, nt, offsets, total);
+    // Both schedules: the vertex chunks, then the raster of all scenes'
+    // triangles as one sample-parallel pass over the batch; then one flat
+    // launch per mip level and one for the pixel tiles, or (grid > 0) one
+    // persistent launch pulling the mip bricks and pixel tiles from the queue.
+    for (const vct_batch::Phase& p : b->phases) {
+        const dim3 g(p.count);
+        if (p.type == kVertex) {
+            hipLaunchKernelGGL(k_phase_vertex, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes);
+        } else if (p.type == kRaster) {
+            const uint32_t ns = (uint32_t)b->scenes.size(), nt = b->ntris;
+            auto* counts = (unsigned long long*)b->counts.ptr;
+            auto* offsets = (unsigned long long*)b->offsets.ptr;
+            auto* total = (unsigned long long*)b->total.ptr;
+            hipLaunchKernelGGL(k_phase_count, dim3((nt + 255) / 256), dim3(256), 0, b->capture, A.scenes,
+                               (const uint32_t*)b->tri_start.ptr, ns, nt, counts);
+            size_t tmp = b->scan_tmp.bytes;
+            hipError_t e = hipcub::DeviceScan::ExclusiveSum(b->scan_tmp.ptr, tmp, counts, offsets, (int)nt, b->capture);
+            if (e != hipSuccess) {
+                hipGraph_t g0 = nullptr;
+                (void)hipStreamEndCapture(b->capture, &g0);
+                if (g0) (void)hipGraphDestroy(g0);
+                return fail(VCT_ERR_HIP, "batched raster scan: %s", hipGetErrorString(e));
             }
-            else if (p.type == kMip)
-                hipLaunchKernelGGL(k_phase_mip, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes, A.mip_brick);
-            else if (A.schedule)
-                hipLaunchKernelGGL(k_phase_trace<true>, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes,
-                                   samples);
-            else
-                hipLaunchKernelGGL(k_phase_trace<false>, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes,
-                                   samples);
+            hipLaunchKernelGGL(k_phase_total, dim3(1), dim3(1), 0, b->capture, counts, offsets, nt, total);
+            hipLaunchKernelGGL(k_phase_raster_flat, dim3((unsigned)(b->cus * 32)), dim3(256), 0, b->capture, A.scenes,
+                               (const uint32_t*)b->tri_start.ptr, ns, nt, offsets, total);
+        } else if (grid) {
+            break;  // the rest is the persistent launch below
+        } else if (p.type == kMip) {
+            hipLaunchKernelGGL(k_phase_mip, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes, A.mip_brick);
+        } else if (A.schedule) {
+            hipLaunchKernelGGL(k_phase_trace<true>, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes, samples);
+        } else {
+            hipLaunchKernelGGL(k_phase_trace<false>, g, dim3(256), 0, b->capture, A.tasks, p.first, A.scenes, samples);
         }
     }
+    if (grid && A.schedule) hipLaunchKernelGGL(k_batch<true>, dim3(grid), dim3(256), 0, b->capture, A);
+    else if (grid) hipLaunchKernelGGL(k_batch<false>, dim3(grid), dim3(256), 0, b->capture, A);
     hipError_t e = hipStreamEndCapture(b->capture, &b->graph);
     if (e != hipSuccess) {
         b->graph = nullptr;
@@ -928,22 +890,20 @@ vct_status vct_batch_autotune(vct_batch* b, vct_stream stream, uint32_t reps, vc
         *ms = v[v.size() / 2];
         return vct_batch_status(b);
     };
-    // coordinate descent over the four knobs, two sweeps
-    // the persistent schedules (workgroups_per_cu 1, 2, 4) are not candidates:
-    // measured at a third of the phased schedule's speed; vct_batch_set_tuning
-    // still selects them
-    static const uint32_t chunks[] = {64, 128, 256}, bricks[] = {4, 8}, scheds[] = {0, 1}, occ[] = {0};
+    // coordinate descent over the knobs, two sweeps (the raster chunk no
+    // longer changes the launches: both schedules raster all scenes' samples
+    // in one sample-parallel pass)
+    static const uint32_t bricks[] = {4, 8}, scheds[] = {0, 1}, occ[] = {0, 1, 2, 4};
     vct_batch_tuning cur = b->tuning;
     float cur_ms = 0.0f;
     vct_status st = time_it(cur, &cur_ms);
     for (int sweep = 0; st == VCT_OK && sweep < 2; ++sweep) {
-        for (int knob = 0; st == VCT_OK && knob < 4; ++knob) {
-            const uint32_t* vals = knob == 0 ? chunks : knob == 1 ? bricks : knob == 2 ? scheds : occ;
-            const int nv = knob == 0 ? 3 : knob == 1 ? 2 : knob == 2 ? 2 : 1;
+        for (int knob = 0; st == VCT_OK && knob < 3; ++knob) {
+            const uint32_t* vals = knob == 0 ? bricks : knob == 1 ? scheds : occ;
+            const int nv = knob == 0 ? 2 : knob == 1 ? 2 : 4;
             for (int k = 0; st == VCT_OK && k < nv; ++k) {
                 vct_batch_tuning t = cur;
-                uint32_t* field = knob == 0 ? &t.raster_chunk : knob == 1 ? &t.mip_brick
-                                : knob == 2 ? &t.trace_schedule : &t.workgroups_per_cu;
+                uint32_t* field = knob == 0 ? &t.mip_brick : knob == 1 ? &t.trace_schedule : &t.workgroups_per_cu;
                 if (*field == vals[k]) continue;
                 *field = vals[k];
                 float ms = 0.0f;

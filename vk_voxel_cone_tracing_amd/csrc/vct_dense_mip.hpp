@@ -77,4 +77,42 @@ __device__ __forceinline__ void dense_mip_texel(const void* src, void* dst, int 
     }
 }
 
+// dense_mip_texel's arithmetic with one face at a time (the face loop is not
+// unrolled and each pair's children are loaded by computed index), for
+// kernels that must keep few registers live: the persistent batched launch,
+// whose pixel tiles would otherwise inherit the unrolled version's registers.
+template <bool kF32>
+__device__ __forceinline__ void dense_mip_texel_lean(const void* src, void* dst, int n, int x, int y, int z,
+                                                     bool src_single_face)
+{
+#pragma clang fp contract(off)
+    const size_t Ds = (size_t)(2 * n) + 2, Dd = (size_t)n + 2;
+    const size_t sblock = Ds * Ds * Ds, dblock = Dd * Dd * Dd;
+    auto child = [&](size_t sface, int i) {
+        return sface + ((size_t)(2 * z + (i >> 2) + 1) * Ds + (size_t)(2 * y + ((i >> 1) & 1) + 1)) * Ds +
+               (size_t)(2 * x + (i & 1) + 1);
+    };
+#pragma unroll 1
+    for (int f = 0; f < kFaces; ++f) {
+        const size_t sface = src_single_face ? 0 : (size_t)f * sblock;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const f4 a = dense_load<kF32>(src, child(sface, face_pair_child(f, p, 0)));
+            const f4 b = dense_load<kF32>(src, child(sface, face_pair_child(f, p, 1)));
+            const float t = 1.0f - a.w;
+            acc[0] = acc[0] + a.x;
+            acc[0] = acc[0] + t * b.x;
+            acc[1] = acc[1] + a.y;
+            acc[1] = acc[1] + t * b.y;
+            acc[2] = acc[2] + a.z;
+            acc[2] = acc[2] + t * b.z;
+            acc[3] = acc[3] + a.w;
+            acc[3] = acc[3] + t * b.w;
+        }
+        const size_t di = (size_t)f * dblock + ((size_t)(z + 1) * Dd + (size_t)(y + 1)) * Dd + (size_t)(x + 1);
+        dense_store<kF32>(dst, di, f4{acc[0] * 0.25f, acc[1] * 0.25f, acc[2] * 0.25f, acc[3] * 0.25f});
+    }
+}
+
 }  // namespace vct

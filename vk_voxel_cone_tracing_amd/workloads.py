@@ -294,16 +294,19 @@ def batched(seed: int = 3, count: int = 64, W: int = 512, H: int = 512, reps: in
         A.call("vct_dense_create", C.byref(h), C.byref(A.DenseDesc(A.f3(*it["gmin"]), it["ext"], it["res"], 0, 1, 0)))
         grids.append(h)
 
-    def loop():
+    def loop(build=True, trace=True):
         for h, it in zip(grids, items):
-            A.call("vct_dense_clear", h, s)
-            A.call("vct_dense_voxelize", h, s, C.byref(it["ds"].mesh))
-            A.call("vct_dense_mipmap", h, s)
-            A.call("vct_dense_trace", h, s, C.byref(it["gb"]), C.byref(it["cam"]), C.byref(td),
-                   C.c_void_p(it["out"].data_ptr()), None)
+            if build:
+                A.call("vct_dense_clear", h, s)
+                A.call("vct_dense_voxelize", h, s, C.byref(it["ds"].mesh))
+                A.call("vct_dense_mipmap", h, s)
+            if trace:
+                A.call("vct_dense_trace", h, s, C.byref(it["gb"]), C.byref(it["cam"]), C.byref(td),
+                       C.c_void_p(it["out"].data_ptr()), None)
 
     try:
         loop_ms = time_ms(loop, reps)
+        loop_trace_ms = time_ms(lambda: loop(build=False), reps)  # the loop's 64 traces alone
     finally:
         for h in grids:
             A.call("vct_dense_destroy", h)
@@ -312,6 +315,9 @@ def batched(seed: int = 3, count: int = 64, W: int = 512, H: int = 512, reps: in
     A.call("vct_batch_create", C.byref(b), arr, len(items), C.byref(td))
     try:
         default_ms = time_ms(lambda: A.call("vct_batch_run", b, s, None), reps)
+        A.call("vct_batch_set_tuning", b, C.byref(A.BatchTuning(256, 8, 0, 4)))  # the persistent queue
+        persistent_ms = time_ms(lambda: A.call("vct_batch_run", b, s, None), reps)
+        A.call("vct_batch_status", b)
         best = A.BatchTuning()
         tuned = C.c_float()
         A.call("vct_batch_autotune", b, s, 2, C.byref(best), C.byref(tuned))
@@ -321,7 +327,10 @@ def batched(seed: int = 3, count: int = 64, W: int = 512, H: int = 512, reps: in
         A.call("vct_batch_destroy", b)
     tris = sum(it["scene"].triangle_count for it in items)
     return dict(workload=f"{count} scenes, {tris} tris in all, grids 64^3-256^3, VXAO {W}x{H} each",
-                per_scene_loop_ms=round(loop_ms, 3), batched_default_ms=round(default_ms, 3),
+                per_scene_loop_ms=round(loop_ms, 3), loop_trace_ms=round(loop_trace_ms, 3),
+                speedup_bound=round(loop_ms / loop_trace_ms, 3),
+                speedup_bound_note="the loop over its own traces: the batched trace runs the same arithmetic",
+                batched_default_ms=round(default_ms, 3), batched_persistent_ms=round(persistent_ms, 3),
                 batched_ms=round(batch_ms, 3), speedup=round(loop_ms / batch_ms, 3),
                 tuning=dict(raster_chunk=best.raster_chunk, mip_brick=best.mip_brick,
                             trace_schedule=best.trace_schedule, workgroups_per_cu=best.workgroups_per_cu))
