@@ -168,15 +168,20 @@ def new_volume(res: int, levels: int, fmt: int = FMT_RGBA8) -> np.ndarray:
 
 
 def vol_struct(arr: np.ndarray, res: int, levels: int) -> Volume:
+    """The volume's structure; it keeps `arr` alive, so a temporary array may
+    be passed."""
     fmt = FMT_RGBA8 if arr.dtype == np.uint32 else FMT_RGBA16F
     assert arr.flags.c_contiguous
-    return Volume(arr.ctypes.data, fmt, res, levels)
+    v = Volume(arr.ctypes.data, fmt, res, levels)
+    v._keep = arr
+    return v
 
 
 def tris_struct(pos, nrm, mat, scene=None) -> Tris:
     """The soup; with a textured `scene`, also its texcoords (in
     world_triangles order) and texture table, kept alive by the structure."""
     t = Tris(pos.ctypes.data, nrm.ctypes.data, mat.ctypes.data, len(mat))
+    t._arrays = (pos, nrm, mat)
     if scene is not None and getattr(scene, "textures", None):
         uv = scene.triangle_uvs()
         texels = [np.ascontiguousarray(x, np.uint32) for x in scene.textures]
@@ -228,9 +233,13 @@ def vox_desc(region, level: int, res: int, extent_l0: float = 16.0, voxel_size=N
 
 
 def gbuf_struct(g: dict) -> GBuf:
+    """The G-buffer's structure; it keeps the arrays of `g` alive, so a
+    temporary dict may be passed."""
     h, w = g["depth"].shape
-    return GBuf(g["diffuse"].ctypes.data, g["normal"].ctypes.data, g["specular"].ctypes.data,
-                g["emission"].ctypes.data, g["depth"].ctypes.data, w, h)
+    s = GBuf(g["diffuse"].ctypes.data, g["normal"].ctypes.data, g["specular"].ctypes.data,
+             g["emission"].ctypes.data, g["depth"].ctypes.data, w, h)
+    s._keep = dict(g)
+    return s
 
 
 def raycast_gbuffer(scene, pos, nrm, mat, vp, vp_inv, width, height):
