@@ -96,6 +96,21 @@ __device__ __forceinline__ float wave_min(float v)
     return v;
 }
 
+// N wave maxima at once: each butterfly step shuffles all N values, so the
+// N chains' latencies overlap instead of adding up.
+template <int N>
+__device__ __forceinline__ void wave_max_n(float (&v)[N])
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        float t[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) t[i] = __shfl_xor(v[i], o, 64);
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = fmaxf(v[i], t[i]);
+    }
+}
+
 // The interior cells of level m that a set of start points can reach while
 // the level is in use: their box swept along the cone over the level's steps
 // [ta, tb], one cell wider on each side for rounding, clipped to the
@@ -177,12 +192,21 @@ __device__ __forceinline__ Window window_of(const Pyramid& P, int m, const float
         const float mid = 0.5f * (blo[ax] + bhi[ax]);
         const bool up = lane_in && start[ax] >= mid, down = lane_in && start[ax] < mid;
         float lo0[3], hi0[3], lo1[3], hi1[3];
+        float r[12];  // the twelve reductions side by side (minima as maxima of negatives)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            lo0[k] = wave_min(down ? start[k] : INFINITY);
-            hi0[k] = wave_max(down ? start[k] : -INFINITY);
-            lo1[k] = wave_min(up ? start[k] : INFINITY);
-            hi1[k] = wave_max(up ? start[k] : -INFINITY);
+            r[k] = down ? -start[k] : -INFINITY;
+            r[3 + k] = down ? start[k] : -INFINITY;
+            r[6 + k] = up ? -start[k] : -INFINITY;
+            r[9 + k] = up ? start[k] : -INFINITY;
+        }
+        wave_max_n<12>(r);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo0[k] = -r[k];
+            hi0[k] = r[3 + k];
+            lo1[k] = -r[6 + k];
+            hi1[k] = r[9 + k];
         }
         const Box b0 = box_of(P, m, lo0, hi0, dir, ta, tb), b1 = box_of(P, m, lo1, hi1, dir, ta, tb);
         if (b0.vol + b1.vol < W.part[0].vol) {
@@ -358,13 +382,19 @@ __device__ __forceinline__ void flush_window(float* __restrict__ grad, int* __re
         float* g0 = grad + 4ull * org;
         float* gf[3];
         for (int a = 0; a < 3; ++a) gf[a] = g0 + (m == 0 ? 0ull : 4ull * (unsigned long long)face[a] * P.block[m]);
+        const int stride = copy_stride(B.vol);
         for (int e = lane; e < 4 * B.vol; e += 64) {
             int acc = 0;
             const int t = e >> 2, ch = e & 3;
-            for (int cc = 0; cc < W.copies; ++cc) {
-                const int li = B.base + cc * copy_stride(B.vol) + ch * B.vol + t;
-                acc += win[li];
-                win[li] = 0;
+            const int li0 = B.base + ch * B.vol + t;
+            // up to kMaxCopies reads in flight, then the zeroing
+            int part[kMaxCopies];
+#pragma unroll
+            for (int cc = 0; cc < kMaxCopies; ++cc) part[cc] = cc < W.copies ? win[li0 + cc * stride] : 0;
+#pragma unroll
+            for (int cc = 0; cc < kMaxCopies; ++cc) {
+                acc += part[cc];
+                if (cc < W.copies) win[li0 + cc * stride] = 0;
             }
             if (acc == 0) continue;
             const int z = (int)(((float)t + 0.5f) * inv_d01);
@@ -714,6 +744,10 @@ __device__ __forceinline__ bool sweep_slot(const Pyramid& P, const March& M, flo
     const int kc = (int)wave_max((float)nt);
     if (kc == 0) return false;
     float4* col = tape + slot * (size_t)cap * 64u + (size_t)lane;
+    // steps below the wave's count are in the slot for every lane (beyond a
+    // lane's own count they hold no sample of it and only feed state that is
+    // never read); the reverse sweep reads them again rather than keeping
+    // them live across it
     float r_rw[kTapeRegs], r_occ[kTapeRegs], r_pc[kTapeRegs];
     f4 result = {0.0f, 0.0f, 0.0f, 0.0f};
     float occlusion = 0.0f;
@@ -746,13 +780,24 @@ __device__ __forceinline__ bool sweep_slot(const Pyramid& P, const March& M, flo
     H.kc = (uint32_t)kc;
     H.base = (uint32_t)(slot * (size_t)cap * 64u);  // the host keeps the tape below 2^32 float4
     H.mask = __ballot(lane_on);
+    // the ten reductions side by side, so their shuffles overlap: the box of
+    // the start points as maxima of (-lo, hi), then the gradient bounds
+    float r[10];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        H.blo[k] = wave_min(lane_on ? start[k] : INFINITY);
-        H.bhi[k] = wave_max(lane_on ? start[k] : -INFINITY);
+        r[k] = lane_on ? -start[k] : -INFINITY;
+        r[3 + k] = lane_on ? start[k] : -INFINITY;
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) H.gmax[c] = wave_max(amax[c]);
+    for (int c = 0; c < 4; ++c) r[6 + c] = amax[c];
+    wave_max_n<10>(r);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        H.blo[k] = -r[k];
+        H.bhi[k] = r[3 + k];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) H.gmax[c] = r[6 + c];
     return true;
 }
 
