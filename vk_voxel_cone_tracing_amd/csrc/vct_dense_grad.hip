@@ -82,18 +82,26 @@ constexpr int kMaxCopies = 4;     // lane-striped copies of a small window
 constexpr int kCoarseN = 4;       // levels with at most 4^3 interior cells stay in LDS per tile
 constexpr int kCoarseFloats = 24 * (64 + 8 + 1);
 
+// A value every lane holds alike, moved to a scalar register: what is
+// derived from it (window boxes, scales, loop bounds) then stays in scalar
+// registers instead of taking a vector register per value.
+__device__ __forceinline__ float uniform(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 __device__ __forceinline__ float wave_max(float v)
 {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+    return uniform(v);
 }
 
 __device__ __forceinline__ float wave_min(float v)
 {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
+    return uniform(v);
 }
 
 // N wave maxima at once: each butterfly step shuffles all N values, so the
@@ -109,6 +117,8 @@ __device__ __forceinline__ void wave_max_n(float (&v)[N])
 #pragma unroll
         for (int i = 0; i < N; ++i) v[i] = fmaxf(v[i], t[i]);
     }
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = uniform(v[i]);
 }
 
 // The interior cells of level m that a set of start points can reach while
@@ -864,7 +874,7 @@ __global__ __launch_bounds__(256, 2) void k_dense_bwd_splat(Pyramid P, March M, 
     for (;;) {
         uint32_t h = 0;
         if (lane == 0) h = atomicAdd(next, 1u);
-        h = __shfl(h, 0, 64);
+        h = __builtin_amdgcn_readfirstlane(h);  // lane 0 took it
         if (h >= nh) break;
         SweepHeader H;
         float start[3] = {0.0f, 0.0f, 0.0f};
