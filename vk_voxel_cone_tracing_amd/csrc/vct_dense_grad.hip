@@ -123,9 +123,14 @@ __device__ __forceinline__ void wave_max_n(float (&v)[N])
 
 // The interior cells of level m that a set of start points can reach while
 // the level is in use: their box swept along the cone over the level's steps
-// [ta, tb], one cell wider on each side for rounding, clipped to the
-// interior (border texels are constant zero and take no gradient). An empty
-// set gives vol 0.
+// [ta, tb], clipped to the interior (border texels are constant zero and take
+// no gradient). An empty set gives vol 0. A footprint with corner u covers
+// cells floor(u) + 1 and + 2 (bordered), so the box is exact: a sample whose
+// position rounds one cell past it (its position is computed in another
+// order) takes splat_level's per-corner path, which adds what falls outside
+// the window to global memory. (One cell of margin on each side, to keep such
+// samples in the window, made the fine levels' windows 2.4x larger and the
+// backward 1.3x slower, measured.)
 struct Box {
     int lo[3], d[3];
     int vol, base;
@@ -144,8 +149,8 @@ __device__ __forceinline__ Box box_of(const Pyramid& P, int m, const float blo[3
         const float hi_w = fmaxf(bhi[k] + dir[k] * ta, bhi[k] + dir[k] * tb);
         const float a = fminf(fmaxf(lo_w * P.inv_vs[m] - 0.5f, -4.0f), (float)n + 4.0f);
         const float b = fminf(fmaxf(hi_w * P.inv_vs[m] - 0.5f, -4.0f), (float)n + 4.0f);
-        const int lo = max((int)floorf(a), 1);
-        const int hi = min((int)floorf(b) + 3, n);
+        const int lo = max((int)floorf(a) + 1, 1);
+        const int hi = min((int)floorf(b) + 2, n);
         B.lo[k] = lo;
         B.d[k] = hi >= lo ? hi - lo + 1 : 0;
         B.vol *= B.d[k];
