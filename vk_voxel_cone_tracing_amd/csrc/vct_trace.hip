@@ -122,9 +122,11 @@ __device__ __forceinline__ uint32_t footprint(const TraceParams& P, const float 
         const float u = (q - floorf(q)) * (float)P.res + 0.5f;
         const float fu = floorf(u);
         fr[k] = u - fu;
-        int i = (int)fu;
-        i = i < 0 ? 0 : (i > P.res ? P.res : i);
-        i0[k] = i;
+        // no clamp needed: q - floor(q) lies in [0, 1] up to one rounding of
+        // the product (the fused form can leave it a hair below 0 or above
+        // 1), so u is within [0.5, res + 0.5] but for res x that hair, and
+        // floor(u) within [0, res]; the oracle's clamp to [0, res] never acts
+        i0[k] = (int)fu;
     }
     const uint32_t D = (uint32_t)P.D;
     return ((uint32_t)i0[2] * D + (uint32_t)i0[1]) * D + (uint32_t)i0[0];
@@ -133,7 +135,8 @@ __device__ __forceinline__ uint32_t footprint(const TraceParams& P, const float 
 // Whether the footprint with corner o000 of `level` holds a nonzero texel.
 __device__ __forceinline__ bool occupied(const TraceParams& P, int level, uint32_t o000)
 {
-    return (P.occ[(size_t)level * P.occ_words + (o000 >> 5)] >> (o000 & 31u)) & 1u;
+    // a 32-bit word index: the mask is levels x occ_words words, far below 2^32
+    return (P.occ[(uint32_t)level * P.occ_words + (o000 >> 5)] >> (o000 & 31u)) & 1u;
 }
 
 // Trilinear sample of the three selected faces of one level, weighted.
@@ -220,8 +223,11 @@ __device__ __forceinline__ float cone_level(const TraceParams& P, const float po
 {
     const float dx = P.center[0] - pos[0], dy = P.center[1] - pos[1], dz = P.center[2] - pos[2];
     const float dist = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
-    const float min_level = ceilf(log2f(dist * P.inv_min_radius));
-    const float cur = log2f(diameter * P.inv_voxel_size);
+    // the hardware log2 (v_log_f32): log2f is the same instruction for every
+    // argument but denormals, which it first scales by 2^32; neither argument
+    // is ever one (0 gives -inf either way)
+    const float min_level = ceilf(__builtin_amdgcn_logf(dist * P.inv_min_radius));
+    const float cur = __builtin_amdgcn_logf(diameter * P.inv_voxel_size);
     return fminf(fmaxf(fmaxf(start_level, cur), min_level), max_level);
 }
 
