@@ -270,8 +270,10 @@ __device__ __forceinline__ f4 cone_sample(const typename Vol<kHalf>::T* __restri
 __device__ __forceinline__ void cone_accumulate(const TraceParams& P, f4& result, float& occlusion, const f4& s,
                                                 float cur, float step, float seg)
 {
-    const float voxel_size = P.voxel_size * exp2f(cur);
-    const float correction = seg * (P.inv_voxel_size * exp2f(-cur));
+    // the hardware exp2 (v_exp_f32): exp2f is the same instruction for every
+    // argument whose result is a normal number, and cur lies in [0, levels - 1]
+    const float voxel_size = P.voxel_size * __builtin_amdgcn_exp2f(cur);
+    const float correction = seg * (P.inv_voxel_size * __builtin_amdgcn_exp2f(-cur));
     // 1 - a clamped at 0: a filtered alpha of texels <= 1 can exceed 1 by an
     // ulp in FP32 (the reference's fixed-point filter weights sum to 1
     // exactly), and pow of a negative base is undefined
