@@ -453,7 +453,7 @@ __device__ __forceinline__ int step_table(const Pyramid& P, const March& M, floa
     while (step < M.max_dist && cnt < kAdjMaxSteps) {
         step_t[cnt] = step;
         seg_t[cnt] = seg;
-        lvl_t[cnt] = fminf(fmaxf(log2f(diameter * inv_vs), 0.0f), max_level);
+        lvl_t[cnt] = fminf(fmaxf(hw_log2(diameter * inv_vs), 0.0f), max_level);
         ++cnt;
         const float prev = step;
         step += fmaxf(diameter, vs) * M.step_factor;
@@ -471,7 +471,7 @@ __device__ __forceinline__ void step_coefs(const Pyramid& P, const March& M, int
                                            float* iq_t)
 {
     for (int k = 0; k < cnt; ++k) {
-        const float vsl = P.vs * exp2f(lvl_t[k]);
+        const float vsl = P.vs * hw_exp2(lvl_t[k]);
         corr_t[k] = seg_t[k] / vsl;
         q_t[k] = 1.0f + (step_t[k] + vsl) * M.decay;
         iq_t[k] = 1.0f / q_t[k];
@@ -482,7 +482,7 @@ __device__ __forceinline__ void step_coefs(const Pyramid& P, const March& M, int
 // (1 - s.a)^corr for the reverse sweep.
 __device__ __forceinline__ float replay_step(f4& result, float& occlusion, const f4& s, float corr, float q)
 {
-    const float pc = powf(fmaxf(1.0f - s.w, 0.0f), corr);
+    const float pc = hw_pow(fmaxf(1.0f - s.w, 0.0f), corr);
     const float op = fminf(fmaxf(1.0f - pc, 0.0f), 1.0f);
     const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
     result.x += blend * (s.x * corr);

@@ -192,13 +192,23 @@ __device__ __forceinline__ f4 level_sample(const void* __restrict__ data, const 
     return s;
 }
 
+// The march's transcendental steps on the hardware units (v_log_f32,
+// v_exp_f32): log2f and exp2f are these instructions for every argument but
+// denormal inputs or results, which the march's level and voxel-size terms
+// never produce; (1 - a)^c is exp2(c log2(1 - a)) as in the clipmap trace
+// (1 - a = 0 gives log2 -inf and exp2 0, the power's value at 0). The adjoint
+// replays the march with the same functions.
+__device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float hw_pow(float base, float c) { return hw_exp2(c * hw_log2(base)); }
+
 // Front-to-back accumulation of one sample (voxelConeTracing.frag:370-386).
 __device__ __forceinline__ void accumulate(f4& result, float& occlusion, const f4& s, float lvl, float step, float seg,
                                            float vs, float decay)
 {
-    const float vsl = vs * exp2f(lvl);
+    const float vsl = vs * hw_exp2(lvl);
     const float corr = seg / vsl;
-    const float op = fminf(fmaxf(1.0f - powf(fmaxf(1.0f - s.w, 0.0f), corr), 0.0f), 1.0f);  // 1 - a >= 0
+    const float op = fminf(fmaxf(1.0f - hw_pow(fmaxf(1.0f - s.w, 0.0f), corr), 0.0f), 1.0f);  // 1 - a >= 0
     const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
     result.x += blend * (s.x * corr);
     result.y += blend * (s.y * corr);
@@ -247,7 +257,7 @@ __device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March
         float pos[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) pos[k] = start[k] + dir[k] * step;
-        const float lvl = fminf(fmaxf(log2f(diameter * inv_vs), 0.0f), max_level);
+        const float lvl = fminf(fmaxf(hw_log2(diameter * inv_vs), 0.0f), max_level);
         if (!kAhead) {
             const f4 s = level_sample<kF32>(data, P, pos, lvl, face, w);
             accumulate(result, occlusion, s, lvl, step, seg, vs, M.decay);
@@ -264,7 +274,7 @@ __device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March
             float pos_b[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) pos_b[k] = start[k] + dir[k] * step_b;
-            const float lvl_b = fminf(fmaxf(log2f(diam_b * inv_vs), 0.0f), max_level);
+            const float lvl_b = fminf(fmaxf(hw_log2(diam_b * inv_vs), 0.0f), max_level);
             const f4 sa = level_sample<kF32>(data, P, pos, lvl, face, w);
             const f4 sb = level_sample<kF32>(data, P, pos_b, lvl_b, face, w);
             accumulate(result, occlusion, sa, lvl, step, seg, vs, M.decay);
