@@ -773,17 +773,21 @@ __device__ f4 svo_trace_cone(const uint2* __restrict__ pool, const TraceParams& 
         for (int k = 0; k < 3; ++k) pos[k] = start[k] + dir[k] * step;
         const float dx = P.center[0] - pos[0], dy = P.center[1] - pos[1], dz = P.center[2] - pos[2];
         const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float min_level = ceilf(log2f(dist / min_radius));
-        float cur = log2f(diameter / P.voxel_size);
+        // hardware log2 / exp2 / pow as in the clipmap march (same results
+        // for these arguments, see cone_level and cone_accumulate)
+        const float min_level = ceilf(__builtin_amdgcn_logf(dist / min_radius));
+        float cur = __builtin_amdgcn_logf(diameter / P.voxel_size);
         cur = fminf(fmaxf(fmaxf(start_level, cur), min_level), max_level);
         const f4 lo = sample_svo(pool, P, S, pos, (uint32_t)floorf(cur));
         const f4 hi = sample_svo(pool, P, S, pos, (uint32_t)ceilf(cur));
         const float fl = cur - floorf(cur);
         const f4 s = {lo.x * (1.0f - fl) + hi.x * fl, lo.y * (1.0f - fl) + hi.y * fl, lo.z * (1.0f - fl) + hi.z * fl,
                       lo.w * (1.0f - fl) + hi.w * fl};
-        voxel_size = P.voxel_size * exp2f(cur);
+        voxel_size = P.voxel_size * __builtin_amdgcn_exp2f(cur);
         const float correction = seg / voxel_size;
-        const float opacity = fminf(fmaxf(1.0f - powf(fmaxf(1.0f - s.w, 0.0f), correction), 0.0f), 1.0f);
+        const float opacity = fminf(
+            fmaxf(1.0f - __builtin_amdgcn_exp2f(correction * __builtin_amdgcn_logf(fmaxf(1.0f - s.w, 0.0f))), 0.0f),
+            1.0f);
         const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
         result.x += blend * (s.x * correction);
         result.y += blend * (s.y * correction);
