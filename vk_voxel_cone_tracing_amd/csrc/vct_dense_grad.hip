@@ -463,34 +463,21 @@ __device__ __forceinline__ int step_table(const Pyramid& P, const March& M, floa
     return step < M.max_dist ? -1 : cnt;
 }
 
-// Per-step constants of accumulate() (vct_dense_march.hpp), with the same
-// arithmetic: the opacity correction seg / vs_l, the occlusion divisor
-// 1 + (t + vs_l) decay and its reciprocal.
+// Per-step constants of accumulate() (vct_dense_march.hpp), from the same
+// step_consts(): the opacity correction and the reciprocal of the occlusion
+// divisor 1 + (t + vs_l) decay.
 __device__ __forceinline__ void step_coefs(const Pyramid& P, const March& M, int cnt, const float* step_t,
-                                           const float* seg_t, const float* lvl_t, float* corr_t, float* q_t,
-                                           float* iq_t)
+                                           const float* seg_t, const float* lvl_t, float* corr_t, float* iq_t)
 {
-    for (int k = 0; k < cnt; ++k) {
-        const float vsl = P.vs * hw_exp2(lvl_t[k]);
-        corr_t[k] = seg_t[k] / vsl;
-        q_t[k] = 1.0f + (step_t[k] + vsl) * M.decay;
-        iq_t[k] = 1.0f / q_t[k];
-    }
+    const float inv_vs = 1.0f / P.vs;
+    for (int k = 0; k < cnt; ++k) step_consts(lvl_t[k], step_t[k], seg_t[k], P.vs, inv_vs, M.decay, corr_t[k], iq_t[k]);
 }
 
 // accumulate() with the step's constants from the table; returns
 // (1 - s.a)^corr for the reverse sweep.
-__device__ __forceinline__ float replay_step(f4& result, float& occlusion, const f4& s, float corr, float q)
+__device__ __forceinline__ float replay_step(f4& result, float& occlusion, const f4& s, float corr, float iq)
 {
-    const float pc = hw_pow(fmaxf(1.0f - s.w, 0.0f), corr);
-    const float op = fminf(fmaxf(1.0f - pc, 0.0f), 1.0f);
-    const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
-    result.x += blend * (s.x * corr);
-    result.y += blend * (s.y * corr);
-    result.z += blend * (s.z * corr);
-    result.w += blend * op;
-    occlusion += (1.0f - occlusion) * op / q;
-    return pc;
+    return accumulate(result, occlusion, s, corr, iq);
 }
 
 // One (wave, cone) of the reverse sweep, handed from the sweep kernel to the
@@ -568,7 +555,7 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
 {
     constexpr bool kStore = kMode != kSweepRecompute;
     __shared__ float s_step[kAdjMaxSteps], s_seg[kAdjMaxSteps], s_lvl[kAdjMaxSteps];
-    __shared__ float s_corr[kAdjMaxSteps], s_q[kAdjMaxSteps], s_iq[kAdjMaxSteps];
+    __shared__ float s_corr[kAdjMaxSteps], s_iq[kAdjMaxSteps];
     __shared__ int s_nsteps;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned tiles_x = ((unsigned)width + 15u) / 16u;
@@ -579,7 +566,7 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
     if (tid == 0) {
         const int cnt = step_table(P, M, s_step, s_seg, s_lvl);
         s_nsteps = cnt;
-        step_coefs(P, M, cnt, s_step, s_seg, s_lvl, s_corr, s_q, s_iq);
+        step_coefs(P, M, cnt, s_step, s_seg, s_lvl, s_corr, s_iq);
     }
     bool on = px < width && py < height;
     size_t idx = 0;
@@ -637,7 +624,7 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
             for (int k = 0; k < nt; ++k) {
                 st_rw[k] = result.w;
                 st_occ[k] = occlusion;
-                st_pc[k] = replay_step(result, occlusion, smp[kStore ? k : 0], s_corr[k], s_q[k]);
+                st_pc[k] = replay_step(result, occlusion, smp[kStore ? k : 0], s_corr[k], s_iq[k]);
             }
             n = nt;
         } else if (lane_on) {
@@ -656,13 +643,13 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
                 st_rw[k] = result.w;
                 st_occ[k] = occlusion;
                 if (kStore) smp[kStore ? k : 0] = sa;
-                st_pc[k] = replay_step(result, occlusion, sa, s_corr[k], s_q[k]);
+                st_pc[k] = replay_step(result, occlusion, sa, s_corr[k], s_iq[k]);
                 n = k + 1;
                 if (!two || !(occlusion < 1.0f)) break;
                 st_rw[k + 1] = result.w;
                 st_occ[k + 1] = occlusion;
                 if (kStore) smp[kStore ? k + 1 : 0] = sb;
-                st_pc[k + 1] = replay_step(result, occlusion, sb, s_corr[k + 1], s_q[k + 1]);
+                st_pc[k + 1] = replay_step(result, occlusion, sb, s_corr[k + 1], s_iq[k + 1]);
                 n = k + 2;
             }
         }
@@ -742,7 +729,7 @@ __global__ __launch_bounds__(256) void k_dense_bwd_sweep(const void* __restrict_
 // fields the splat reads. Returns false when the slot holds no step.
 __device__ __forceinline__ bool sweep_slot(const Pyramid& P, const March& M, float4* __restrict__ tape,
                                            const uint8_t* __restrict__ counts, uint32_t cap, size_t slot, int cone,
-                                           int nsteps, const float* s_corr, const float* s_q, const float* s_iq,
+                                           int nsteps, const float* s_corr, const float* s_iq,
                                            bool on, const float start[3], const float normal[3],
                                            const float albedo[3], const float4& go, int lane,
                                            uint32_t* __restrict__ status, SweepHeader& H)
@@ -772,7 +759,7 @@ __device__ __forceinline__ bool sweep_slot(const Pyramid& P, const March& M, flo
         const float4 t = col[(size_t)k * 64u];
         r_rw[k] = result.w;
         r_occ[k] = occlusion;
-        r_pc[k] = replay_step(result, occlusion, {t.x, t.y, t.z, t.w}, s_corr[k], s_q[k]);
+        r_pc[k] = replay_step(result, occlusion, {t.x, t.y, t.z, t.w}, s_corr[k], s_iq[k]);
     }
     const float inv = 1.0f / (float)M.ncones;
     f4 gR = {go.x * inv * albedo[0] * M.diffuse_intensity * ct, go.y * inv * albedo[1] * M.diffuse_intensity * ct,
@@ -857,7 +844,7 @@ __global__ __launch_bounds__(256, 2) void k_dense_bwd_splat(Pyramid P, March M, 
     __shared__ double s_win[4][2 * kPoolUnits];
     __shared__ double s_coarse[kCoarseFloats];
     __shared__ float s_step[kAdjMaxSteps], s_seg[kAdjMaxSteps], s_lvl[kAdjMaxSteps];
-    __shared__ float s_corr[kAdjMaxSteps], s_q[kAdjMaxSteps], s_iq[kAdjMaxSteps];
+    __shared__ float s_corr[kAdjMaxSteps], s_iq[kAdjMaxSteps];
     __shared__ int s_nsteps;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const Coarse C = coarse_layout(P);
@@ -867,7 +854,7 @@ __global__ __launch_bounds__(256, 2) void k_dense_bwd_splat(Pyramid P, March M, 
     if (tid == 0) {
         const int cnt = step_table(P, M, s_step, s_seg, s_lvl);
         s_nsteps = cnt;
-        if (kFused && cnt > 0) step_coefs(P, M, cnt, s_step, s_seg, s_lvl, s_corr, s_q, s_iq);
+        if (kFused && cnt > 0) step_coefs(P, M, cnt, s_step, s_seg, s_lvl, s_corr, s_iq);
     }
     __syncthreads();
     if (kFused && s_nsteps < 0) {
@@ -908,7 +895,7 @@ __global__ __launch_bounds__(256, 2) void k_dense_bwd_splat(Pyramid P, March M, 
                 }
             }
             if (!sweep_slot(P, M, VW.tape[v], VW.counts[v], cap, slot, (int)(slot % (uint32_t)M.ncones), s_nsteps,
-                            s_corr, s_q, s_iq, on, start, normal, albedo, go, lane, status, H))
+                            s_corr, s_iq, on, start, normal, albedo, go, lane, status, H))
                 continue;
         } else {
             H = hdr[h];

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "vct_dense.hpp"
 #include "vct_texel.hpp"
@@ -70,24 +71,37 @@ __device__ __forceinline__ const float* cone_dir(const March& M, int i)
     return M.cones32 ? c_dense_dirs32[i] : c_dense_dirs16[i];
 }
 
-// A texel's channels: the floats of an RGBA32F grid, or the bytes (0..255,
-// not yet divided by 255) of an RGBA8 grid; trilinear() applies the 1/255
-// once to its result instead of to each of the 8 texels.
-template <bool kF32>
-__device__ __forceinline__ f4 texel(const void* __restrict__ data, unsigned long long i)
+// A texel's channels: the floats of an RGBA32F grid, or the bytes of an
+// RGBA8 grid as half-float denormals, b x 2^-24 (exact): the half-to-float
+// conversion then happens inside the trilinear FMAs (v_fma_mix_f32), two
+// bit operations per texel instead of four byte conversions. trilinear()
+// applies 2^24 / 255 once to its result; scaling every product and partial
+// sum by 2^-24 is exact, and fl(2^24 / 255) = 2^24 fl(1 / 255), so the result
+// is bit for bit that of bytes 0..255 scaled by 1 / 255.
+__device__ __forceinline__ float half_bits(uint32_t x)
 {
-    if (kF32) {
-        const float4 t = ((const float4*)data)[i];
-        return {t.x, t.y, t.z, t.w};
-    }
-    const uint32_t t = ((const uint32_t*)data)[i];
-    return {(float)(t & 0xffu), (float)((t >> 8) & 0xffu), (float)((t >> 16) & 0xffu), (float)(t >> 24)};
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)x);
 }
 
-// Footprint of one trilinear fetch of level m (bordered indices).
+template <bool kF32>
+__device__ __forceinline__ f4 texel(const void* __restrict__ p)
+{
+    if (kF32) {
+        const float4 t = *(const float4*)p;
+        return {t.x, t.y, t.z, t.w};
+    }
+    const uint32_t t = *(const uint32_t*)p;
+    const uint32_t rb = t & 0x00ff00ffu;                         // R | B << 16
+    const uint32_t ga = __builtin_amdgcn_perm(t, t, 0x0c030c01u);  // G | A << 16
+    return {half_bits(rb), half_bits(ga), half_bits(rb >> 16), half_bits(ga >> 16)};
+}
+
+// Footprint of one trilinear fetch of level m (bordered indices). Texel
+// indices are 32-bit: a pyramid of at most 1024^3 holds about 2.0e9 texels
+// (make_params refuses more than 2^32).
 struct Footprint {
-    unsigned long long base;  // texel index of the (0,0,0) corner inside face 0
-    unsigned long long dy, dz;
+    uint32_t base;  // texel index of the (0,0,0) corner inside face 0
+    uint32_t dy, dz;
     float fx, fy, fz;
     bool valid;
 };
@@ -96,8 +110,8 @@ __device__ __forceinline__ Footprint footprint(const Pyramid& P, int m, const fl
 {
     Footprint F;
     const int n = P.n[m];
-    const unsigned long long D = (unsigned long long)n + 2;
-    int i0[3];
+    const uint32_t D = (uint32_t)n + 2u;
+    uint32_t i0[3];
     float fr[3];
     F.valid = true;
 #pragma unroll
@@ -106,10 +120,12 @@ __device__ __forceinline__ Footprint footprint(const Pyramid& P, int m, const fl
         const float f = floorf(u);
         fr[k] = u - f;
         if (!(f >= -1.0f) || f > (float)(n - 1)) F.valid = false;
-        int i = (int)f + 1;
-        i0[k] = i < 0 ? 0 : (i > n ? n : i);
+        // f + 1 lies in [0, n] where the footprint is valid; the median only
+        // keeps the conversion defined for the others, whose texels are not read
+        i0[k] = (uint32_t)((int)__builtin_amdgcn_fmed3f(f, -1.0f, (float)(n - 1)) + 1);
     }
-    F.base = P.offset[m] + ((unsigned long long)i0[2] * D + (unsigned long long)i0[1]) * D + (unsigned long long)i0[0];
+    // (i2 D + i1) < 2^24 and D < 2^24: two 24-bit multiply-adds
+    F.base = (uint32_t)P.offset[m] + __umul24(__umul24(i0[2], D) + i0[1], D) + i0[0];
     F.dy = D;
     F.dz = D * D;
     F.fx = fr[0];
@@ -119,13 +135,17 @@ __device__ __forceinline__ Footprint footprint(const Pyramid& P, int m, const fl
 }
 
 template <bool kF32>
-__device__ __forceinline__ f4 trilinear(const void* __restrict__ data, const Footprint& F, unsigned long long face_off)
+__device__ __forceinline__ f4 trilinear(const void* __restrict__ data, const Footprint& F, uint32_t face_off)
 {
-    const unsigned long long b = F.base + face_off;
-    const f4 t000 = texel<kF32>(data, b), t100 = texel<kF32>(data, b + 1);
-    const f4 t010 = texel<kF32>(data, b + F.dy), t110 = texel<kF32>(data, b + F.dy + 1);
-    const f4 t001 = texel<kF32>(data, b + F.dz), t101 = texel<kF32>(data, b + F.dz + 1);
-    const f4 t011 = texel<kF32>(data, b + F.dz + F.dy), t111 = texel<kF32>(data, b + F.dz + F.dy + 1);
+    using T = typename std::conditional<kF32, float4, uint32_t>::type;
+    const T* r0 = (const T*)data + (F.base + face_off);  // rows (y, z), (y+1, z), (y, z+1), (y+1, z+1)
+    const T* r1 = r0 + F.dy;
+    const T* r2 = r0 + F.dz;
+    const T* r3 = r2 + F.dy;
+    const f4 t000 = texel<kF32>(r0), t100 = texel<kF32>(r0 + 1);
+    const f4 t010 = texel<kF32>(r1), t110 = texel<kF32>(r1 + 1);
+    const f4 t001 = texel<kF32>(r2), t101 = texel<kF32>(r2 + 1);
+    const f4 t011 = texel<kF32>(r3), t111 = texel<kF32>(r3 + 1);
     const float ax = F.fx, ay = F.fy, az = F.fz;
     const float w000 = (1.0f - ax) * (1.0f - ay) * (1.0f - az), w100 = ax * (1.0f - ay) * (1.0f - az);
     const float w010 = (1.0f - ax) * ay * (1.0f - az), w110 = ax * ay * (1.0f - az);
@@ -140,7 +160,7 @@ __device__ __forceinline__ f4 trilinear(const void* __restrict__ data, const Foo
     VCT_T(w);
 #undef VCT_T
     if (!kF32) {
-        const float s = 1.0f / 255.0f;
+        const float s = 16777216.0f / 255.0f;
         r.x *= s;
         r.y *= s;
         r.z *= s;
@@ -164,7 +184,7 @@ __device__ __forceinline__ f4 sample(const void* __restrict__ data, const Pyrami
     f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const f4 t = trilinear<kF32>(data, F, (unsigned long long)face[a] * P.block[m]);
+        const f4 t = trilinear<kF32>(data, F, (uint32_t)face[a] * (uint32_t)P.block[m]);
         acc.x += t.x * w[a];
         acc.y += t.y * w[a];
         acc.z += t.z * w[a];
@@ -202,19 +222,41 @@ __device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf
 __device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float hw_pow(float base, float c) { return hw_exp2(c * hw_log2(base)); }
 
-// Front-to-back accumulation of one sample (voxelConeTracing.frag:370-386).
-__device__ __forceinline__ void accumulate(f4& result, float& occlusion, const f4& s, float lvl, float step, float seg,
-                                           float vs, float decay)
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// The step's constants of accumulate(): the opacity correction seg / vs_l as
+// seg x (1 / vs) x 2^-l, and the reciprocal of the occlusion divisor
+// 1 + (t + vs_l) decay (hardware reciprocal, 1 ulp). The adjoint's replay
+// uses this function too, so forward and replay agree bit for bit.
+__device__ __forceinline__ void step_consts(float lvl, float step, float seg, float vs, float inv_vs, float decay,
+                                            float& corr, float& iq)
 {
     const float vsl = vs * hw_exp2(lvl);
-    const float corr = seg / vsl;
-    const float op = fminf(fmaxf(1.0f - hw_pow(fmaxf(1.0f - s.w, 0.0f), corr), 0.0f), 1.0f);  // 1 - a >= 0
+    corr = seg * (inv_vs * hw_exp2(-lvl));
+    iq = hw_rcp(1.0f + (step + vsl) * decay);
+}
+
+// Front-to-back accumulation of one sample (voxelConeTracing.frag:370-386)
+// with the step's constants; returns (1 - s.a)^corr for the adjoint.
+__device__ __forceinline__ float accumulate(f4& result, float& occlusion, const f4& s, float corr, float iq)
+{
+    const float pc = hw_pow(fmaxf(1.0f - s.w, 0.0f), corr);  // 1 - a >= 0
+    const float op = fminf(fmaxf(1.0f - pc, 0.0f), 1.0f);
     const float blend = fminf(fmaxf(1.0f - result.w, 0.0f), 1.0f);
     result.x += blend * (s.x * corr);
     result.y += blend * (s.y * corr);
     result.z += blend * (s.z * corr);
     result.w += blend * op;
-    occlusion += (1.0f - occlusion) * op / (1.0f + (step + vsl) * decay);
+    occlusion += (1.0f - occlusion) * op * iq;
+    return pc;
+}
+
+__device__ __forceinline__ void accumulate(f4& result, float& occlusion, const f4& s, float lvl, float step, float seg,
+                                           float vs, float inv_vs, float decay)
+{
+    float corr, iq;
+    step_consts(lvl, step, seg, vs, inv_vs, decay, corr, iq);
+    accumulate(result, occlusion, s, corr, iq);
 }
 
 // A cone's slot of the adjoint tape: the sample of accumulated step k goes
@@ -260,7 +302,7 @@ __device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March
         const float lvl = fminf(fmaxf(hw_log2(diameter * inv_vs), 0.0f), max_level);
         if (!kAhead) {
             const f4 s = level_sample<kF32>(data, P, pos, lvl, face, w);
-            accumulate(result, occlusion, s, lvl, step, seg, vs, M.decay);
+            accumulate(result, occlusion, s, lvl, step, seg, vs, inv_vs, M.decay);
             if (tape) tape_put(tape, s);
             const float prev = step;
             step += fmaxf(diameter, vs) * M.step_factor;
@@ -277,14 +319,14 @@ __device__ f4 march(const void* __restrict__ data, const Pyramid& P, const March
             const float lvl_b = fminf(fmaxf(hw_log2(diam_b * inv_vs), 0.0f), max_level);
             const f4 sa = level_sample<kF32>(data, P, pos, lvl, face, w);
             const f4 sb = level_sample<kF32>(data, P, pos_b, lvl_b, face, w);
-            accumulate(result, occlusion, sa, lvl, step, seg, vs, M.decay);
+            accumulate(result, occlusion, sa, lvl, step, seg, vs, inv_vs, M.decay);
             if (tape) tape_put(tape, sa);
             ++steps;
             step = step_b;
             seg = seg_b;
             diameter = diam_b;
             if (!(step < M.max_dist && occlusion < 1.0f)) break;
-            accumulate(result, occlusion, sb, lvl_b, step, seg, vs, M.decay);
+            accumulate(result, occlusion, sb, lvl_b, step, seg, vs, inv_vs, M.decay);
             if (tape) tape_put(tape, sb);
             ++steps;
             const float prev = step;
