@@ -469,6 +469,9 @@ inline vct_status make_params(const vct_dense* g, const vct_camera_ubo* cam, con
                               Pyramid* P, March* M)
 {
     if (g->nlevels > kMaxLevels) return fail(VCT_ERR_UNSUPPORTED, "too many levels");
+    const int last = g->nlevels - 1;
+    if (last < 0 || (unsigned long long)g->offset[last] + (last ? 6ull : 1ull) * g->block[last] > 0xffffffffull)
+        return fail(VCT_ERR_UNSUPPORTED, "the march indexes texels in 32 bits; this pyramid holds more");
     if (!(d->aperture > 0.0f) || !(d->max_distance > 0.0f))
         return fail(VCT_ERR_INVALID_ARGUMENT, "aperture and max_distance must be positive");
     for (int m = 0; m < g->nlevels; ++m) {
