@@ -71,18 +71,11 @@ __device__ __forceinline__ const float* cone_dir(const March& M, int i)
     return M.cones32 ? c_dense_dirs32[i] : c_dense_dirs16[i];
 }
 
-// A texel's channels: the floats of an RGBA32F grid, or the bytes of an
-// RGBA8 grid as half-float denormals, b x 2^-24 (exact): the half-to-float
-// conversion then happens inside the trilinear FMAs (v_fma_mix_f32), two
-// bit operations per texel instead of four byte conversions. trilinear()
-// applies 2^24 / 255 once to its result; scaling every product and partial
-// sum by 2^-24 is exact, and fl(2^24 / 255) = 2^24 fl(1 / 255), so the result
-// is bit for bit that of bytes 0..255 scaled by 1 / 255.
-__device__ __forceinline__ float half_bits(uint32_t x)
-{
-    return (float)__builtin_bit_cast(_Float16, (uint16_t)x);
-}
-
+// A texel's channels: the floats of an RGBA32F grid, or the bytes (0..255,
+// not yet divided by 255) of an RGBA8 grid; trilinear() applies the 1/255
+// once to its result instead of to each of the 8 texels. (Bytes passed as
+// half-float denormals to v_fma_mix_f32 take fewer instructions but ran 5%
+// slower: the byte conversions issue beside the FMAs, the mixed FMAs do not.)
 template <bool kF32>
 __device__ __forceinline__ f4 texel(const void* __restrict__ p)
 {
@@ -91,9 +84,7 @@ __device__ __forceinline__ f4 texel(const void* __restrict__ p)
         return {t.x, t.y, t.z, t.w};
     }
     const uint32_t t = *(const uint32_t*)p;
-    const uint32_t rb = t & 0x00ff00ffu;                         // R | B << 16
-    const uint32_t ga = __builtin_amdgcn_perm(t, t, 0x0c030c01u);  // G | A << 16
-    return {half_bits(rb), half_bits(ga), half_bits(rb >> 16), half_bits(ga >> 16)};
+    return {(float)(t & 0xffu), (float)((t >> 8) & 0xffu), (float)((t >> 16) & 0xffu), (float)(t >> 24)};
 }
 
 // Footprint of one trilinear fetch of level m (bordered indices). Texel
@@ -160,7 +151,7 @@ __device__ __forceinline__ f4 trilinear(const void* __restrict__ data, const Foo
     VCT_T(w);
 #undef VCT_T
     if (!kF32) {
-        const float s = 16777216.0f / 255.0f;
+        const float s = 1.0f / 255.0f;
         r.x *= s;
         r.y *= s;
         r.z *= s;
