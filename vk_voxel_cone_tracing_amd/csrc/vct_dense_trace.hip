@@ -14,8 +14,12 @@
 namespace vct {
 namespace dense {
 
+// Register budget of four waves per SIMD: a 512x512 view is 1024
+// workgroups, one round of four waves per SIMD, so a five-wave budget buys no
+// occupancy and only makes the scheduler keep fewer texel loads in flight
+// (64 views of 512x512: 28.8 -> 22.6 ms).
 template <bool kF32>
-__global__ __launch_bounds__(256) void k_dense_trace(const void* __restrict__ data, Pyramid P, March M,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_dense_trace(const void* __restrict__ data, Pyramid P, March M,
                                                      const uint32_t* __restrict__ g_diffuse,
                                                      const uint16_t* __restrict__ g_normal,
                                                      const float* __restrict__ g_depth, int width, int height,
