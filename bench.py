@@ -405,6 +405,13 @@ def main():
             valu = json.loads(vf.read_text()).get("summary")
         except Exception:
             valu = None
+    vmem = None
+    mf = ROOT / "profiles" / "trace_clipmap_1080p_vmem.json"
+    if mf.exists():
+        try:
+            vmem = json.loads(mf.read_text()).get("summary")
+        except Exception:
+            vmem = None
     roofline = {
         "bound": "valu",
         "kernel": "cone-trace pass: k_spec_bin + k_trace<half,split> + k_trace_specular<half>",
@@ -426,9 +433,10 @@ def main():
             "bytes_per_launch": gather_bytes,
             "achieved_gbs": round(gather_bytes / (t_trace * 1e-3) / 1e9, 1),
             "note": f"{B_SAMPLE_RGBA16F} B of texels per sample + {B_PIXEL} B per pixel, requested from the "
-                    f"cache hierarchy (mostly L2 hits), not HBM bytes",
+                    f"cache hierarchy (98.5% L1 hits), not HBM bytes",
         },
         "valu_issue": valu,
+        "vector_memory": vmem,
     }
     out = {
         "metric": METRIC,
