@@ -973,7 +973,9 @@ __global__ __launch_bounds__(256) void k_occ_bits(const T* __restrict__ vol, uin
     if (i < D3) {
         const size_t base = (size_t)level * kFaces * D3 + i;
 #pragma unroll
-        for (int f = 0; f < kFaces; ++f) nz = nz || texel_nonzero(vol[base + (size_t)f * D3]);
+        // a plain OR, not ||: the six loads are independent and all in flight
+        // (a short-circuit chain would wait for each face before the next)
+        for (int f = 0; f < kFaces; ++f) nz = nz | texel_nonzero(vol[base + (size_t)f * D3]);
     }
     const unsigned long long b = __ballot(nz);
     const uint32_t lane = threadIdx.x & 63u, w = i >> 5;
