@@ -401,7 +401,7 @@ vct_status vct_trace(vct_ctx* ctx, vct_stream stream, const vct_gbuffer* gbuffer
  * context scratch, and the radiance occupancy mask rebuilt on every call
  * (vct_trace rebuilds it only when the host saw the radiance volume change,
  * which a replayed graph would not). It makes no allocation; an image needing
- * more than 2048 specular tiles of 64x64 pixels (32 MiB; 3840x2160 needs 2040) returns
+ * more than 2048 specular tiles of 64x64 pixels (32.25 MiB; 3840x2160 needs 2040) returns
  * VCT_ERR_CAPACITY. Every GPU holds one such array for the whole process and
  * all calls on that GPU share it: calls that overlap on one GPU race, so the
  * caller serializes them (one stream, or events between streams). */
